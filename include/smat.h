@@ -1,0 +1,227 @@
+/*
+ * smat.h — C ABI of the MI355X-native SMaT library (libsmat.so).
+ *
+ * Two layers:
+ *
+ *  1. Reference drop-in launchers.  Same names, parameter order and meaning
+ *     as the reference's kernel launchers (src/cuda_hgemm/src/main.cu:10-77),
+ *     with `half*` spelled `uint16_t*` (IEEE binary16 bits) so the signatures
+ *     stay plain C.  Like the reference they return void, launch on the
+ *     default stream and are asynchronous; unlike the reference they never
+ *     exit the process: a failure is recorded and readable through
+ *     smat_last_error() / smat_last_status().
+ *
+ *  2. The native API (smat_*): explicit streams, status codes, a device
+ *     resident BCSR object built on the GPU from CSR, and SpMM plans that
+ *     hold the load-balanced work decomposition.
+ *
+ * All pointers are device pointers unless the parameter name ends in `_host`.
+ * Nothing here allocates except *_create / smat_bcsr_from_csr, and the
+ * matching *_destroy frees.  Calls on distinct objects and streams may run
+ * concurrently; one object must not be used from two threads at once.
+ */
+#ifndef SMAT_H_
+#define SMAT_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status and errors                                                        */
+/* ------------------------------------------------------------------------ */
+typedef int32_t smat_status;
+#define SMAT_OK 0
+#define SMAT_ERR_INVALID (-1)     /* bad argument: null pointer, size, option */
+#define SMAT_ERR_HIP (-2)         /* HIP runtime error; message has the detail */
+#define SMAT_ERR_ALLOC (-3)       /* device allocation failed */
+#define SMAT_ERR_NOT_24 (-4)      /* a block tagged 2:4 has a 1x4 group with >2 nonzeros */
+#define SMAT_ERR_UNSUPPORTED (-5) /* valid request this build does not implement */
+
+/* Thread-local message of the last failing call on this thread ("" if none). */
+const char* smat_last_error(void);
+/* Status of the last failing call on this thread (SMAT_OK if none). */
+smat_status smat_last_status(void);
+/* Clears the thread-local error state. */
+void smat_clear_error(void);
+const char* smat_version(void);
+
+/* hipStream_t, kept opaque so this header needs no HIP include. NULL = default stream. */
+typedef void* smat_stream;
+
+/* ------------------------------------------------------------------------ */
+/* Element types and layouts                                                */
+/* ------------------------------------------------------------------------ */
+#define SMAT_F16 1
+#define SMAT_BF16 2
+#define SMAT_F32 3
+
+/* Layout of the dense operand B (K x N). */
+#define SMAT_B_ROW_MAJOR 0   /* B[k*ldb + n] */
+#define SMAT_B_REFERENCE 1   /* B[n*ldb + k] — the reference kernels' layout (mmaOBT.cu:75) */
+
+/* Block tagging rules for CSR -> BCSR. */
+#define SMAT_TAG_REFERENCE 0 /* csrToBcsr heuristic, matrix.h:689-709 (CSR-order dependent) */
+#define SMAT_TAG_EXACT24 1   /* 1 iff every aligned 1x4 group of every row has <= 2 nonzeros */
+
+/* ------------------------------------------------------------------------ */
+/* BCSR object: 16 x bk blocks, built on the GPU                            */
+/* ------------------------------------------------------------------------ */
+typedef struct smat_bcsr_t* smat_bcsr;
+
+typedef struct {
+  int64_t M;          /* rows, multiple of 16 */
+  int64_t K;          /* columns, multiple of bk */
+  int32_t bm;         /* block rows (16) */
+  int32_t bk;         /* block columns: 16 or 32 */
+  int64_t nbrows;     /* M / 16 */
+  int64_t nnzb;       /* stored blocks */
+  int32_t* rowptr;    /* [nbrows + 1] */
+  int32_t* colidx;    /* [nnzb] first column of the block (not the block index), matrix.h:743 */
+  uint16_t* values;   /* [nnzb * 16 * bk] FP16, row-major inside each block */
+  int32_t* tags;      /* [nnzb] 1 = 2:4 block, 2 = dense block */
+} smat_bcsr_desc;
+
+/*
+ * CSR (int32 row pointers and columns, FP16 values, columns of a row in any
+ * order — mmio_allinone keeps file order, mmio_highlevel.h:655-661) -> BCSR.
+ * Bit-identical to SparseMatrix::csrToBcsr (matrix.h:666-792) for rowptr,
+ * colidx, values and, with SMAT_TAG_REFERENCE, the block tags.  M and K must
+ * already be padded (padSparseMatrix, mmio_highlevel.h:514-520).  Runs on
+ * `stream`; returns after the block count is known (one device->host copy).
+ * Duplicate (row, col) entries are not supported (the reference keeps the
+ * last one in CSR order; here which one is kept is unspecified).
+ */
+smat_status smat_bcsr_from_csr(int64_t M, int64_t K, int64_t nnz,
+                               const int32_t* csr_rowptr, const int32_t* csr_colidx,
+                               const uint16_t* csr_values, int32_t bk, int32_t tag_rule,
+                               smat_stream stream, smat_bcsr* out);
+smat_status smat_bcsr_describe(smat_bcsr A, smat_bcsr_desc* out);
+smat_status smat_bcsr_destroy(smat_bcsr A);
+
+/*
+ * Re-tag every block with the exact 2:4 rule (SMAT_TAG_EXACT24) in place.
+ */
+smat_status smat_bcsr_retag_exact24(smat_bcsr A, smat_stream stream);
+
+/*
+ * Dense-grid views the reference kernels take (matrix.h:672-720): blockInfo
+ * (0 empty, 1 2:4, 2 dense) and relativeBlockIndexMapping (-1 or the BCSR
+ * block index), each [nbrows * (K / bk)] int32, caller-allocated.
+ */
+smat_status smat_bcsr_grid_maps(const smat_bcsr_desc* A, int32_t* block_info,
+                                int32_t* relative_block_index, smat_stream stream);
+
+/*
+ * Split into the 2:4 list and the dense list (SparseMatrix::filterBcsrBlocks,
+ * matrix.h:1140-1288).  Counts first: s_nnzb_host / d_nnzb_host receive the
+ * list sizes when the list pointers are NULL; otherwise the caller passes
+ * buffers of at least those sizes and the lists are written.
+ */
+smat_status smat_bcsr_filter(const smat_bcsr_desc* A, int64_t* s_nnzb_host,
+                             int64_t* d_nnzb_host, int32_t* s_rowptr, int32_t* s_colidx,
+                             uint16_t* s_values, int32_t* d_rowptr, int32_t* d_colidx,
+                             uint16_t* d_values, smat_stream stream);
+
+/* ------------------------------------------------------------------------ */
+/* 2:4 compression                                                          */
+/* ------------------------------------------------------------------------ */
+/* Byte layouts of compressed 2:4 blocks. */
+#define SMAT_24_REFERENCE_K16 0 /* preprocessing_mmaSTKernelSparse, mmaST.cu:22-140:
+                                   values [blk][32 lanes][4] halves, metadata [blk][32] bytes */
+#define SMAT_24_SMFMAC 1        /* this library's SMFMAC operand layout: values
+                                   [blk][16 rows][bk/2] halves, index [blk][16 rows][bk/8] uint16 */
+
+/*
+ * Compress every block with tag 1 of a 16x16 (or 16x32 for SMAT_24_SMFMAC)
+ * BCSR.  Values and metadata arrays are indexed by BCSR block; blocks with
+ * tag 2 are written as zeros.  Returns SMAT_ERR_NOT_24 if a tag-1 block
+ * violates 2:4 (the reference overflows its slots in that case,
+ * mmaST.cu:103-107); the output is then still written with the extra
+ * nonzeros dropped.
+ */
+smat_status smat_compress_24(const smat_bcsr_desc* A, int32_t layout, uint16_t* values24,
+                             void* metadata, smat_stream stream);
+
+/* ------------------------------------------------------------------------ */
+/* SpMM: C = A * B, A BCSR (FP16), B dense K x N, FP32 accumulation         */
+/* ------------------------------------------------------------------------ */
+#define SMAT_KERNEL_AUTO 0
+#define SMAT_KERNEL_DENSE 1   /* every stored block on dense MFMA */
+#define SMAT_KERNEL_MIXED24 2 /* tag-1 blocks on SMFMAC (2:4), tag-2 blocks on MFMA */
+
+typedef struct {
+  int32_t kernel;          /* SMAT_KERNEL_* */
+  int32_t out_dtype;       /* SMAT_F32 or SMAT_F16 */
+  int32_t b_layout;        /* SMAT_B_ROW_MAJOR or SMAT_B_REFERENCE */
+  int32_t n_tile;          /* columns per wave: 16, 32 or 64; 0 = auto */
+  int32_t max_blocks_per_item; /* split block rows longer than this; 0 = auto */
+  int32_t reserved[3];
+} smat_spmm_opts;
+
+typedef struct smat_plan_t* smat_plan;
+
+/* Builds the work decomposition (on the GPU) and the scratch for split rows. */
+smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N,
+                                  const smat_spmm_opts* opts, smat_stream stream,
+                                  smat_plan* out);
+/*
+ * Runs the SpMM.  B: K x N (layout from the plan), leading dimension ldb;
+ * C: M x N row-major, leading dimension ldc, overwritten (beta = 0).
+ * For SMAT_KERNEL_MIXED24, values24/index24 are the SMAT_24_SMFMAC arrays of
+ * smat_compress_24 (NULL otherwise).
+ */
+smat_status smat_spmm_execute(smat_plan plan, const uint16_t* values24, const void* index24,
+                              const uint16_t* B, int64_t ldb, void* C, int64_t ldc,
+                              smat_stream stream);
+smat_status smat_spmm_plan_destroy(smat_plan plan);
+
+/* Number of work items and split rows of a plan (for reports and tests). */
+smat_status smat_spmm_plan_info(smat_plan plan, int64_t* n_items, int64_t* n_split_rows,
+                                int64_t* scratch_bytes);
+
+/* ------------------------------------------------------------------------ */
+/* Reference drop-in launchers (src/cuda_hgemm/src/main.cu:10-77)           */
+/* ------------------------------------------------------------------------ */
+/*
+ * HGEMM_FUNC_SPARSE2 (main.cu:25-28) — replaces mmaOBTKernel
+ * (src/cuda_hgemm/src/mma/mmaOBT.cu:160-170): plain dense-block BCSR SpMM,
+ * 16x16 blocks.  B[n*K + k] (the reference layout), C row-major M x N FP16.
+ * Block values are read through relativeBlockIndexMapping exactly as the
+ * reference does; blockInfo is ignored (as in mmaOBT.cu).  Accumulates in
+ * FP32 (the reference accumulates in FP16, ptx.h:20-25).
+ */
+void mmaOBTKernel(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdxA, uint16_t* B,
+                  uint16_t* C, size_t M, size_t N, size_t K, size_t nonzeroBlocks,
+                  int* blockInfo, int* relativeBlockIndexMapping);
+
+/*
+ * HGEMM_FUNC_SPARSE24_2 (main.cu:20-24) — replaces mmaOBTSKernel
+ * (src/cuda_hgemm/src/mma/mmaOBTS.cu:18-215): mixed 16x16 BCSR SpMM; blocks
+ * with blockInfo 2 are read from bcsrValuesA, blocks with blockInfo 1 from
+ * the compressed (sparseMatrixA, metadata) pair in SMAT_24_REFERENCE_K16
+ * layout.
+ */
+void mmaOBTSKernel(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdxA, char* metadata,
+                   uint16_t* sparseMatrixA, uint16_t* B, uint16_t* C, size_t M, size_t N,
+                   size_t K, size_t nonzeroBlocks, int* blockInfo,
+                   int* relativeBlockIndexMapping);
+
+/*
+ * Replaces preprocessing_mmaSTKernel (main.cu:61-64, mmaST.cu:316-326):
+ * compresses every blockInfo==1 block of a 16x16 BCSR into the reference
+ * k=16 layout.  K here is the column count of A (the reference passes the
+ * harness --K, tester.h:360-364, which is wrong whenever A is not K wide).
+ */
+void preprocessing_mmaSTKernel(uint16_t* bcsrValuesA, char* metadata, uint16_t* sparseMatrixA,
+                               size_t M, size_t N, size_t K, size_t nonzeroBlocks,
+                               int* blockInfo, int* relativeBlockIndexMapping);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SMAT_H_ */

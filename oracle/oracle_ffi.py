@@ -1,0 +1,269 @@
+"""ctypes binding of the CPU oracle (liboracle.so) and, where it was built,
+of the reference's own preprocessing code (_ref/libsmat_ref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py.  The product never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libsmat_ref.so")
+
+_p = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int32
+_f32 = ctypes.c_float
+
+_SIG = {
+    "or_h2f": (_f32, [ctypes.c_uint16]),
+    "or_f2h": (ctypes.c_uint16, [_f32]),
+    "or_csr_to_bcsr": (_i64, [_i64, _i64, ctypes.c_int, _p, _p, _p, _p, _p, _p, _p, _i64]),
+    "or_tag_exact24": (None, [_i64, ctypes.c_int, _p, _p]),
+    "or_filter_blocks": (None, [_i64, ctypes.c_int, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "or_compress24_k16": (_i64, [_i64, _p, _p, _p, _p]),
+    "or_decompress24_k16": (None, [_i64, _p, _p, _p]),
+    "or_spmm_bcsr_f64": (None, [_i64, ctypes.c_int, _p, _p, _p, _i64, _p, _i64, _p, _i64]),
+    "or_spmm_csr_f64": (None, [_i64, _p, _p, _p, _i64, _p, _i64, _p, _i64]),
+    "or_spmm_refsim_fp16acc": (None, [_i64, ctypes.c_int, _p, _p, _p, _i64, _p, _i64, _p, _i64]),
+    "or_csr_multiply_f32": (None, [_i64, _p, _p, _p, _p, _i64, _p]),
+    "or_jaccard_distance_group": (_f32, [_p, _i64, _i64, _p, _i64, _i64, _i64]),
+    "or_merge_rows": (_i64, [_p, _i64, _p, _i64, _p]),
+    "or_blocking_algo0": (None, [_i64, _p, _p, _f32, _i64, ctypes.c_int, _p, _p]),
+    "or_keeper": (None, [_i64, _p, _p, _f32, _i64, _i64, _p, _p]),
+    "or_get_permutation": (None, [_i64, _p, _p]),
+    "or_fixed_size_grouping": (None, [_i64, _p, _i64, _p]),
+    "or_collect_blocking_info": (None, [_i64, _i64, _p, _p, _p, _i64, _p, ctypes.POINTER(_f32)]),
+}
+
+_REF_SIG = {
+    "ref_jaccard_distance_group": (_f32, [_p, _i64, _i64, _p, _i64, _i64, _i64]),
+    "ref_merge_rows": (_i64, [_p, _i64, _p, _i64, _p]),
+    "ref_keeper": (ctypes.c_double, [_i64, _i64, _p, _p, _f32, _i64, _i64, _p, _p]),
+    "ref_algo0": (None, [_i64, _i64, _p, _p, _f32, _i64, ctypes.c_int, _p, _p]),
+    "ref_get_permutation": (None, [_i64, _p, _p]),
+    "ref_fixed_size_grouping": (None, [_i64, _p, _i64, _p]),
+    "ref_collect_blocking_info": (None, [_i64, _i64, _p, _p, _p, _i64, _p, ctypes.POINTER(_f32)]),
+    "ref_csr_multiply": (ctypes.c_double, [_i64, _i64, _p, _p, _p, _p, _i64, _p]),
+    "ref_reorder_rows": (None, [_i64, _i64, _p, _p, _p, _p, _p]),
+}
+
+_lib = None
+_ref = None
+
+
+def _load(path, sig):
+    lib = ctypes.CDLL(path)
+    for k, (r, a) in sig.items():
+        f = getattr(lib, k)
+        f.restype = r
+        f.argtypes = a
+    return lib
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        _lib = _load(ORACLE_SO, _SIG)
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        _ref = _load(REF_SO, _REF_SIG)
+    return _ref
+
+
+def P(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"], "arrays passed to the oracle must be contiguous"
+    return a.ctypes.data
+
+
+# ----------------------------------------------------------------- wrappers
+def csr_to_bcsr(M, K, bk, rowptr, colidx, values):
+    rowptr = np.ascontiguousarray(rowptr, np.int32)
+    colidx = np.ascontiguousarray(colidx, np.int32)
+    values = np.ascontiguousarray(values, np.uint16)
+    cap = max(int(colidx.size), 1)
+    o_rp = np.zeros(M // 16 + 1, np.int32)
+    o_ci = np.zeros(cap, np.int32)
+    o_v = np.zeros(cap * 16 * bk, np.uint16)
+    o_t = np.zeros(cap, np.int32)
+    nnzb = lib().or_csr_to_bcsr(M, K, bk, P(rowptr), P(colidx), P(values), P(o_rp), P(o_ci), P(o_v),
+                                P(o_t), cap)
+    assert nnzb >= 0
+    return o_rp, o_ci[:nnzb].copy(), o_v[: nnzb * 16 * bk].copy(), o_t[:nnzb].copy()
+
+
+def tag_exact24(bk, values):
+    values = np.ascontiguousarray(values, np.uint16)
+    nnzb = values.size // (16 * bk)
+    t = np.zeros(nnzb, np.int32)
+    lib().or_tag_exact24(nnzb, bk, P(values), P(t))
+    return t
+
+
+def filter_blocks(bk, rowptr, colidx, values, tags):
+    nbrows = rowptr.size - 1
+    ns, nd = int((tags == 1).sum()), int((tags == 2).sum())
+    s_rp, d_rp = np.zeros(nbrows + 1, np.int32), np.zeros(nbrows + 1, np.int32)
+    s_ci, d_ci = np.zeros(max(ns, 1), np.int32), np.zeros(max(nd, 1), np.int32)
+    s_v, d_v = np.zeros(max(ns, 1) * 16 * bk, np.uint16), np.zeros(max(nd, 1) * 16 * bk, np.uint16)
+    args = [np.ascontiguousarray(x) for x in (rowptr, colidx, values, tags)]
+    lib().or_filter_blocks(nbrows, bk, *(P(a) for a in args), P(s_rp), P(s_ci), P(s_v), P(d_rp), P(d_ci),
+                           P(d_v))
+    return (s_rp, s_ci[:ns], s_v[: ns * 256 * bk // 16]), (d_rp, d_ci[:nd], d_v[: nd * 256 * bk // 16])
+
+
+def compress24_k16(values, tags):
+    nnzb = tags.size
+    sp = np.zeros(max(nnzb, 1) * 128, np.uint16)
+    me = np.zeros(max(nnzb, 1) * 32, np.uint8)
+    ov = lib().or_compress24_k16(nnzb, P(np.ascontiguousarray(values, np.uint16)),
+                                 P(np.ascontiguousarray(tags, np.int32)), P(sp), P(me))
+    return sp[: nnzb * 128], me[: nnzb * 32], int(ov)
+
+
+def decompress24_k16(sparse, meta):
+    nnzb = meta.size // 32
+    v = np.zeros(max(nnzb, 1) * 256, np.uint16)
+    lib().or_decompress24_k16(nnzb, P(sparse), P(meta), P(v))
+    return v[: nnzb * 256]
+
+
+def spmm_bcsr_f64(bk, rowptr, colidx, values, B_u16):
+    nbrows = rowptr.size - 1
+    K, N = B_u16.shape
+    C = np.zeros((nbrows * 16, N), np.float64)
+    B_u16 = np.ascontiguousarray(B_u16, np.uint16)
+    lib().or_spmm_bcsr_f64(nbrows, bk, P(np.ascontiguousarray(rowptr, np.int32)),
+                           P(np.ascontiguousarray(colidx, np.int32)), P(np.ascontiguousarray(values, np.uint16)),
+                           N, P(B_u16), N, P(C), N)
+    return C
+
+
+def spmm_csr_f64(M, rowptr, colidx, values, B_u16):
+    K, N = B_u16.shape
+    C = np.zeros((M, N), np.float64)
+    B_u16 = np.ascontiguousarray(B_u16, np.uint16)
+    lib().or_spmm_csr_f64(M, P(np.ascontiguousarray(rowptr, np.int32)), P(np.ascontiguousarray(colidx, np.int32)),
+                          P(np.ascontiguousarray(values, np.uint16)), N, P(B_u16), N, P(C), N)
+    return C
+
+
+def spmm_refsim_fp16acc(bk, rowptr, colidx, values, B_u16):
+    nbrows = rowptr.size - 1
+    K, N = B_u16.shape
+    C = np.zeros((nbrows * 16, N), np.uint16)
+    B_u16 = np.ascontiguousarray(B_u16, np.uint16)
+    lib().or_spmm_refsim_fp16acc(nbrows, bk, P(np.ascontiguousarray(rowptr, np.int32)),
+                                 P(np.ascontiguousarray(colidx, np.int32)),
+                                 P(np.ascontiguousarray(values, np.uint16)), N, P(B_u16), N, P(C), N)
+    return C.view(np.float16)
+
+
+def csr_multiply_f32(rows, rowptr64, colidx64, vals32, B_colmajor, B_cols):
+    C = np.zeros(rows * B_cols, np.float32)
+    lib().or_csr_multiply_f32(rows, P(rowptr64), P(colidx64), P(vals32), P(B_colmajor), B_cols, P(C))
+    return C
+
+
+def jaccard(a, ga, b, gb, block, use_ref=False):
+    a = np.ascontiguousarray(a, np.int64)
+    b = np.ascontiguousarray(b, np.int64)
+    f = ref().ref_jaccard_distance_group if use_ref else lib().or_jaccard_distance_group
+    return float(np.float32(f(P(a) if a.size else None, a.size, ga, P(b) if b.size else None, b.size, gb, block)))
+
+
+def merge_rows(a, b, use_ref=False):
+    a = np.ascontiguousarray(a, np.int64)
+    b = np.ascontiguousarray(b, np.int64)
+    out = np.zeros(a.size + b.size + 1, np.int64)
+    f = ref().ref_merge_rows if use_ref else lib().or_merge_rows
+    n = f(P(a) if a.size else None, a.size, P(b) if b.size else None, b.size, P(out))
+    return out[:n].copy()
+
+
+def keeper(rowptr64, colidx64, tau, col_block, max_rows, cols=None, use_ref=False):
+    rows = rowptr64.size - 1
+    rowptr64 = np.ascontiguousarray(rowptr64, np.int64)
+    colidx64 = np.ascontiguousarray(colidx64, np.int64) if colidx64.size else np.zeros(1, np.int64)
+    g = np.zeros(rows, np.int64)
+    cnt = np.zeros(2, np.int64)
+    if use_ref:
+        cols = cols if cols is not None else int(colidx64.max()) + 1
+        t = ref().ref_keeper(rows, cols, P(rowptr64), P(colidx64), tau, col_block, max_rows, P(g), P(cnt))
+        return g, cnt, t
+    lib().or_keeper(rows, P(rowptr64), P(colidx64), tau, col_block, max_rows, P(g), P(cnt))
+    return g, cnt
+
+
+def algo0(rowptr64, colidx64, tau, block, use_size=0, cols=None, use_ref=False):
+    rows = rowptr64.size - 1
+    g = np.zeros(rows, np.int64)
+    cnt = np.zeros(2, np.int64)
+    rowptr64 = np.ascontiguousarray(rowptr64, np.int64)
+    colidx64 = np.ascontiguousarray(colidx64, np.int64)
+    if use_ref:
+        cols = cols if cols is not None else int(colidx64.max()) + 1
+        ref().ref_algo0(rows, cols, P(rowptr64), P(colidx64), tau, block, use_size, P(g), P(cnt))
+    else:
+        lib().or_blocking_algo0(rows, P(rowptr64), P(colidx64), tau, block, use_size, P(g), P(cnt))
+    return g, cnt
+
+
+def get_permutation(grouping, use_ref=False):
+    g = np.ascontiguousarray(grouping, np.int64)
+    out = np.zeros(g.size, np.int64)
+    (ref().ref_get_permutation if use_ref else lib().or_get_permutation)(g.size, P(g), P(out))
+    return out
+
+
+def fixed_size_grouping(grouping, row_block, use_ref=False):
+    g = np.ascontiguousarray(grouping, np.int64)
+    out = np.zeros(g.size, np.int64)
+    (ref().ref_fixed_size_grouping if use_ref else lib().or_fixed_size_grouping)(g.size, P(g), row_block, P(out))
+    return out
+
+
+def collect_blocking_info(rowptr64, colidx64, grouping, cols, col_block, use_ref=False):
+    rows = rowptr64.size - 1
+    out = np.zeros(3, np.int64)
+    avg = _f32(0)
+    args = [np.ascontiguousarray(x, np.int64) for x in (rowptr64, colidx64, grouping)]
+    f = ref().ref_collect_blocking_info if use_ref else lib().or_collect_blocking_info
+    f(rows, cols, P(args[0]), P(args[1]), P(args[2]), col_block, P(out), ctypes.byref(avg))
+    return out, float(avg.value)
+
+
+def read_edgelist(text: str, weighted: bool = True):
+    """Edge-list reader with CSR::read_from_edgelist conventions
+    (preprocess/src/general/csr.cpp:166-290, el format): rows must ascend,
+    columns kept in file order, rows = max row + 1, cols = max col + 1."""
+    rows, cols, vals = [], [], []
+    for line in text.splitlines():
+        if not line.strip() or line[0] in "#%":
+            continue
+        parts = line.split(" ")
+        rows.append(int(parts[0]))
+        cols.append(int(parts[1]))
+        vals.append(float(parts[2]) if weighted and len(parts) > 2 else 1.0)
+    r = np.array(rows, np.int64)
+    assert np.all(np.diff(r) >= 0), "rows must be in ascending order"
+    nrows = int(r.max()) + 1
+    rowptr = np.zeros(nrows + 1, np.int64)
+    np.add.at(rowptr, r + 1, 1)
+    rowptr = np.cumsum(rowptr)
+    return nrows, int(max(cols)) + 1, rowptr, np.array(cols, np.int64), np.array(vals, np.float32)

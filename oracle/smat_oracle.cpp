@@ -1,0 +1,608 @@
+// SMaT CPU oracle — TEST INFRASTRUCTURE ONLY.
+//
+// Plain, serial C++ restatements of the reference SMaT algorithms on the SpMM
+// hot path. Only the test-suite, the smoke check and the CPU-baseline leg of
+// the benchmark may load this library; the product (libsmat.so) never links
+// or calls it.
+//
+// Each function cites the reference file:line it restates. Paths are relative
+// to the reference checkout root. Integer outputs are meant to be bit-exact
+// with the reference; floating-point outputs are the FP64 ground truth, or an
+// emulation of the reference's FP16-accumulate tensor-core arithmetic.
+//
+// Pinning (how each function is checked, beyond restating the formula):
+//   or_jaccard_distance_group  closed form 6/12 on the reference's
+//                              TEST_similarities inputs; compiled-reference vectors
+//   or_merge_rows              hand-derived truncation cases; compiled-reference vectors
+//   or_blocking_algo0          the reference's TEST_results KAT (9x9, 13 nnz)
+//   or_keeper                  compiled-reference groupings + counters (fixtures)
+//   or_get_permutation,
+//   or_fixed_size_grouping     compiled-reference vectors; permutation invariants
+//   or_collect_blocking_info   KAT (VBR_nzcount 36, nzblocks 5, longest 3)
+//   or_csr_to_bcsr             invariants (every nnz at its offset, value sums,
+//                              monotone rowptr), hand cases of the tag rule
+//   or_filter_blocks           partition invariants vs or_csr_to_bcsr
+//   or_compress24_k16          round trip on exact-2:4 blocks, hand-built bytes
+//   or_spmm_*_f64              dense brute-force GEMM, all-ones closed form,
+//                              identity / permutation / linearity properties
+//   or_spmm_refsim_fp16acc     parity unpinned against NVIDIA tensor cores (no
+//                              reference test holds their outputs); pinned only
+//                              against FP64 within FP16-accumulation error
+//   or_csr_multiply_f32        dense brute force (FP32)
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <set>
+#include <utility>
+#include <vector>
+
+typedef long intT;  // preprocess/include/definitions.h:4
+
+namespace {
+
+// IEEE binary16 <-> float, round-to-nearest-even; used to keep the oracle
+// independent of compiler _Float16 support.
+float h2f(uint16_t h) {
+  uint32_t s = (uint32_t)(h >> 15) << 31;
+  uint32_t e = (h >> 10) & 0x1f;
+  uint32_t m = h & 0x3ff;
+  uint32_t f;
+  if (e == 0) {
+    if (m == 0) {
+      f = s;
+    } else {  // subnormal
+      int shift = 0;
+      while ((m & 0x400) == 0) { m <<= 1; shift++; }
+      m &= 0x3ff;
+      f = s | ((uint32_t)(127 - 15 - shift + 1) << 23) | (m << 13);
+    }
+  } else if (e == 31) {
+    f = s | 0x7f800000u | (m << 13);
+  } else {
+    f = s | ((e + 127 - 15) << 23) | (m << 13);
+  }
+  float out;
+  std::memcpy(&out, &f, 4);
+  return out;
+}
+
+uint16_t f2h(float x) {
+  uint32_t f;
+  std::memcpy(&f, &x, 4);
+  uint32_t s = (f >> 16) & 0x8000;
+  int32_t e = (int32_t)((f >> 23) & 0xff) - 127 + 15;
+  uint32_t m = f & 0x7fffff;
+  if (((f >> 23) & 0xff) == 0xff) return (uint16_t)(s | 0x7c00 | (m ? 0x200 : 0));
+  if (e >= 31) return (uint16_t)(s | 0x7c00);
+  if (e <= 0) {
+    if (e < -10) return (uint16_t)s;
+    m |= 0x800000;
+    uint32_t shift = (uint32_t)(14 - e);
+    uint32_t hm = m >> shift;
+    uint32_t rem = m & ((1u << shift) - 1);
+    uint32_t half = 1u << (shift - 1);
+    if (rem > half || (rem == half && (hm & 1))) hm++;
+    return (uint16_t)(s | hm);
+  }
+  uint32_t hm = m >> 13;
+  uint32_t rem = m & 0x1fff;
+  uint16_t out = (uint16_t)(s | ((uint32_t)e << 10) | hm);
+  if (rem > 0x1000 || (rem == 0x1000 && (hm & 1))) out++;
+  return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+float or_h2f(uint16_t h) { return h2f(h); }
+uint16_t or_f2h(float x) { return f2h(x); }
+
+// ---------------------------------------------------------------------------
+// CSR -> BCSR.  Restates SparseMatrix::csrToBcsr
+// (src/cuda_hgemm/src/common/matrix.h:666-792) for blocks of 16 x bk.
+// Inputs follow mmio_allinone conventions (mmio_highlevel.h:523-680): M and K
+// already padded, columns of a row in file order (not necessarily sorted).
+// Outputs, in BCSR (row-major block) order:
+//   out_rowptr[M/16+1], out_colidx[nnzb] (block column START, matrix.h:743),
+//   out_val[nnzb*16*bk] (row-major inside a block, matrix.h:757-775),
+//   out_tag[nnzb]: 1 = "2:4" and 2 = dense under the reference rule
+//   (matrix.h:689-709), i.e. blockInfo gathered in BCSR order.
+// Returns nnzb, or -1 if cap is too small.
+// ---------------------------------------------------------------------------
+int64_t or_csr_to_bcsr(int64_t M, int64_t K, int bk, const int32_t* rowptr,
+                       const int32_t* colidx, const uint16_t* val,
+                       int32_t* out_rowptr, int32_t* out_colidx,
+                       uint16_t* out_val, int32_t* out_tag, int64_t cap) {
+  const int bm = 16;
+  int64_t ncr = (K + bk - 1) / bk;
+  int64_t nrr = (M + bm - 1) / bm;
+  std::vector<int32_t> info((size_t)(nrr * ncr), 0);
+  int64_t nnzb = 0;
+  for (int64_t row = 0; row < M; row++) {
+    for (int64_t j = rowptr[row]; j < rowptr[row + 1]; j++) {
+      int64_t col = colidx[j];
+      int64_t bi = (row / bm) * ncr + col / bk;
+      if (info[bi] == 0) {
+        info[bi] = 1;
+        nnzb++;
+      } else if (info[bi] == 1) {
+        int64_t r4 = col % 4;
+        if (r4 == 2 || r4 == 3) {
+          if (j >= rowptr[row] + 2 && colidx[j - 1] == col - 1 &&
+              colidx[j - 2] == col - 2)
+            info[bi] = 2;
+        }
+      }
+    }
+  }
+  if (nnzb > cap) return -1;
+  std::vector<int64_t> rel((size_t)(nrr * ncr), -1);
+  int64_t r = 0;
+  for (int64_t i = 0; i < nrr * ncr; i++)
+    if (info[i] != 0) rel[i] = r++;
+  int64_t nb = 0;
+  for (int64_t row = 0; row < M; row += bm) {
+    out_rowptr[row / bm] = (int32_t)nb;
+    for (int64_t col = 0; col < K; col += bk) {
+      int64_t cur = (row / bm) * ncr + col / bk;
+      if (info[cur] == 0) continue;
+      out_colidx[nb] = (int32_t)col;
+      out_tag[nb] = info[cur];
+      nb++;
+    }
+  }
+  out_rowptr[M / bm] = (int32_t)nb;
+  std::memset(out_val, 0, sizeof(uint16_t) * (size_t)(nnzb * bm * bk));
+  for (int64_t row = 0; row < M; row++) {
+    for (int64_t j = rowptr[row]; j < rowptr[row + 1]; j++) {
+      int64_t col = colidx[j];
+      int64_t bi = (row / bm) * ncr + col / bk;
+      int64_t off = (row % bm) * bk + col % bk;
+      out_val[rel[bi] * bm * bk + off] = val[j];
+    }
+  }
+  return nnzb;
+}
+
+// Exact 2:4 classification of BCSR blocks: tag 1 iff every aligned 1x4 group
+// of every row holds at most two nonzero values, else 2.  This is the
+// correctness rule that the reference's heuristic (matrix.h:689-709)
+// approximates; its compressor (mmaST.cu:98-111) silently overflows when a
+// group holds more than two nonzeros.
+void or_tag_exact24(int64_t nnzb, int bk, const uint16_t* val, int32_t* tag) {
+  for (int64_t b = 0; b < nnzb; b++) {
+    int t = 1;
+    for (int r = 0; r < 16 && t == 1; r++)
+      for (int g = 0; g < bk / 4 && t == 1; g++) {
+        int c = 0;
+        for (int i = 0; i < 4; i++)
+          if ((val[b * 16 * bk + r * bk + g * 4 + i] & 0x7fff) != 0) c++;
+        if (c > 2) t = 2;
+      }
+    tag[b] = t;
+  }
+}
+
+// Split a tagged BCSR into the 2:4 list and the dense list, each with its own
+// rowptr/colidx/values.  Restates SparseMatrix::filterBcsrBlocks
+// (matrix.h:1140-1288); the order inside each list is the BCSR order.
+void or_filter_blocks(int64_t nbrows, int bk, const int32_t* rowptr,
+                      const int32_t* colidx, const uint16_t* val,
+                      const int32_t* tag, int32_t* s_rowptr, int32_t* s_colidx,
+                      uint16_t* s_val, int32_t* d_rowptr, int32_t* d_colidx,
+                      uint16_t* d_val) {
+  const int64_t bsz = 16 * (int64_t)bk;
+  s_rowptr[0] = 0;
+  d_rowptr[0] = 0;
+  int64_t ns = 0, nd = 0;
+  for (int64_t r = 0; r < nbrows; r++) {
+    for (int64_t p = rowptr[r]; p < rowptr[r + 1]; p++) {
+      if (tag[p] == 1) {
+        s_colidx[ns] = colidx[p];
+        std::memcpy(s_val + ns * bsz, val + p * bsz, sizeof(uint16_t) * bsz);
+        ns++;
+      } else if (tag[p] == 2) {
+        d_colidx[nd] = colidx[p];
+        std::memcpy(d_val + nd * bsz, val + p * bsz, sizeof(uint16_t) * bsz);
+        nd++;
+      }
+    }
+    s_rowptr[r + 1] = (int32_t)ns;
+    d_rowptr[r + 1] = (int32_t)nd;
+  }
+}
+
+// 2:4 compression of 16x16 blocks in the reference byte layout.  Restates
+// preprocessing_mmaSTKernelSparse (src/cuda_hgemm/src/mma/mmaST.cu:22-140):
+// thread `lane` (0..31) of a block handles row lane/2, columns
+// (lane%2)*8 .. +8, i.e. two 4-groups.  Within a group the first two nonzero
+// values are kept in order (slot 0, slot 1; missing slots are zero).  The
+// metadata byte holds the 2-bit positions: group 0 slot 0 -> bits 4-5, slot 1
+// -> bits 6-7, group 1 slot 0 -> bits 0-1, slot 1 -> bits 2-3 (mmaST.cu:106).
+// Values: sparse[b*128 + lane*4 + 0..3], metadata: meta[b*32 + lane].
+// Blocks with tag != 1 are left zero.  Returns the number of 4-groups that
+// held more than two nonzeros (the reference overflows on those; here the
+// extra values are dropped and counted).
+int64_t or_compress24_k16(int64_t nnzb, const uint16_t* val, const int32_t* tag,
+                          uint16_t* sparse, uint8_t* meta) {
+  int64_t overflow = 0;
+  std::memset(sparse, 0, sizeof(uint16_t) * (size_t)(nnzb * 128));
+  std::memset(meta, 0, (size_t)(nnzb * 32));
+  for (int64_t b = 0; b < nnzb; b++) {
+    if (tag[b] != 1) continue;
+    for (int lane = 0; lane < 32; lane++) {
+      const uint16_t* src = val + b * 256 + (lane / 2) * 16 + (lane % 2) * 8;
+      uint16_t out[4] = {0, 0, 0, 0};
+      uint8_t m = 0;
+      for (int j = 0; j < 2; j++) {
+        int cur = 0;
+        for (int i = 0; i < 4; i++) {
+          if ((src[i + 4 * j] & 0x7fff) != 0) {
+            if (cur < 2) {
+              out[cur + 2 * j] = src[i + 4 * j];
+              m |= (uint8_t)(i << (6 - (2 * (1 - cur) + 4 * j)));
+            } else {
+              overflow++;
+            }
+            cur++;
+          }
+        }
+      }
+      std::memcpy(sparse + b * 128 + lane * 4, out, sizeof(out));
+      meta[b * 32 + lane] = m;
+    }
+  }
+  return overflow;
+}
+
+// Inverse of or_compress24_k16: rebuild the dense 16x16 blocks.
+void or_decompress24_k16(int64_t nnzb, const uint16_t* sparse,
+                         const uint8_t* meta, uint16_t* val) {
+  std::memset(val, 0, sizeof(uint16_t) * (size_t)(nnzb * 256));
+  for (int64_t b = 0; b < nnzb; b++)
+    for (int lane = 0; lane < 32; lane++) {
+      uint8_t m = meta[b * 32 + lane];
+      uint16_t* dst = val + b * 256 + (lane / 2) * 16 + (lane % 2) * 8;
+      for (int j = 0; j < 2; j++)
+        for (int cur = 0; cur < 2; cur++) {
+          int shift = 6 - (2 * (1 - cur) + 4 * j);
+          int pos = (m >> shift) & 3;
+          uint16_t v = sparse[b * 128 + lane * 4 + cur + 2 * j];
+          if ((v & 0x7fff) != 0) dst[4 * j + pos] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SpMM oracles.  A is BCSR (16 x bk blocks, FP16 bits), B is row-major K x N
+// FP16 bits with leading dimension ldb, C is row-major M x N.
+// ---------------------------------------------------------------------------
+void or_spmm_bcsr_f64(int64_t nbrows, int bk, const int32_t* rowptr,
+                      const int32_t* colidx, const uint16_t* val, int64_t N,
+                      const uint16_t* B, int64_t ldb, double* C, int64_t ldc) {
+  std::vector<double> acc((size_t)N);
+  for (int64_t br = 0; br < nbrows; br++)
+    for (int r = 0; r < 16; r++) {
+      std::fill(acc.begin(), acc.end(), 0.0);
+      for (int64_t p = rowptr[br]; p < rowptr[br + 1]; p++)
+        for (int c = 0; c < bk; c++) {
+          uint16_t a = val[p * 16 * bk + r * bk + c];
+          if ((a & 0x7fff) == 0) continue;
+          double av = h2f(a);
+          const uint16_t* brow = B + (int64_t)(colidx[p] + c) * ldb;
+          for (int64_t n = 0; n < N; n++) acc[n] += av * (double)h2f(brow[n]);
+        }
+      double* crow = C + (br * 16 + r) * ldc;
+      for (int64_t n = 0; n < N; n++) crow[n] = acc[n];
+    }
+}
+
+void or_spmm_csr_f64(int64_t M, const int32_t* rowptr, const int32_t* colidx,
+                     const uint16_t* val, int64_t N, const uint16_t* B,
+                     int64_t ldb, double* C, int64_t ldc) {
+  for (int64_t i = 0; i < M; i++) {
+    double* crow = C + i * ldc;
+    for (int64_t n = 0; n < N; n++) crow[n] = 0.0;
+    for (int64_t j = rowptr[i]; j < rowptr[i + 1]; j++) {
+      double av = h2f(val[j]);
+      const uint16_t* brow = B + (int64_t)colidx[j] * ldb;
+      for (int64_t n = 0; n < N; n++) crow[n] += av * (double)h2f(brow[n]);
+    }
+  }
+}
+
+// Emulation of the reference plain kernel's arithmetic
+// (mmaOBT.cu:87-147 with HMMA16816 FP16 accumulate, ptx.h:20-25): per C row
+// and column, blocks are visited in BCSR order; each 16-wide k-step product
+// is formed in FP32 and added to the FP16 accumulator, rounding to FP16 after
+// each step.  The hardware's internal accumulation order inside one mma is not
+// published, so this is an emulation, compared with a tolerance.
+void or_spmm_refsim_fp16acc(int64_t nbrows, int bk, const int32_t* rowptr,
+                            const int32_t* colidx, const uint16_t* val,
+                            int64_t N, const uint16_t* B, int64_t ldb,
+                            uint16_t* C, int64_t ldc) {
+  for (int64_t br = 0; br < nbrows; br++)
+    for (int r = 0; r < 16; r++)
+      for (int64_t n = 0; n < N; n++) {
+        uint16_t acc = 0;
+        for (int64_t p = rowptr[br]; p < rowptr[br + 1]; p++)
+          for (int ks = 0; ks < bk; ks += 16) {
+            float s = 0.f;
+            for (int c = ks; c < ks + 16 && c < bk; c++)
+              s += h2f(val[p * 16 * bk + r * bk + c]) *
+                   h2f(B[(int64_t)(colidx[p] + c) * ldb + n]);
+            acc = f2h(h2f(acc) + s);
+          }
+        C[(br * 16 + r) * ldc + n] = acc;
+      }
+}
+
+// CSR::multiply (preprocess/src/general/csr.cpp:47-63): serial, float,
+// B and C column-major with leading dimension `rows` (the reference indexes B
+// with `rows`, which is only right for square A).  C is accumulated into.
+void or_csr_multiply_f32(int64_t rows, const int64_t* rowptr,
+                         const int64_t* colidx, const float* val,
+                         const float* B, int64_t B_cols, float* C) {
+  for (int64_t i = 0; i < rows; i++)
+    for (int64_t nz = rowptr[i]; nz < rowptr[i + 1]; nz++) {
+      int64_t col = colidx[nz];
+      float v = val[nz];
+      for (int64_t j = 0; j < B_cols; j++) C[i + j * rows] += v * B[col + j * rows];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reordering (SPARTA "Keeper" Jaccard grouping).
+// ---------------------------------------------------------------------------
+
+// JaccardDistanceGroup (preprocess/src/general/blocking.cpp:904-975), with
+// count_zeros = 1.  Rows are sorted column lists; distance is computed over
+// column blocks of width block_size.
+float or_jaccard_distance_group(const intT* row_A, intT size_A,
+                                intT group_size_A, const intT* row_B,
+                                intT size_B, intT group_size_B,
+                                intT block_size) {
+  if (size_A == 0 && size_B == 0) return 0;
+  if (size_A == 0 || size_B == 0) return 1;
+  intT add_A = group_size_B, add_B = group_size_A;
+  intT i = 0, j = 0, count = 0, nA = 0, nB = 0;
+  while (i < size_A && j < size_B) {
+    intT pa = row_A[i] / block_size, pb = row_B[j] / block_size;
+    if (pa < pb) {
+      count += add_A;
+      nA++;
+      while (i < size_A && row_A[i] / block_size == pa) i++;
+    } else if (pa > pb) {
+      count += add_B;
+      nB++;
+      while (j < size_B && row_B[j] / block_size == pb) j++;
+    } else {
+      nA++;
+      nB++;
+      while (i < size_A && row_A[i] / block_size == pa) i++;
+      while (j < size_B && row_B[j] / block_size == pb) j++;
+    }
+  }
+  while (i < size_A) {
+    intT pa = row_A[i] / block_size;
+    count += add_A;
+    nA++;
+    while (i < size_A && row_A[i] / block_size == pa) i++;
+  }
+  while (j < size_B) {
+    intT pb = row_B[j] / block_size;
+    count += add_B;
+    nB++;
+    while (j < size_B && row_B[j] / block_size == pb) j++;
+  }
+  return (float)((2.0 * count) / (nA * group_size_A + nB * group_size_B + count));
+}
+
+// merge_rows (preprocess/src/general/utilities.cpp:145-173).  Not a plain
+// union: when lower_bound reaches the end of A the loop stops, dropping the
+// A elements not yet copied, and then B's tail is appended; A elements beyond
+// the last B element are also dropped.  Returns the output length.
+intT or_merge_rows(const intT* A, intT size_A, const intT* B, intT size_B,
+                   intT* out) {
+  intT n = 0;
+  const intT* i = A;
+  const intT* end = A + size_A;
+  intT j = 0;
+  while (j < size_B) {
+    intT bv = B[j];
+    const intT* ni = std::lower_bound(i, end, bv);
+    if (ni == end) break;
+    for (const intT* p = i; p < ni; p++) out[n++] = *p;
+    out[n++] = bv;
+    if (*ni == bv) ni++;
+    i = ni;
+    j++;
+  }
+  for (intT q = j; q < size_B; q++) out[n++] = B[q];
+  return n;
+}
+
+static std::vector<intT> merge_vec(const std::vector<intT>& A, const intT* B,
+                                   intT nB) {
+  std::vector<intT> out((size_t)(A.size() + nB));
+  intT n = or_merge_rows(A.data(), (intT)A.size(), B, nB, out.data());
+  out.resize((size_t)n);
+  return out;
+}
+
+// IterativeBlockingPattern, algorithm 0 (blocking.cpp:87-152): strict
+// `dist < tau`, no pruning, group size fixed at 1 unless use_size.  The
+// pattern is updated on every merge (the `if (use_pattern)` at
+// blocking.cpp:131-133 guards only a timer macro).
+void or_blocking_algo0(intT rows, const intT* rowptr, const intT* colidx,
+                       float tau, intT block_size, int use_size,
+                       intT* grouping, intT* counters /* [comparisons, merges] */) {
+  intT comparisons = 0, merges = 0;
+  for (intT i = 0; i < rows; i++) grouping[i] = -1;
+  for (intT i = 0; i < rows; i++) {
+    if (grouping[i] != -1) continue;
+    intT gsize = 1;
+    grouping[i] = i;
+    std::vector<intT> pattern(colidx + rowptr[i], colidx + rowptr[i + 1]);
+    for (intT j = i + 1; j < rows; j++) {
+      if (grouping[j] != -1) continue;
+      comparisons++;
+      float d = or_jaccard_distance_group(pattern.data(), (intT)pattern.size(),
+                                          gsize, colidx + rowptr[j],
+                                          rowptr[j + 1] - rowptr[j], 1, block_size);
+      if (d < tau) {
+        merges++;
+        grouping[j] = i;
+        pattern = merge_vec(pattern, colidx + rowptr[j], rowptr[j + 1] - rowptr[j]);
+        if (use_size) gsize++;
+      }
+    }
+  }
+  counters[0] = comparisons;
+  counters[1] = merges;
+}
+
+// IterativeBlockingKeeper, algorithm 5 (blocking.cpp:431-547), the grouping
+// used by the SMaT reorder pipeline (reorder.py:131 runs `-a 5`).
+//   * distances[] starts as {-1, 0, 0, ...}: g++'s initialisation of the VLA
+//     `float distances[rows] = {-1}` (blocking.cpp:434) sets element 0 only.
+//   * a candidate j is skipped when distances[i] != -1, distances[j] != -1 and
+//     |distances[i] - distances[j]| > tau (float abs), and then distances[j]
+//     is set to -1 (blocking.cpp:467-471), whether or not j is grouped.
+//   * merge when dist <= tau; group full at max_rows stops the scan.
+//   * non-merged compared candidates are kept as a (dist, j)-ordered set
+//     truncated to (max_rows - merged) after each insert; after the scan the
+//     group is topped up from the front of that set (blocking.cpp:501-531).
+//   * complete groups are numbered by their seed i, incomplete ones i + rows.
+// counters = [comparison_counter, merge_counter].
+void or_keeper(intT rows, const intT* rowptr, const intT* colidx, float tau,
+               intT col_block, intT max_rows, intT* grouping, intT* counters) {
+  intT comparisons = 0, merges = 0;
+  std::vector<float> distances((size_t)rows, 0.0f);
+  if (rows > 0) distances[0] = -1.0f;
+  for (intT i = 0; i < rows; i++) grouping[i] = -1;
+  for (intT i = 0; i < rows; i++) {
+    if (grouping[i] != -1) continue;
+    std::set<std::pair<float, intT>> best;
+    std::vector<intT> merged;
+    intT group = i + rows;
+    std::vector<intT> pattern(colidx + rowptr[i], colidx + rowptr[i + 1]);
+    intT gsize = 1;
+    grouping[i] = group;
+    merged.push_back(i);
+    for (intT j = i + 1; j < rows; j++) {
+      if (gsize == max_rows) break;
+      if (distances[i] != -1 && distances[j] != -1 &&
+          std::fabs(distances[i] - distances[j]) > tau) {
+        distances[j] = -1;
+        continue;
+      }
+      if (grouping[j] != -1) continue;
+      comparisons++;
+      float d = or_jaccard_distance_group(pattern.data(), (intT)pattern.size(),
+                                          gsize, colidx + rowptr[j],
+                                          rowptr[j + 1] - rowptr[j], 1, col_block);
+      distances[j] = d;
+      if (d <= tau) {
+        merges++;
+        grouping[j] = group;
+        merged.push_back(j);
+        pattern = merge_vec(pattern, colidx + rowptr[j], rowptr[j + 1] - rowptr[j]);
+        gsize++;
+      } else {
+        best.insert(std::make_pair(d, j));
+        size_t keep = (size_t)(max_rows - (intT)merged.size());
+        if (best.size() > keep) {
+          // blocking.cpp:505-507 advances from end(), not begin().  Under
+          // libstdc++'s red-black tree, ++end() lands on the rightmost node's
+          // left child if it has one, else on the rightmost node, and from
+          // there the walk returns to end(); so this erases 0, 1 or 2 of the
+          // largest elements depending on the tree shape.  The same
+          // operation on the same library reproduces it exactly.
+          auto it = best.end();
+          std::advance(it, keep);
+          best.erase(it, best.end());
+        }
+      }
+    }
+    if (gsize < max_rows) {
+      for (auto it = best.begin(); it != best.end() && gsize != max_rows; ++it) {
+        grouping[it->second] = group;
+        merged.push_back(it->second);
+        gsize++;
+      }
+    }
+    if (gsize == max_rows)
+      for (intT idx : merged) grouping[idx] -= rows;
+  }
+  counters[0] = comparisons;
+  counters[1] = merges;
+}
+
+// get_permutation (preprocess/src/general/utilities.cpp:8-20): iota sorted by
+// grouping key with std::sort (unstable; ties resolved by libstdc++'s
+// introsort, which this oracle shares with the reference build).
+void or_get_permutation(intT n, const intT* grouping, intT* perm) {
+  std::vector<intT> v((size_t)n);
+  std::iota(v.begin(), v.end(), 0);
+  auto comp = [&](int a, int b) { return grouping[a] < grouping[b]; };
+  std::sort(v.begin(), v.end(), comp);
+  std::copy(v.begin(), v.end(), perm);
+}
+
+// get_fixed_size_grouping (utilities.cpp:45-54).
+void or_fixed_size_grouping(intT n, const intT* grouping, intT row_block,
+                            intT* out) {
+  std::vector<intT> perm((size_t)n);
+  or_get_permutation(n, grouping, perm.data());
+  for (intT i = 0; i < n; i++) out[perm[i]] = i / row_block;
+}
+
+// get_partition (utilities.cpp:23-43) + BlockingEngine::CollectBlockingInfo
+// (blocking.cpp:562-614).  out = [VBR_nzcount, VBR_nzblocks_count,
+// VBR_longest_row]; *avg_height = VBR_average_height.
+void or_collect_blocking_info(intT rows, intT cols, const intT* rowptr,
+                              const intT* colidx, const intT* grouping,
+                              intT col_block, intT* out, float* avg_height) {
+  std::vector<intT> sorted(grouping, grouping + rows);
+  std::sort(sorted.begin(), sorted.end());
+  std::vector<intT> part;
+  intT cur = -1;
+  for (intT i = 0; i < rows; i++)
+    if (cur != sorted[i]) {
+      cur = sorted[i];
+      part.push_back(i);
+    }
+  part.push_back(rows);
+  std::vector<intT> perm((size_t)rows);
+  or_get_permutation(rows, grouping, perm.data());
+  intT block_cols = (intT)std::ceil(((float)cols) / col_block);
+  intT nzcount = 0, nzblocks = 0, longest = 0, total_h = 0;
+  for (size_t ib = 0; ib + 1 < part.size(); ib++) {
+    std::vector<bool> flags((size_t)block_cols, false);
+    intT h = part[ib + 1] - part[ib];
+    for (intT ir = part[ib]; ir < part[ib + 1]; ir++) {
+      intT i = perm[ir];
+      for (intT nz = rowptr[i]; nz < rowptr[i + 1]; nz++)
+        flags[colidx[nz] / col_block] = true;
+    }
+    longest = std::max(longest, (intT)std::count(flags.begin(), flags.end(), true));
+    for (size_t jb = 0; jb < flags.size(); jb++)
+      if (flags[jb]) {
+        nzcount += col_block * h;
+        nzblocks++;
+        total_h += h;
+      }
+    if (cols % col_block != 0 && flags[flags.size() - 1])
+      nzcount -= h * (col_block - cols % col_block);
+  }
+  out[0] = nzcount;
+  out[1] = nzblocks;
+  out[2] = longest;
+  *avg_height = ((float)total_h) / nzblocks;
+}
+
+}  // extern "C"

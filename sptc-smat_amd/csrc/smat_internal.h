@@ -1,0 +1,59 @@
+// Internal helpers shared by the libsmat translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "smat.h"
+
+namespace smat {
+
+// Records `st` and a formatted message as this thread's last error; returns st.
+smat_status set_error(smat_status st, const char* fmt, ...);
+
+inline hipStream_t to_hip(smat_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Device buffer owned by library objects.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  smat_status alloc(size_t n);
+  void release();
+  ~DevBuf() { release(); }
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace smat
+
+#define SMAT_HIP_TRY(expr)                                                        \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return ::smat::set_error(SMAT_ERR_HIP, "%s failed: %s (%s:%d)", #expr,      \
+                               hipGetErrorString(e_), __FILE__, __LINE__);        \
+  } while (0)
+
+#define SMAT_TRY(expr)               \
+  do {                               \
+    smat_status s_ = (expr);         \
+    if (s_ != SMAT_OK) return s_;    \
+  } while (0)
+
+#define SMAT_REQUIRE(cond, ...)                                    \
+  do {                                                             \
+    if (!(cond)) return ::smat::set_error(SMAT_ERR_INVALID, __VA_ARGS__); \
+  } while (0)
+
+// Launch check: reports the launch error of the last kernel on this thread.
+#define SMAT_LAUNCH_CHECK() SMAT_HIP_TRY(hipGetLastError())
+
+// Device-side vector types.
+typedef _Float16 smat_h8 __attribute__((ext_vector_type(8)));
+typedef _Float16 smat_h16 __attribute__((ext_vector_type(16)));
+typedef __bf16 smat_b8 __attribute__((ext_vector_type(8)));
+typedef float smat_f4 __attribute__((ext_vector_type(4)));
+typedef short smat_s4 __attribute__((ext_vector_type(4)));

@@ -1,0 +1,227 @@
+"""Python binding of libsmat (the MI355X-native SMaT C ABI, include/smat.h).
+
+The binding is a thin ctypes layer: every call goes straight to the C entry
+point.  Device memory is passed as raw pointers; torch tensors are accepted
+for convenience (PyTorch is used only as an allocator and stream provider).
+The library must be built (``make -C sptc-smat_amd``); loading fails loudly
+otherwise — there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libsmat.so")
+HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "smat.h")
+
+SMAT_OK = 0
+SMAT_ERR_INVALID = -1
+SMAT_ERR_HIP = -2
+SMAT_ERR_ALLOC = -3
+SMAT_ERR_NOT_24 = -4
+SMAT_ERR_UNSUPPORTED = -5
+
+SMAT_F16, SMAT_BF16, SMAT_F32 = 1, 2, 3
+SMAT_B_ROW_MAJOR, SMAT_B_REFERENCE = 0, 1
+SMAT_TAG_REFERENCE, SMAT_TAG_EXACT24 = 0, 1
+SMAT_24_REFERENCE_K16, SMAT_24_SMFMAC = 0, 1
+SMAT_KERNEL_AUTO, SMAT_KERNEL_DENSE, SMAT_KERNEL_MIXED24 = 0, 1, 2
+
+_c = ctypes
+_p = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_sz = ctypes.c_size_t
+
+
+class BcsrDesc(ctypes.Structure):
+    _fields_ = [
+        ("M", _i64), ("K", _i64), ("bm", _i32), ("bk", _i32), ("nbrows", _i64),
+        ("nnzb", _i64), ("rowptr", _p), ("colidx", _p), ("values", _p), ("tags", _p),
+    ]
+
+
+class SpmmOpts(ctypes.Structure):
+    _fields_ = [
+        ("kernel", _i32), ("out_dtype", _i32), ("b_layout", _i32), ("n_tile", _i32),
+        ("max_blocks_per_item", _i32), ("reserved", _i32 * 3),
+    ]
+
+
+# name -> (restype, argtypes); must cover every function declared in smat.h
+SIGNATURES = {
+    "smat_last_error": (ctypes.c_char_p, []),
+    "smat_last_status": (_i32, []),
+    "smat_clear_error": (None, []),
+    "smat_version": (ctypes.c_char_p, []),
+    "smat_bcsr_from_csr": (_i32, [_i64, _i64, _i64, _p, _p, _p, _i32, _i32, _p, ctypes.POINTER(_p)]),
+    "smat_bcsr_describe": (_i32, [_p, ctypes.POINTER(BcsrDesc)]),
+    "smat_bcsr_destroy": (_i32, [_p]),
+    "smat_bcsr_retag_exact24": (_i32, [_p, _p]),
+    "smat_bcsr_grid_maps": (_i32, [ctypes.POINTER(BcsrDesc), _p, _p, _p]),
+    "smat_bcsr_filter": (_i32, [ctypes.POINTER(BcsrDesc), ctypes.POINTER(_i64), ctypes.POINTER(_i64),
+                                _p, _p, _p, _p, _p, _p, _p]),
+    "smat_compress_24": (_i32, [ctypes.POINTER(BcsrDesc), _i32, _p, _p, _p]),
+    "smat_spmm_plan_create": (_i32, [ctypes.POINTER(BcsrDesc), _i64, ctypes.POINTER(SpmmOpts), _p,
+                                     ctypes.POINTER(_p)]),
+    "smat_spmm_execute": (_i32, [_p, _p, _p, _p, _i64, _p, _i64, _p]),
+    "smat_spmm_plan_destroy": (_i32, [_p]),
+    "smat_spmm_plan_info": (_i32, [_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "mmaOBTKernel": (None, [_p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
+    "mmaOBTSKernel": (None, [_p, _p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
+    "preprocessing_mmaSTKernel": (None, [_p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libsmat.so (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libsmat.so not built at {LIB_PATH}; run `make -C sptc-smat_amd`")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+class SmatError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"smat status {status}: {msg}")
+        self.status = status
+
+
+def check(status: int) -> None:
+    if status != SMAT_OK:
+        raise SmatError(status, lib().smat_last_error().decode(errors="replace"))
+
+
+def ptr(x) -> int | None:
+    """Device pointer of a torch tensor (or an int / None passthrough)."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def stream_ptr(stream=None) -> int | None:
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    return stream if isinstance(stream, int) else stream.cuda_stream
+
+
+class Bcsr:
+    """Device-resident BCSR (16 x bk blocks) built on the GPU from a device CSR."""
+
+    def __init__(self, M: int, K: int, rowptr, colidx, values, bk: int = 16,
+                 tag_rule: int = SMAT_TAG_REFERENCE, stream=None):
+        h = _p()
+        nnz = int(colidx.numel()) if hasattr(colidx, "numel") else 0
+        check(lib().smat_bcsr_from_csr(M, K, nnz, ptr(rowptr), ptr(colidx), ptr(values), bk, tag_rule,
+                                       stream_ptr(stream), ctypes.byref(h)))
+        self.handle = h
+        self.desc = BcsrDesc()
+        check(lib().smat_bcsr_describe(h, ctypes.byref(self.desc)))
+
+    @property
+    def nnzb(self) -> int:
+        return int(self.desc.nnzb)
+
+    def arrays(self):
+        """Copies (rowptr, colidx, values[u16], tags) to torch CPU tensors."""
+        import torch
+        d = self.desc
+        out = []
+        for p, n, dt in ((d.rowptr, d.nbrows + 1, torch.int32), (d.colidx, d.nnzb, torch.int32),
+                         (d.values, d.nnzb * 16 * d.bk, torch.int16), (d.tags, d.nnzb, torch.int32)):
+            t = torch.empty(int(n), dtype=dt)
+            if n:
+                torch.cuda.synchronize()
+                copy_to_host(t.data_ptr(), p, int(n) * t.element_size())
+            out.append(t)
+        return out
+
+    def retag_exact24(self, stream=None):
+        check(lib().smat_bcsr_retag_exact24(self.handle, stream_ptr(stream)))
+
+    def close(self):
+        if self.handle:
+            lib().smat_bcsr_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_hip = None
+
+
+def copy_to_host(dst_host: int, src_dev: int, nbytes: int) -> None:
+    """Synchronous device-to-host copy of library-owned memory (hipMemcpy)."""
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.restype = ctypes.c_int
+        _hip.hipMemcpy.argtypes = [_p, _p, _sz, ctypes.c_int]
+    ret = _hip.hipMemcpy(dst_host, src_dev, nbytes, 2)  # hipMemcpyDeviceToHost
+    if ret != 0:
+        raise RuntimeError(f"hipMemcpy D2H failed: {ret}")
+
+
+@dataclass
+class PlanInfo:
+    n_items: int
+    n_split_rows: int
+    scratch_bytes: int
+
+
+class SpmmPlan:
+    def __init__(self, A: Bcsr | BcsrDesc, N: int, *, kernel: int = SMAT_KERNEL_AUTO,
+                 out_dtype: int = SMAT_F32, b_layout: int = SMAT_B_ROW_MAJOR, n_tile: int = 0,
+                 max_blocks_per_item: int = 0, stream=None):
+        desc = A.desc if isinstance(A, Bcsr) else A
+        self._keep = A
+        o = SpmmOpts(kernel, out_dtype, b_layout, n_tile, max_blocks_per_item)
+        h = _p()
+        check(lib().smat_spmm_plan_create(ctypes.byref(desc), N, ctypes.byref(o), stream_ptr(stream),
+                                          ctypes.byref(h)))
+        self.handle = h
+        self.N = N
+
+    def info(self) -> PlanInfo:
+        a, b, c = _i64(), _i64(), _i64()
+        check(lib().smat_spmm_plan_info(self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return PlanInfo(a.value, b.value, c.value)
+
+    def __call__(self, B, C, ldb: int | None = None, ldc: int | None = None, values24=None,
+                 index24=None, stream=None):
+        if ldb is None:
+            ldb = B.shape[-1]
+        if ldc is None:
+            ldc = C.shape[-1]
+        check(lib().smat_spmm_execute(self.handle, ptr(values24), ptr(index24), ptr(B), ldb, ptr(C),
+                                      ldc, stream_ptr(stream)))
+        return C
+
+    def close(self):
+        if self.handle:
+            lib().smat_spmm_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,140 @@
+"""Seeded synthetic sparse matrices for the SMaT workloads.
+
+Every generator returns a host CSR: (M, K, rowptr int32[M+1], colidx int32[nnz],
+values uint16[nnz] = FP16 bits), rows padded to a multiple of 16 and columns to
+a multiple of 32 (the reference pads rows to 16 and columns to k,
+mmio_highlevel.h:514-520; padding to 32 keeps one matrix valid for bk = 16
+and bk = 32).  Columns are sorted inside each row.  The reference ships only
+unseeded generators (src/generate_matrices.py uses `random`), so these follow
+the workload definitions instead:
+
+  band           n, half-bandwidth hb, each in-band entry kept with prob p,
+                 values U[-1, 1]
+  rmat           scale s, 16 * 2^s edge draws, quadrant probs (a, b, c, d),
+                 duplicates merged, values U[-1, 1]
+  block_clustered  n, `tiles` dense 16x16 tiles at distinct random block
+                 coordinates, every 1x4 group pruned to its 2 largest |v|
+  power_law      n, row lengths Zipf(s) truncated at max_len, columns uniform
+                 without repetition, values N(0, 1) with 1% scaled by 1e3
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def pad_to(x: int, m: int) -> int:
+    return (x + m - 1) // m * m
+
+
+def _to_csr(n_rows: int, n_cols: int, rows: np.ndarray, cols: np.ndarray, vals: np.ndarray):
+    order = np.lexsort((cols, rows))
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    M, K = pad_to(n_rows, 16), pad_to(n_cols, 32)
+    counts = np.bincount(rows, minlength=M)
+    rowptr = np.zeros(M + 1, dtype=np.int64)
+    np.cumsum(counts, out=rowptr[1:])
+    return M, K, rowptr.astype(np.int32), cols.astype(np.int32), vals.astype(np.float16).view(np.uint16)
+
+
+def _uniform_vals(rng: np.random.Generator, n: int) -> np.ndarray:
+    v = rng.uniform(-1.0, 1.0, size=n).astype(np.float16)
+    # keep explicit zeros out of the pattern (they would change the 2:4 tags)
+    v[v == 0] = np.float16(0.5)
+    return v
+
+
+def band(n: int = 4096, hb: int = 64, p: float = 0.5, seed: int = 1001):
+    rng = np.random.default_rng(seed)
+    i = np.repeat(np.arange(n, dtype=np.int64), 2 * hb + 1)
+    off = np.tile(np.arange(-hb, hb + 1, dtype=np.int64), n)
+    j = i + off
+    ok = (j >= 0) & (j < n)
+    i, j = i[ok], j[ok]
+    keep = rng.random(i.size) < p
+    i, j = i[keep], j[keep]
+    return _to_csr(n, n, i, j, _uniform_vals(rng, i.size))
+
+
+def rmat(scale: int = 18, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+         seed: int = 1002):
+    rng = np.random.default_rng(seed)
+    n = 1 << scale
+    m = edge_factor * n
+    rows = np.zeros(m, dtype=np.int64)
+    cols = np.zeros(m, dtype=np.int64)
+    ab, abc = a + b, a + b + c
+    for lvl in range(scale):
+        u = rng.random(m)
+        bit = 1 << (scale - 1 - lvl)
+        right = ((u >= a) & (u < ab)) | (u >= abc)   # quadrants b, d
+        down = u >= ab                                # quadrants c, d
+        cols += right * bit
+        rows += down * bit
+    key = np.unique(rows * n + cols)
+    rows, cols = key // n, key % n
+    return _to_csr(n, n, rows, cols, _uniform_vals(rng, rows.size))
+
+
+def prune_2of4(dense: np.ndarray) -> np.ndarray:
+    """Keeps the two largest-magnitude entries of every aligned 1x4 group
+    (ties: lower column first)."""
+    g = dense.reshape(*dense.shape[:-1], dense.shape[-1] // 4, 4)
+    order = np.argsort(-np.abs(g), axis=-1, kind="stable")
+    mask = np.zeros_like(g, dtype=bool)
+    np.put_along_axis(mask, order[..., :2], True, axis=-1)
+    return np.where(mask, g, 0).reshape(dense.shape)
+
+
+def block_clustered(n: int = 65536, tiles: int = 12000, seed: int = 1003, prune: bool = True):
+    rng = np.random.default_rng(seed)
+    nb = n // 16
+    flat = rng.choice(nb * nb, size=tiles, replace=False)
+    br, bc = flat // nb, flat % nb
+    vals = rng.uniform(-1.0, 1.0, size=(tiles, 16, 16)).astype(np.float16)
+    vals[vals == 0] = np.float16(0.5)
+    if prune:
+        vals = prune_2of4(vals)
+    r = np.arange(16)
+    rows = (br[:, None, None] * 16 + r[None, :, None]) * np.ones((1, 1, 16), dtype=np.int64)
+    cols = (bc[:, None, None] * 16 + r[None, None, :]) * np.ones((1, 16, 1), dtype=np.int64)
+    keep = vals != 0
+    return _to_csr(n, n, rows[keep], cols[keep], vals[keep])
+
+
+def power_law(n: int = 131072, s: float = 1.5, max_len: int = 4096, seed: int = 1004,
+              outlier_frac: float = 0.01, outlier_scale: float = 1e3):
+    """Returns the CSR of FP16 values plus the FP32 values (for split modes)."""
+    rng = np.random.default_rng(seed)
+    lens = rng.zipf(s, size=n)
+    lens = np.minimum(lens, max_len).astype(np.int64)
+    rows = np.repeat(np.arange(n, dtype=np.int64), lens)
+    cols = np.empty(rows.size, dtype=np.int64)
+    pos = 0
+    for i, L in enumerate(lens):
+        cols[pos:pos + L] = rng.choice(n, size=L, replace=False) if L > 64 else _distinct_small(rng, n, L)
+        pos += L
+    vals = rng.standard_normal(rows.size).astype(np.float32)
+    out = rng.random(rows.size) < outlier_frac
+    vals[out] *= outlier_scale
+    vals[vals == 0] = 1.0
+    return _to_csr(n, n, rows, cols, vals)
+
+
+def _distinct_small(rng: np.random.Generator, n: int, L: int) -> np.ndarray:
+    out = set()
+    while len(out) < L:
+        out.update(rng.integers(0, n, size=L - len(out)).tolist())
+    return np.fromiter(out, dtype=np.int64, count=L)
+
+
+def dense_b(K: int, N: int, seed: int = 7) -> np.ndarray:
+    """Row-major K x N FP16 matrix, U[-1, 1], as uint16 bits."""
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-1.0, 1.0, size=(K, N)).astype(np.float16).view(np.uint16)
+
+
+def csr_to_dense(M, K, rowptr, colidx, values_u16) -> np.ndarray:
+    d = np.zeros((M, K), dtype=np.float32)
+    rows = np.repeat(np.arange(M), np.diff(rowptr))
+    d[rows, colidx] = values_u16.view(np.float16).astype(np.float32)
+    return d

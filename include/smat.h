@@ -153,13 +153,19 @@ smat_status smat_compress_24(const smat_bcsr_desc* A, int32_t layout, uint16_t* 
 #define SMAT_KERNEL_DENSE 1   /* every stored block on dense MFMA */
 #define SMAT_KERNEL_MIXED24 2 /* tag-1 blocks on SMFMAC (2:4), tag-2 blocks on MFMA */
 
+/* How row-major B tiles reach the matrix cores. */
+#define SMAT_BSTAGE_AUTO 0   /* LDS when B rows are 16-byte aligned, else direct */
+#define SMAT_BSTAGE_DIRECT 1 /* per-lane 16-bit gathers straight into MFMA operands */
+#define SMAT_BSTAGE_LDS 2    /* 128-bit row loads into LDS, ds_read_b64_tr_b16 operand reads */
+
 typedef struct {
   int32_t kernel;          /* SMAT_KERNEL_* */
   int32_t out_dtype;       /* SMAT_F32 or SMAT_F16 */
   int32_t b_layout;        /* SMAT_B_ROW_MAJOR or SMAT_B_REFERENCE */
   int32_t n_tile;          /* columns per wave: 16, 32 or 64; 0 = auto */
   int32_t max_blocks_per_item; /* split block rows longer than this; 0 = auto */
-  int32_t reserved[3];
+  int32_t b_staging;       /* SMAT_BSTAGE_* (row-major B only) */
+  int32_t reserved[2];
 } smat_spmm_opts;
 
 typedef struct smat_plan_t* smat_plan;

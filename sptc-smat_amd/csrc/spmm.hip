@@ -141,6 +141,116 @@ __global__ __launch_bounds__(256) void k_spmm_dense(
   }
 }
 
+// Row-major B through LDS.  Per MFMA step the 32 B rows it reduces over
+// (two 16x16 blocks or one 16x32 block) x the wave's 16*NSUB columns are
+// fetched with 128-bit loads (each row segment is contiguous in HBM), stored
+// to a per-wave LDS tile, and read back as MFMA B operands with
+// ds_read_b64_tr_b16: a 16-lane group reading a 4 x 16 sub-tile (lane L
+// addressing row L/4, columns 4*(L%4)..+3) receives column L of it, i.e.
+// B[k0..k0+3][n0+L] — exactly 4 of the 8 k-values the operand layout wants.
+// Two such reads give a lane its 8 values.  The next step's global loads are
+// issued before the current step's MFMAs.  Waves never share LDS, so no
+// barrier is needed: a wave's LDS accesses complete in issue order.
+template <int BK, int NSUB, bool OUT_F16>
+__global__ __launch_bounds__(256) void k_spmm_lds(
+    const Item* __restrict__ items, int64_t n_items, int32_t ntiles,
+    const int32_t* __restrict__ colidx, const uint16_t* __restrict__ values,
+    const uint16_t* __restrict__ B, int64_t ldb, int64_t N, void* __restrict__ C, int64_t ldc,
+    float* __restrict__ scratch) {
+  constexpr int NT = 16 * NSUB;           // columns per wave
+  constexpr int CPR = NT / 8;             // 16-byte chunks per B row segment
+  constexpr int RPL = 64 / CPR;           // B rows per load instruction
+  constexpr int NLD = 32 / RPL;           // load instructions per MFMA step
+  constexpr int PITCH = NT + 8;           // LDS row pitch in halves (16-byte pad)
+  constexpr int STEP = 32 / BK;           // blocks per MFMA step
+  __shared__ __attribute__((aligned(16))) uint16_t lds_all[kWaves * 32 * PITCH];
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t task = (int64_t)wg * kWaves + wave;
+  const int64_t item_id = task / ntiles;
+  if (item_id >= n_items) return;
+  const int nt = (int)(task % ntiles);
+  const Item it = items[item_id];
+  const int64_t n0 = (int64_t)nt * NT;
+  uint16_t* lds = lds_all + wave * 32 * PITCH;
+
+  const int r16 = lane & 15, grp = lane >> 4;
+  const int sub = (STEP == 2) ? (lane >> 5) : 0;
+  const int kin = (STEP == 2) ? ((grp & 1) * 8) : grp * 8;
+  // load coordinates: instruction i covers rows i*RPL .. i*RPL+RPL-1
+  const int lrow = lane / CPR, lch = lane % CPR;
+  const int64_t ncol = n0 + lch * 8;
+  const bool col_ok = ncol < N;  // N % 8 == 0: a chunk is all in or all out
+
+  smat_f4 acc[NSUB];
+#pragma unroll
+  for (int j = 0; j < NSUB; j++) acc[j] = smat_f4{0.f, 0.f, 0.f, 0.f};
+
+  uint4 breg[NLD];
+  uint4 areg;
+  auto gload = [&](int32_t p) {
+#pragma unroll
+    for (int i = 0; i < NLD; i++) {
+      const int row = i * RPL + lrow;  // 0..31 = k-slot
+      const int32_t blk = p + row / BK;
+      if (blk < it.pend && col_ok)
+        breg[i] = *reinterpret_cast<const uint4*>(B + (int64_t)(colidx[blk] + row % BK) * ldb + ncol);
+      else
+        breg[i] = uint4{0, 0, 0, 0};
+    }
+    const int32_t blk = p + sub;
+    if (blk < it.pend)
+      areg = *reinterpret_cast<const uint4*>(values + (int64_t)blk * 16 * BK + r16 * BK + kin);
+    else
+      areg = uint4{0, 0, 0, 0};
+  };
+
+  if (it.pbeg < it.pend) gload(it.pbeg);
+  for (int32_t p = it.pbeg; p < it.pend; p += STEP) {
+#pragma unroll
+    for (int i = 0; i < NLD; i++)
+      *reinterpret_cast<uint4*>(lds + (i * RPL + lrow) * PITCH + lch * 8) = breg[i];
+    const smat_h8 a = __builtin_bit_cast(smat_h8, areg);
+    if (p + STEP < it.pend) gload(p + STEP);
+#pragma unroll
+    for (int j = 0; j < NSUB; j++) {
+      const uint16_t* base = lds + (8 * grp + (r16 >> 2)) * PITCH + 16 * j + (r16 & 3) * 4;
+      const smat_s4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+          (__attribute__((address_space(3))) smat_s4*)(base));
+      const smat_s4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+          (__attribute__((address_space(3))) smat_s4*)(base + 4 * PITCH));
+      // Reinterpret the two 64-bit results as one operand.  (Extracting the
+      // 16-bit elements one by one was lowered to a splat of element 0 by
+      // this compiler; the whole-vector cast maps the loaded registers
+      // straight onto the MFMA operand.)
+      const smat_s4 both[2] = {lo, hi};
+      const smat_h8 b = __builtin_bit_cast(smat_h8, both);
+      acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc[j], 0, 0, 0);
+    }
+  }
+
+#pragma unroll
+  for (int j = 0; j < NSUB; j++) {
+    const int64_t n = n0 + 16 * j + r16;
+    if (n >= N) continue;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int row = 4 * grp + i;
+      if (it.slot >= 0) {
+        scratch[((int64_t)it.slot * 16 + row) * N + n] = acc[j][i];
+      } else {
+        const int64_t crow = (int64_t)it.brow * 16 + row;
+        if (OUT_F16)
+          reinterpret_cast<_Float16*>(C)[crow * ldc + n] = (_Float16)acc[j][i];
+        else
+          reinterpret_cast<float*>(C)[crow * ldc + n] = acc[j][i];
+      }
+    }
+  }
+}
+
 struct Reduce {
   int32_t brow;
   int32_t slot_beg;
@@ -244,9 +354,27 @@ void launch_dense(const smat_plan_t* P, const uint16_t* B, int64_t ldb, void* C,
                      C, ldc, P->scratch.as<float>());
 }
 
+template <int BK, int NSUB, bool OUT_F16>
+void launch_lds(const smat_plan_t* P, const uint16_t* B, int64_t ldb, void* C, int64_t ldc, hipStream_t s) {
+  const int32_t ntiles = (int32_t)((P->N + 16 * NSUB - 1) / (16 * NSUB));
+  const int64_t tasks = P->n_items * ntiles;
+  const unsigned grid = blocks_for(tasks, kWaves);
+  hipLaunchKernelGGL((k_spmm_lds<BK, NSUB, OUT_F16>), grid, 64 * kWaves, 0, s, P->items.as<Item>(),
+                     P->n_items, ntiles, P->A.colidx, P->A.values, B, ldb, P->N, C, ldc,
+                     P->scratch.as<float>());
+}
+
 template <int BK, int BLAYOUT, bool OUT_F16>
 void dispatch_nsub(const smat_plan_t* P, const uint16_t* B, int64_t ldb, void* C, int64_t ldc,
                    hipStream_t s) {
+  const bool lds_ok = (P->N % 8 == 0) && (ldb % 8 == 0) && ((reinterpret_cast<uintptr_t>(B) & 15) == 0);
+  if (BLAYOUT == SMAT_B_ROW_MAJOR && P->opts.b_staging != SMAT_BSTAGE_DIRECT && lds_ok) {
+    switch (P->nsub) {
+      case 1: launch_lds<BK, 1, OUT_F16>(P, B, ldb, C, ldc, s); return;
+      case 2: launch_lds<BK, 2, OUT_F16>(P, B, ldb, C, ldc, s); return;
+      default: launch_lds<BK, 4, OUT_F16>(P, B, ldb, C, ldc, s); return;
+    }
+  }
   switch (P->nsub) {
     case 1: launch_dense<BK, 1, BLAYOUT, OUT_F16>(P, B, ldb, C, ldc, s); break;
     case 2: launch_dense<BK, 2, BLAYOUT, OUT_F16>(P, B, ldb, C, ldc, s); break;
@@ -288,6 +416,8 @@ smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat
                "smat_spmm_plan_create: kernel %d not available in this build", o.kernel);
   SMAT_REQUIRE(o.n_tile == 0 || o.n_tile == 16 || o.n_tile == 32 || o.n_tile == 64,
                "smat_spmm_plan_create: n_tile must be 0, 16, 32 or 64");
+  SMAT_REQUIRE(o.b_staging >= SMAT_BSTAGE_AUTO && o.b_staging <= SMAT_BSTAGE_LDS,
+               "smat_spmm_plan_create: unknown b_staging %d", o.b_staging);
   hipStream_t s = smat::to_hip(stream);
 
   smat_plan_t* P = new smat_plan_t();
@@ -362,6 +492,9 @@ smat_status smat_spmm_execute(smat_plan P, const uint16_t* values24, const void*
     SMAT_REQUIRE(ldb >= P->N, "smat_spmm_execute: ldb < N for row-major B");
   else
     SMAT_REQUIRE(ldb >= P->A.K, "smat_spmm_execute: ldb < K for reference-layout B");
+  if (P->opts.b_staging == SMAT_BSTAGE_LDS && P->opts.b_layout == SMAT_B_ROW_MAJOR)
+    SMAT_REQUIRE(P->N % 8 == 0 && ldb % 8 == 0 && (reinterpret_cast<uintptr_t>(B) & 15) == 0,
+                 "smat_spmm_execute: LDS staging needs N %% 8 == 0, ldb %% 8 == 0 and a 16-byte aligned B");
   hipStream_t s = smat::to_hip(stream);
   if (P->A.bk == 16) dispatch_layout<16>(P, B, ldb, C, ldc, s);
   else dispatch_layout<32>(P, B, ldb, C, ldc, s);

@@ -46,8 +46,11 @@ class BcsrDesc(ctypes.Structure):
 class SpmmOpts(ctypes.Structure):
     _fields_ = [
         ("kernel", _i32), ("out_dtype", _i32), ("b_layout", _i32), ("n_tile", _i32),
-        ("max_blocks_per_item", _i32), ("reserved", _i32 * 3),
+        ("max_blocks_per_item", _i32), ("b_staging", _i32), ("reserved", _i32 * 2),
     ]
+
+
+SMAT_BSTAGE_AUTO, SMAT_BSTAGE_DIRECT, SMAT_BSTAGE_LDS = 0, 1, 2
 
 
 # name -> (restype, argtypes); must cover every function declared in smat.h
@@ -153,6 +156,46 @@ class Bcsr:
     def retag_exact24(self, stream=None):
         check(lib().smat_bcsr_retag_exact24(self.handle, stream_ptr(stream)))
 
+    def grid_maps(self, stream=None):
+        """(blockInfo, relativeBlockIndexMapping) as device int32 tensors over
+        the dense block grid, as the reference kernels take them."""
+        import torch
+        d = self.desc
+        ng = int(d.nbrows * (d.K // d.bk))
+        info = torch.empty(ng, dtype=torch.int32, device="cuda")
+        rel = torch.empty(ng, dtype=torch.int32, device="cuda")
+        check(lib().smat_bcsr_grid_maps(ctypes.byref(d), ptr(info), ptr(rel), stream_ptr(stream)))
+        return info, rel
+
+    def compress_24(self, layout: int = SMAT_24_SMFMAC, stream=None):
+        """Compressed 2:4 arrays (values, metadata) as device tensors."""
+        import torch
+        d = self.desc
+        n = int(d.nnzb)
+        if layout == SMAT_24_REFERENCE_K16:
+            vals = torch.empty(max(n, 1) * 128, dtype=torch.int16, device="cuda")
+            meta = torch.empty(max(n, 1) * 32, dtype=torch.uint8, device="cuda")
+        else:
+            vals = torch.empty(max(n, 1) * 16 * (d.bk // 2), dtype=torch.int16, device="cuda")
+            meta = torch.empty(max(n, 1) * 16 * (d.bk // 16), dtype=torch.int16, device="cuda")
+        check(lib().smat_compress_24(ctypes.byref(d), layout, ptr(vals), ptr(meta), stream_ptr(stream)))
+        return vals, meta
+
+    def filter(self, stream=None):
+        """(sparse list, dense list), each (rowptr, colidx, values) device tensors."""
+        import torch
+        d = self.desc
+        ns, nd = _i64(), _i64()
+        check(lib().smat_bcsr_filter(ctypes.byref(d), ctypes.byref(ns), ctypes.byref(nd), None, None, None,
+                                     None, None, None, stream_ptr(stream)))
+        mk = lambda n, dt: torch.empty(max(int(n), 1), dtype=dt, device="cuda")  # noqa: E731
+        bs = 16 * d.bk
+        s = (mk(d.nbrows + 1, torch.int32), mk(ns.value, torch.int32), mk(ns.value * bs, torch.int16))
+        dn = (mk(d.nbrows + 1, torch.int32), mk(nd.value, torch.int32), mk(nd.value * bs, torch.int16))
+        check(lib().smat_bcsr_filter(ctypes.byref(d), None, None, *(ptr(x) for x in s), *(ptr(x) for x in dn),
+                                     stream_ptr(stream)))
+        return (s[0], s[1][: ns.value], s[2][: ns.value * bs]), (dn[0], dn[1][: nd.value], dn[2][: nd.value * bs])
+
     def close(self):
         if self.handle:
             lib().smat_bcsr_destroy(self.handle)
@@ -190,10 +233,10 @@ class PlanInfo:
 class SpmmPlan:
     def __init__(self, A: Bcsr | BcsrDesc, N: int, *, kernel: int = SMAT_KERNEL_AUTO,
                  out_dtype: int = SMAT_F32, b_layout: int = SMAT_B_ROW_MAJOR, n_tile: int = 0,
-                 max_blocks_per_item: int = 0, stream=None):
+                 max_blocks_per_item: int = 0, b_staging: int = 0, stream=None):
         desc = A.desc if isinstance(A, Bcsr) else A
         self._keep = A
-        o = SpmmOpts(kernel, out_dtype, b_layout, n_tile, max_blocks_per_item)
+        o = SpmmOpts(kernel, out_dtype, b_layout, n_tile, max_blocks_per_item, b_staging)
         h = _p()
         check(lib().smat_spmm_plan_create(ctypes.byref(desc), N, ctypes.byref(o), stream_ptr(stream),
                                           ctypes.byref(h)))

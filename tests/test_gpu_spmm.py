@@ -1,0 +1,195 @@
+"""GPU parity of the BCSR build and the dense-block MFMA SpMM against the
+CPU oracle (FP64 ground truth, bit-exact integer layout).  Calls go through
+the C ABI (libsmat.so via ctypes)."""
+import numpy as np
+import pytest
+
+import oracle_ffi as of
+from smat import gen
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def _dev(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.view(dtype)
+    return t.cuda()
+
+
+def _bcsr(M, K, rp, ci, v, bk=16, tag_rule=None):
+    import smat
+    tr = smat.SMAT_TAG_REFERENCE if tag_rule is None else tag_rule
+    return smat.Bcsr(M, K, _dev(rp), _dev(ci), _dev(v.view(np.int16)), bk=bk, tag_rule=tr)
+
+
+def _rel_fro(C, R):
+    return float(np.linalg.norm(C - R) / max(np.linalg.norm(R), 1e-300))
+
+
+CASES = [
+    ("band_small", lambda: gen.band(n=512, hb=32, p=0.5, seed=21)),
+    ("rmat_small", lambda: gen.rmat(scale=10, edge_factor=8, seed=22)),
+    ("clustered_small", lambda: gen.block_clustered(n=1024, tiles=300, seed=23)),
+]
+
+
+@pytest.mark.parametrize("name,make", CASES)
+@pytest.mark.parametrize("bk", [16, 32])
+def test_bcsr_build_bit_exact(name, make, bk):
+    _need_gpu()
+    M, K, rp, ci, v = make()
+    print(f"case {name} bk={bk} M={M} nnz={ci.size}", flush=True)
+    A = _bcsr(M, K, rp, ci, v, bk=bk)
+    got = A.arrays()
+    exp = of.csr_to_bcsr(M, K, bk, rp, ci, v)
+    assert A.nnzb == exp[1].size
+    for g, e, what in zip(got, exp, ("rowptr", "colidx", "values", "tags")):
+        assert np.array_equal(g.numpy().view(e.dtype), e), what
+
+
+def test_bcsr_build_unsorted_columns_and_tags():
+    _need_gpu()
+    # columns in file order (not sorted) change the reference's tag rule
+    rng = np.random.default_rng(3)
+    M, K = 64, 64
+    rows, cols = [], []
+    for r in range(M):
+        c = rng.choice(K, size=rng.integers(0, 20), replace=False)
+        rows += [r] * c.size
+        cols += c.tolist()
+    rows, cols = np.array(rows), np.array(cols, np.int32)
+    rp = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=M))]).astype(np.int32)
+    v = rng.uniform(-1, 1, cols.size).astype(np.float16)
+    v[v == 0] = 1
+    v = v.view(np.uint16)
+    A = _bcsr(M, K, rp, cols, v, 16)
+    got = A.arrays()
+    exp = of.csr_to_bcsr(M, K, 16, rp, cols, v)
+    for g, e in zip(got, exp):
+        assert np.array_equal(g.numpy().view(e.dtype), e)
+    import smat
+    A2 = _bcsr(M, K, rp, cols, v, 16, tag_rule=smat.SMAT_TAG_EXACT24)
+    assert np.array_equal(A2.arrays()[3].numpy(), of.tag_exact24(16, exp[2]))
+
+
+SPMM_CASES = [
+    ("band", lambda: gen.band(n=1024, hb=64, p=0.5, seed=31), 128),
+    ("rmat", lambda: gen.rmat(scale=11, edge_factor=16, seed=32), 64),
+    ("rmat_n256", lambda: gen.rmat(scale=10, edge_factor=16, seed=33), 256),
+    ("clustered", lambda: gen.block_clustered(n=2048, tiles=600, seed=34), 48),
+    ("band_n40", lambda: gen.band(n=512, hb=16, p=0.7, seed=35), 40),
+    ("band_n36", lambda: gen.band(n=512, hb=16, p=0.7, seed=36), 36),
+]
+
+
+@pytest.mark.parametrize("name,make,N", SPMM_CASES)
+@pytest.mark.parametrize("bk", [16, 32])
+@pytest.mark.parametrize("n_tile", [0, 16, 32])
+@pytest.mark.parametrize("stage", [1, 2])
+def test_spmm_dense_vs_fp64(name, make, N, bk, n_tile, stage):
+    _need_gpu()
+    import smat
+    if stage == smat.SMAT_BSTAGE_LDS and N % 8:
+        pytest.skip("LDS staging needs N % 8 == 0 (AUTO falls back to direct gathers)")
+    M, K, rp, ci, v = make()
+    print(f"case {name} bk={bk} N={N} n_tile={n_tile} stage={stage} M={M} nnz={ci.size}", flush=True)
+    A = _bcsr(M, K, rp, ci, v, bk=bk)
+    B = gen.dense_b(K, N, seed=5)
+    brp, bci, bv, _ = of.csr_to_bcsr(M, K, bk, rp, ci, v)
+    R = of.spmm_bcsr_f64(bk, brp, bci, bv, B)
+    dB = _dev(B.view(np.int16))
+    C = torch.full((M, N), float("nan"), dtype=torch.float32, device="cuda")
+    plan = smat.SpmmPlan(A, N, n_tile=n_tile, b_staging=stage)
+    plan(dB, C)
+    torch.cuda.synchronize()
+    Cn = C.cpu().numpy().astype(np.float64)
+    assert np.all(np.isfinite(Cn))
+    # FP32 accumulation of FP16 products: relative Frobenius error well below 1e-5
+    assert _rel_fro(Cn, R) < 1e-5
+
+
+@pytest.mark.parametrize("S", [2, 4, 10])
+@pytest.mark.parametrize("stage", [1, 2])
+def test_spmm_split_rows_deterministic(S, stage):
+    """Forces block-row splitting (slices of S blocks) and checks the partial
+    tile reduction, and that two runs give identical bits."""
+    _need_gpu()
+    import smat
+    M, K, rp, ci, v = gen.rmat(scale=10, edge_factor=16, seed=41)
+    N = 64
+    A = _bcsr(M, K, rp, ci, v)
+    B = gen.dense_b(K, N, seed=6)
+    brp, bci, bv, _ = of.csr_to_bcsr(M, K, 16, rp, ci, v)
+    R = of.spmm_bcsr_f64(16, brp, bci, bv, B)
+    plan = smat.SpmmPlan(A, N, max_blocks_per_item=S, b_staging=stage)
+    info = plan.info()
+    assert info.n_split_rows > 0
+    dB = _dev(B.view(np.int16))
+    C1 = torch.empty((M, N), dtype=torch.float32, device="cuda")
+    C2 = torch.empty((M, N), dtype=torch.float32, device="cuda")
+    plan(dB, C1)
+    plan(dB, C2)
+    torch.cuda.synchronize()
+    assert torch.equal(C1, C2)
+    assert _rel_fro(C1.cpu().numpy().astype(np.float64), R) < 1e-5
+
+
+def test_spmm_fp16_out_and_reference_layout():
+    """FP16 output and the reference's B[n*K+k] layout, compared with the
+    FP64 oracle and the FP16-accumulate emulation of the reference kernel."""
+    _need_gpu()
+    import smat
+    M, K, rp, ci, v = gen.band(n=1024, hb=64, p=0.5, seed=51)
+    N = 64
+    A = _bcsr(M, K, rp, ci, v)
+    B = gen.dense_b(K, N, seed=7)
+    brp, bci, bv, _ = of.csr_to_bcsr(M, K, 16, rp, ci, v)
+    R = of.spmm_bcsr_f64(16, brp, bci, bv, B)
+    Rsim = of.spmm_refsim_fp16acc(16, brp, bci, bv, B).astype(np.float64)
+    Bt = np.ascontiguousarray(B.T)  # N x K: element (k, n) at n*K + k
+    plan = smat.SpmmPlan(A, N, out_dtype=smat.SMAT_F16, b_layout=smat.SMAT_B_REFERENCE)
+    C = torch.empty((M, N), dtype=torch.float16, device="cuda")
+    plan(_dev(Bt.view(np.int16)), C, ldb=K)
+    torch.cuda.synchronize()
+    Cn = C.cpu().numpy().astype(np.float64)
+    assert _rel_fro(Cn, R) < 1e-3
+    # parity with the reference semantics (FP16 accumulate): normwise max error
+    assert np.max(np.abs(Cn - Rsim)) / np.max(np.abs(Rsim)) < 1e-2
+
+
+def test_spmm_all_ones_closed_form():
+    """All-ones B (the only input the reference harness ever used,
+    matrix.h:45): every output column is the row sum of A."""
+    _need_gpu()
+    import smat
+    M, K, rp, ci, v = gen.rmat(scale=9, edge_factor=8, seed=61)
+    N = 32
+    A = _bcsr(M, K, rp, ci, v)
+    B = np.full((K, N), 0x3C00, np.uint16)
+    C = torch.empty((M, N), dtype=torch.float32, device="cuda")
+    smat.SpmmPlan(A, N)(_dev(B.view(np.int16)), C)
+    torch.cuda.synchronize()
+    rows = gen.csr_to_dense(M, K, rp, ci, v).astype(np.float64).sum(1)
+    Cn = C.cpu().numpy()
+    assert np.allclose(Cn, rows[:, None] * np.ones((1, N)), rtol=1e-5, atol=1e-5)
+
+
+def test_invalid_arguments_raise():
+    _need_gpu()
+    import smat
+    M, K, rp, ci, v = gen.band(n=256, hb=8, p=0.5, seed=71)
+    with pytest.raises(smat.SmatError):
+        _bcsr(M, K, rp, ci, v, bk=8)
+    A = _bcsr(M, K, rp, ci, v)
+    with pytest.raises(smat.SmatError):
+        smat.SpmmPlan(A, 0)
+    with pytest.raises(smat.SmatError):
+        smat.SpmmPlan(A, 64, n_tile=48)

@@ -39,7 +39,22 @@ def parse():
     ap.add_argument("--stage", type=int, default=0, help="B staging: 0 auto, 1 direct, 2 LDS")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline duration")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="only run a device copy of this many MiB (HBM counter calibration) and exit")
     return ap.parse_args()
+
+
+def calib_copy(mib: int) -> None:
+    """A plain device-to-device copy of known size: reads and writes `mib` MiB
+    per launch; its FETCH_SIZE / WRITE_SIZE readings calibrate the counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calib_copy_mib": mib, "launches": 5}))
 
 
 def cpu_baseline(M, rp, ci, vals_u16, N, target_s):
@@ -89,6 +104,9 @@ def cpu_baseline(M, rp, ci, vals_u16, N, target_s):
 
 def main():
     args = parse()
+    if args.calib_copy_mib:
+        calib_copy(args.calib_copy_mib)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding of the SpMM "
                                    "(block-row ranges, B replicated) is a design note, not built"}))

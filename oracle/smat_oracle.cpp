@@ -150,7 +150,8 @@ int64_t or_csr_to_bcsr(int64_t M, int64_t K, int bk, const int32_t* rowptr,
       int64_t cur = (row / bm) * ncr + col / bk;
       if (info[cur] == 0) continue;
       out_colidx[nb] = (int32_t)col;
-      out_tag[nb] = info[cur];
+      // blocks narrower than a 1x4 group (bk < 4, not a reference width) are dense
+      out_tag[nb] = bk >= 4 ? info[cur] : 2;
       nb++;
     }
   }
@@ -174,7 +175,7 @@ int64_t or_csr_to_bcsr(int64_t M, int64_t K, int bk, const int32_t* rowptr,
 // group holds more than two nonzeros.
 void or_tag_exact24(int64_t nnzb, int bk, const uint16_t* val, int32_t* tag) {
   for (int64_t b = 0; b < nnzb; b++) {
-    int t = 1;
+    int t = bk >= 4 ? 1 : 2;
     for (int r = 0; r < 16 && t == 1; r++)
       for (int g = 0; g < bk / 4 && t == 1; g++) {
         int c = 0;

@@ -186,7 +186,8 @@ smat_status smat_bcsr_from_csr(int64_t M, int64_t K, int64_t nnz, const int32_t*
                                int32_t tag_rule, smat_stream stream, smat_bcsr* out) {
   SMAT_REQUIRE(out != nullptr, "smat_bcsr_from_csr: out is NULL");
   *out = nullptr;
-  SMAT_REQUIRE(bk == 16 || bk == 32, "smat_bcsr_from_csr: bk must be 16 or 32 (got %d)", bk);
+  SMAT_REQUIRE(bk == 1 || bk == 2 || bk == 4 || bk == 8 || bk == 16 || bk == 32,
+               "smat_bcsr_from_csr: bk must be 1, 2, 4, 8, 16 or 32 (got %d)", bk);
   SMAT_REQUIRE(M > 0 && M % 16 == 0, "smat_bcsr_from_csr: M=%lld must be a positive multiple of 16",
                (long long)M);
   SMAT_REQUIRE(K > 0 && K % bk == 0, "smat_bcsr_from_csr: K=%lld must be a positive multiple of bk",
@@ -253,7 +254,8 @@ smat_status smat_bcsr_from_csr(int64_t M, int64_t K, int64_t nnz, const int32_t*
     return fail(st);
   hipLaunchKernelGGL(k_fill_u16, grid_for(nnzb * 16 * bk), kThreads, 0, s, nnzb * 16 * bk,
                      A->values.as<uint16_t>(), (uint16_t)0);
-  hipLaunchKernelGGL(k_fill_i32, grid_for(nnzb), kThreads, 0, s, nnzb, A->tags.as<int32_t>(), 1);
+  // blocks narrower than one 1x4 group cannot be 2:4: tag them dense
+  hipLaunchKernelGGL(k_fill_i32, grid_for(nnzb), kThreads, 0, s, nnzb, A->tags.as<int32_t>(), bk >= 4 ? 1 : 2);
   if (nnz > 0) {
     hipLaunchKernelGGL(k_block_meta, grid_for(nnz), kThreads, 0, s, nnz, ncb, (int)bk,
                        keys_sorted.as<uint64_t>(), head.as<int32_t>(), blkid.as<int32_t>(),
@@ -264,7 +266,7 @@ smat_status smat_bcsr_from_csr(int64_t M, int64_t K, int64_t nnz, const int32_t*
   }
   hipLaunchKernelGGL(k_rowptr, grid_for(nnzb + 1), kThreads, 0, s, nnzb, nbrows, brow.as<int32_t>(),
                      A->rowptr.as<int32_t>());
-  if (tag_rule == SMAT_TAG_EXACT24 && nnzb > 0)
+  if (tag_rule == SMAT_TAG_EXACT24 && nnzb > 0 && bk >= 4)
     hipLaunchKernelGGL(k_tag_exact24, grid_for(nnzb), kThreads, 0, s, nnzb, (int)bk,
                        A->values.as<uint16_t>(), A->tags.as<int32_t>());
   if (hipGetLastError() != hipSuccess) return fail(smat::set_error(SMAT_ERR_HIP, "bcsr build launch failed"));
@@ -299,7 +301,7 @@ smat_status smat_bcsr_destroy(smat_bcsr A) {
 
 smat_status smat_bcsr_retag_exact24(smat_bcsr A, smat_stream stream) {
   SMAT_REQUIRE(A, "smat_bcsr_retag_exact24: null BCSR");
-  if (A->d.nnzb == 0) return SMAT_OK;
+  if (A->d.nnzb == 0 || A->d.bk < 4) return SMAT_OK;
   hipLaunchKernelGGL(k_tag_exact24, grid_for(A->d.nnzb), kThreads, 0, smat::to_hip(stream),
                      A->d.nnzb, A->d.bk, A->d.values, A->d.tags);
   SMAT_LAUNCH_CHECK();

@@ -60,7 +60,52 @@ __device__ inline uint16_t load_b(const uint16_t* __restrict__ B, int64_t ldb, i
   return B[n * ldb + k];
 }
 
-// One wave: C[16 x 16*NSUB] tile of one work item.
+// The 32 k-slots of one MFMA step starting at block p are filled by the
+// 32/BK blocks p, p+1, ...: k-slot s is column s % BK of block p + s / BK.
+// A lane of group g (= lane / 16) holds k-slots 8g .. 8g+7 of row lane % 16;
+// for BK >= 8 that is one 16-byte piece of one block row, for smaller BK it
+// is 8/BK pieces of BK halves from consecutive blocks.  Blocks at or beyond
+// `pend` contribute zeros.
+template <int BK>
+__device__ inline uint4 load_a(const uint16_t* __restrict__ values, int32_t p, int32_t pend, int r16,
+                               int grp) {
+  static_assert(BK == 1 || BK == 2 || BK == 4 || BK == 8 || BK == 16 || BK == 32, "block width");
+  if constexpr (BK >= 8) {
+    const int32_t blk = p + (8 * grp) / BK;
+    if (blk >= pend) return uint4{0, 0, 0, 0};
+    return *reinterpret_cast<const uint4*>(values + (int64_t)blk * 16 * BK + r16 * BK + (8 * grp) % BK);
+  } else if constexpr (BK == 4) {
+    uint2 w[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int32_t blk = p + 2 * grp + q;
+      w[q] = blk < pend ? *reinterpret_cast<const uint2*>(values + (int64_t)blk * 64 + r16 * 4) : uint2{0, 0};
+    }
+    return uint4{w[0].x, w[0].y, w[1].x, w[1].y};
+  } else if constexpr (BK == 2) {
+    uint32_t w[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int32_t blk = p + 4 * grp + q;
+      w[q] = blk < pend ? *reinterpret_cast<const uint32_t*>(values + (int64_t)blk * 32 + r16 * 2) : 0u;
+    }
+    return uint4{w[0], w[1], w[2], w[3]};
+  } else {
+    uint32_t w[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int32_t b0 = p + 8 * grp + 2 * q, b1 = b0 + 1;
+      const uint32_t lo = b0 < pend ? values[(int64_t)b0 * 16 + r16] : 0u;
+      const uint32_t hi = b1 < pend ? values[(int64_t)b1 * 16 + r16] : 0u;
+      w[q] = lo | (hi << 16);
+    }
+    return uint4{w[0], w[1], w[2], w[3]};
+  }
+}
+
+// One wave: C[16 x 16*NSUB] tile of one work item; B read straight into the
+// MFMA operands (per-lane 16-bit gathers for row-major B; 16-byte loads for
+// the reference layout when BK >= 8).
 template <int BK, int NSUB, int BLAYOUT, bool OUT_F16>
 __global__ __launch_bounds__(256) void k_spmm_dense(
     const Item* __restrict__ items, int64_t n_items, int32_t ntiles,
@@ -68,7 +113,7 @@ __global__ __launch_bounds__(256) void k_spmm_dense(
     const uint16_t* __restrict__ B, int64_t ldb, int64_t N, void* __restrict__ C, int64_t ldc,
     float* __restrict__ scratch) {
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
   const int64_t task = (int64_t)wg * kWaves + wave;
   const int64_t item_id = task / ntiles;
@@ -83,38 +128,32 @@ __global__ __launch_bounds__(256) void k_spmm_dense(
 
   const int r16 = lane & 15;   // A row / B column within the tile
   const int grp = lane >> 4;   // 0..3: k-slot group of 8
-  // per-lane position inside the 32-deep MFMA reduction
-  constexpr int BLK_PER_MFMA = 32 / BK;  // 2 (bk=16) or 1 (bk=32)
-  const int sub = (BLK_PER_MFMA == 2) ? (lane >> 5) : 0;          // which block of the pair
-  const int kin = (BLK_PER_MFMA == 2) ? ((grp & 1) * 8) : grp * 8;  // column offset inside block
+  constexpr int STEP = 32 / BK;  // blocks per MFMA step
 
-  for (int32_t p = it.pbeg; p < it.pend; p += BLK_PER_MFMA) {
-    const int32_t blk = p + sub;
-    const bool valid = blk < it.pend;
-    smat_h8 a;
-    int32_t c0;
-    if (valid) {
-      const uint4 raw = *reinterpret_cast<const uint4*>(values + (int64_t)blk * 16 * BK + r16 * BK + kin);
-      a = __builtin_bit_cast(smat_h8, raw);
-      c0 = colidx[blk] + kin;
-    } else {
-      a = smat_h8{0, 0, 0, 0, 0, 0, 0, 0};
-      c0 = 0;
+  for (int32_t p = it.pbeg; p < it.pend; p += STEP) {
+    const smat_h8 a = __builtin_bit_cast(smat_h8, load_a<BK>(values, p, it.pend, r16, grp));
+    // B row (and validity) of each of this lane's 8 k-slots
+    int32_t krow[8];
+    bool kok[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int s = 8 * grp + e;
+      const int32_t blk = p + s / BK;
+      kok[e] = blk < it.pend;
+      krow[e] = kok[e] ? colidx[blk] + s % BK : 0;
     }
 #pragma unroll
     for (int j = 0; j < NSUB; j++) {
       const int64_t n = n0 + 16 * j + r16;
-      smat_h8 b;
-      if (valid && n < N) {
-        if (BLAYOUT == SMAT_B_REFERENCE && (ldb % 8) == 0 && (c0 % 8) == 0) {
-          const uint4 raw = *reinterpret_cast<const uint4*>(B + n * ldb + c0);
-          b = __builtin_bit_cast(smat_h8, raw);
+      smat_h8 b = smat_h8{0, 0, 0, 0, 0, 0, 0, 0};
+      if (n < N) {
+        if (BLAYOUT == SMAT_B_REFERENCE && BK >= 8 && (ldb % 8) == 0) {
+          if (kok[0]) b = __builtin_bit_cast(smat_h8, *reinterpret_cast<const uint4*>(B + n * ldb + krow[0]));
         } else {
 #pragma unroll
-          for (int e = 0; e < 8; e++) b[e] = u16_as_h(load_b<BLAYOUT>(B, ldb, c0 + e, n));
+          for (int e = 0; e < 8; e++)
+            if (kok[e]) b[e] = u16_as_h(load_b<BLAYOUT>(B, ldb, krow[e], n));
         }
-      } else {
-        b = smat_h8{0, 0, 0, 0, 0, 0, 0, 0};
       }
       acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc[j], 0, 0, 0);
     }
@@ -148,9 +187,17 @@ __global__ __launch_bounds__(256) void k_spmm_dense(
 // ds_read_b64_tr_b16: a 16-lane group reading a 4 x 16 sub-tile (lane L
 // addressing row L/4, columns 4*(L%4)..+3) receives column L of it, i.e.
 // B[k0..k0+3][n0+L] — exactly 4 of the 8 k-values the operand layout wants.
-// Two such reads give a lane its 8 values.  The next step's global loads are
-// issued before the current step's MFMAs.  Waves never share LDS, so no
-// barrier is needed: a wave's LDS accesses complete in issue order.
+// Two such reads give a lane its 8 values.
+//
+// Pipeline (per wave, two register stages X/Y, loop unrolled by two so the
+// stage index is static): while step p is transposed and multiplied, the B
+// rows and A values of steps p+STEP and p+2*STEP are in flight.  A step's
+// block column starts are fetched one phase before its B rows, with one
+// coalesced load (lane l holds colidx[p + l % STEP]) and handed to the B-row
+// lanes by a lane shuffle; because each phase issues the column-index load
+// before the B loads, the in-order vmcnt wait for an index never drains the
+// other stage's B loads.  Waves never share LDS, so no barrier is needed: a
+// wave's LDS accesses complete in issue order.
 template <int BK, int NSUB, bool OUT_F16>
 __global__ __launch_bounds__(256) void k_spmm_lds(
     const Item* __restrict__ items, int64_t n_items, int32_t ntiles,
@@ -166,7 +213,7 @@ __global__ __launch_bounds__(256) void k_spmm_lds(
   __shared__ __attribute__((aligned(16))) uint16_t lds_all[kWaves * 32 * PITCH];
 
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
   const int64_t task = (int64_t)wg * kWaves + wave;
   const int64_t item_id = task / ntiles;
@@ -177,8 +224,6 @@ __global__ __launch_bounds__(256) void k_spmm_lds(
   uint16_t* lds = lds_all + wave * 32 * PITCH;
 
   const int r16 = lane & 15, grp = lane >> 4;
-  const int sub = (STEP == 2) ? (lane >> 5) : 0;
-  const int kin = (STEP == 2) ? ((grp & 1) * 8) : grp * 8;
   // load coordinates: instruction i covers rows i*RPL .. i*RPL+RPL-1
   const int lrow = lane / CPR, lch = lane % CPR;
   const int64_t ncol = n0 + lch * 8;
@@ -188,32 +233,34 @@ __global__ __launch_bounds__(256) void k_spmm_lds(
 #pragma unroll
   for (int j = 0; j < NSUB; j++) acc[j] = smat_f4{0.f, 0.f, 0.f, 0.f};
 
-  uint4 breg[NLD];
-  uint4 areg;
-  auto gload = [&](int32_t p) {
+  const int32_t pe = it.pend;
+  // column starts of the blocks of step p: lane l holds colidx[p + l % STEP]
+  auto cload = [&](int32_t p) -> int32_t {
+    const int32_t b = p + (lane % STEP);
+    return b < pe ? colidx[b] : 0;
+  };
+  // B rows and A values of step p (zeros past the end of the item)
+  auto issue = [&](int32_t p, int32_t cid, uint4 (&bb)[NLD], uint4& aa) {
+    if (p >= pe) return;
 #pragma unroll
     for (int i = 0; i < NLD; i++) {
       const int row = i * RPL + lrow;  // 0..31 = k-slot
-      const int32_t blk = p + row / BK;
-      if (blk < it.pend && col_ok)
-        breg[i] = *reinterpret_cast<const uint4*>(B + (int64_t)(colidx[blk] + row % BK) * ldb + ncol);
+      const int boff = row / BK;
+      const int32_t c = __shfl(cid, boff);
+      if (p + boff < pe && col_ok)
+        bb[i] = *reinterpret_cast<const uint4*>(B + (int64_t)(c + row % BK) * ldb + ncol);
       else
-        breg[i] = uint4{0, 0, 0, 0};
+        bb[i] = uint4{0, 0, 0, 0};
     }
-    const int32_t blk = p + sub;
-    if (blk < it.pend)
-      areg = *reinterpret_cast<const uint4*>(values + (int64_t)blk * 16 * BK + r16 * BK + kin);
-    else
-      areg = uint4{0, 0, 0, 0};
+    aa = load_a<BK>(values, p, pe, r16, grp);
   };
-
-  if (it.pbeg < it.pend) gload(it.pbeg);
-  for (int32_t p = it.pbeg; p < it.pend; p += STEP) {
+  auto consume = [&](const uint4 (&bb)[NLD], const uint4& aa) -> smat_h8 {
 #pragma unroll
     for (int i = 0; i < NLD; i++)
-      *reinterpret_cast<uint4*>(lds + (i * RPL + lrow) * PITCH + lch * 8) = breg[i];
-    const smat_h8 a = __builtin_bit_cast(smat_h8, areg);
-    if (p + STEP < it.pend) gload(p + STEP);
+      *reinterpret_cast<uint4*>(lds + (i * RPL + lrow) * PITCH + lch * 8) = bb[i];
+    return __builtin_bit_cast(smat_h8, aa);
+  };
+  auto compute = [&](const smat_h8& a) {
 #pragma unroll
     for (int j = 0; j < NSUB; j++) {
       const uint16_t* base = lds + (8 * grp + (r16 >> 2)) * PITCH + 16 * j + (r16 & 3) * 4;
@@ -229,6 +276,26 @@ __global__ __launch_bounds__(256) void k_spmm_lds(
       const smat_h8 b = __builtin_bit_cast(smat_h8, both);
       acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc[j], 0, 0, 0);
     }
+  };
+
+  uint4 bX[NLD], bY[NLD], aX, aY;
+  const int32_t pb = it.pbeg;
+  int32_t cX = cload(pb);
+  int32_t cY = cload(pb + STEP);
+  issue(pb, cX, bX, aX);
+  cX = cload(pb + 2 * STEP);
+  issue(pb + STEP, cY, bY, aY);
+  cY = cload(pb + 3 * STEP);
+  for (int32_t p = pb; p < pe; p += 2 * STEP) {
+    smat_h8 a = consume(bX, aX);
+    issue(p + 2 * STEP, cX, bX, aX);
+    cX = cload(p + 4 * STEP);
+    compute(a);
+    if (p + STEP >= pe) break;
+    a = consume(bY, aY);
+    issue(p + 3 * STEP, cY, bY, aY);
+    cY = cload(p + 5 * STEP);
+    compute(a);
   }
 
 #pragma unroll
@@ -403,7 +470,8 @@ smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat
                                   smat_stream stream, smat_plan* out) {
   SMAT_REQUIRE(A && out, "smat_spmm_plan_create: null argument");
   *out = nullptr;
-  SMAT_REQUIRE(A->bk == 16 || A->bk == 32, "smat_spmm_plan_create: bk must be 16 or 32");
+  SMAT_REQUIRE(A->bk == 1 || A->bk == 2 || A->bk == 4 || A->bk == 8 || A->bk == 16 || A->bk == 32,
+               "smat_spmm_plan_create: bk must be 1, 2, 4, 8, 16 or 32");
   SMAT_REQUIRE(N > 0, "smat_spmm_plan_create: N must be positive");
   SMAT_REQUIRE(A->nbrows > 0 && A->nbrows < (1 << 30), "smat_spmm_plan_create: bad block-row count");
   smat_spmm_opts o{};
@@ -433,7 +501,10 @@ smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat
     int64_t target_items = (65536 + ntl - 1) / ntl;
     int64_t s_auto = (A->nnzb + target_items - 1) / (target_items > 0 ? target_items : 1);
     S = (int32_t)(s_auto < 64 ? 64 : (s_auto > 4096 ? 4096 : s_auto));
-    S = (S + 1) & ~1;  // keep block pairs inside one slice
+  }
+  {
+    const int32_t step = 32 / A->bk;  // blocks per MFMA step: keep steps inside one slice
+    S = (S + step - 1) / step * step;
   }
   auto fail = [&](smat_status st) {
     delete P;
@@ -496,8 +567,14 @@ smat_status smat_spmm_execute(smat_plan P, const uint16_t* values24, const void*
     SMAT_REQUIRE(P->N % 8 == 0 && ldb % 8 == 0 && (reinterpret_cast<uintptr_t>(B) & 15) == 0,
                  "smat_spmm_execute: LDS staging needs N %% 8 == 0, ldb %% 8 == 0 and a 16-byte aligned B");
   hipStream_t s = smat::to_hip(stream);
-  if (P->A.bk == 16) dispatch_layout<16>(P, B, ldb, C, ldc, s);
-  else dispatch_layout<32>(P, B, ldb, C, ldc, s);
+  switch (P->A.bk) {
+    case 1: dispatch_layout<1>(P, B, ldb, C, ldc, s); break;
+    case 2: dispatch_layout<2>(P, B, ldb, C, ldc, s); break;
+    case 4: dispatch_layout<4>(P, B, ldb, C, ldc, s); break;
+    case 8: dispatch_layout<8>(P, B, ldb, C, ldc, s); break;
+    case 16: dispatch_layout<16>(P, B, ldb, C, ldc, s); break;
+    default: dispatch_layout<32>(P, B, ldb, C, ldc, s); break;
+  }
   if (P->n_red > 0) {
     const int32_t chunks = (int32_t)((16 * P->N + 255) / 256);
     const unsigned g = (unsigned)(chunks * P->n_red);

@@ -47,7 +47,7 @@ def test_error_state_without_gpu():
     assert l.smat_version().startswith(b"smat")
     # argument validation happens before any device work
     h = ctypes.c_void_p()
-    st = l.smat_bcsr_from_csr(16, 16, 0, None, None, None, 8, 0, None, ctypes.byref(h))
+    st = l.smat_bcsr_from_csr(16, 16, 0, None, None, None, 12, 0, None, ctypes.byref(h))
     assert st == smat.SMAT_ERR_INVALID
     assert b"bk" in l.smat_last_error()
 

@@ -42,7 +42,7 @@ CASES = [
 
 
 @pytest.mark.parametrize("name,make", CASES)
-@pytest.mark.parametrize("bk", [16, 32])
+@pytest.mark.parametrize("bk", [1, 2, 4, 8, 16, 32])
 def test_bcsr_build_bit_exact(name, make, bk):
     _need_gpu()
     M, K, rp, ci, v = make()
@@ -91,7 +91,7 @@ SPMM_CASES = [
 
 
 @pytest.mark.parametrize("name,make,N", SPMM_CASES)
-@pytest.mark.parametrize("bk", [16, 32])
+@pytest.mark.parametrize("bk", [1, 2, 4, 8, 16, 32])
 @pytest.mark.parametrize("n_tile", [0, 16, 32])
 @pytest.mark.parametrize("stage", [1, 2])
 def test_spmm_dense_vs_fp64(name, make, N, bk, n_tile, stage):
@@ -187,7 +187,7 @@ def test_invalid_arguments_raise():
     import smat
     M, K, rp, ci, v = gen.band(n=256, hb=8, p=0.5, seed=71)
     with pytest.raises(smat.SmatError):
-        _bcsr(M, K, rp, ci, v, bk=8)
+        _bcsr(M, K, rp, ci, v, bk=12)
     A = _bcsr(M, K, rp, ci, v)
     with pytest.raises(smat.SmatError):
         smat.SpmmPlan(A, 0)

@@ -165,12 +165,17 @@ typedef struct {
   int32_t n_tile;          /* columns per wave: 16, 32 or 64; 0 = auto */
   int32_t max_blocks_per_item; /* split block rows longer than this; 0 = auto */
   int32_t b_staging;       /* SMAT_BSTAGE_* (row-major B only) */
-  int32_t reserved[2];
+  int32_t in_dtype;        /* element type of A values and B: SMAT_F16 (0 = F16) or SMAT_BF16 */
+  int32_t reserved;
 } smat_spmm_opts;
 
 typedef struct smat_plan_t* smat_plan;
 
-/* Builds the work decomposition (on the GPU) and the scratch for split rows. */
+/*
+ * Builds the work decomposition (on the GPU) and the scratch for split rows.
+ * SMAT_KERNEL_MIXED24 needs bk = 16 and row-major B; its plan also holds the
+ * per-block-row lists of tag-1 (2:4) and tag-2 (dense) blocks.
+ */
 smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N,
                                   const smat_spmm_opts* opts, smat_stream stream,
                                   smat_plan* out);

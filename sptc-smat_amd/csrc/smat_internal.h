@@ -55,5 +55,6 @@ struct DevBuf {
 typedef _Float16 smat_h8 __attribute__((ext_vector_type(8)));
 typedef _Float16 smat_h16 __attribute__((ext_vector_type(16)));
 typedef __bf16 smat_b8 __attribute__((ext_vector_type(8)));
+typedef __bf16 smat_b16 __attribute__((ext_vector_type(16)));
 typedef float smat_f4 __attribute__((ext_vector_type(4)));
 typedef short smat_s4 __attribute__((ext_vector_type(4)));

@@ -1,28 +1,36 @@
-// BCSR x dense SpMM on gfx950 MFMA (FP16 in, FP32 accumulate).
+// BCSR x dense SpMM on gfx950 matrix cores (FP16 or BF16 in, FP32 accumulate).
 //
 // Reference semantics: mmaOBTKernelSparse (src/cuda_hgemm/src/mma/mmaOBT.cu:18-158)
-// computes C = A*B over the blocks of one block row in BCSR order.  The
-// design here is MI355X-first:
+// computes C = A*B over the blocks of one block row in BCSR order, and
+// mmaOBTSKernelSparse[_large] (mmaOBTS.cu:18-205, mmaOBTS_large.cu:19-254)
+// sends 2:4 blocks to the sparse tensor-core path.  The design here is
+// MI355X-first:
 //
 //  * One 64-lane wave owns a task = (work item, N-tile).  A work item is a
 //    block row, or a contiguous slice of a long block row.  Items come from a
-//    plan built on the GPU with a prefix sum over block-row lengths, so long
-//    rows (R-MAT hubs) are split into slices of at most `S` blocks; their
-//    partial 16 x N tiles go to a scratch area and a second kernel reduces
-//    them in slice order (deterministic, no atomics, no pre-zeroing of C).
+//    plan built on the GPU with prefix sums over block-row lengths, so long
+//    rows (R-MAT hubs) are split into slices; their partial 16 x N tiles go to
+//    a scratch area and a second kernel reduces them in slice order
+//    (deterministic, no atomics, no pre-zeroing of C).
 //
-//  * v_mfma_f32_16x16x32_f16: the 32-deep reduction of one MFMA is filled by
-//    two 16x16 blocks (bk = 16) or one 16x32 block (bk = 32).  The two blocks
-//    need not be adjacent in the matrix: lanes 0-31 carry block p, lanes
-//    32-63 block p+1, and each lane gathers the B rows of its own block.
-//    Operand layouts (lane l, element e):
+//  * v_mfma_f32_16x16x32_{f16,bf16}: the 32-deep reduction of one MFMA is
+//    filled by 32/bk blocks of width bk (bk = 1 .. 32).  The blocks need not
+//    be adjacent in the matrix: each k-slot gathers the B row of its own
+//    block column.  Operand layouts (lane l, element e):
 //        A: A[l%16][8*(l/16)+e]      B: B[8*(l/16)+e][l%16]
 //        D: D[4*(l/16)+e][l%16]
-//    A lane's 8 A values are 16 contiguous bytes of one block row, so one
-//    128-bit load per lane fetches a whole block pair (1 KiB, coalesced).
 //
-//  * N is covered by NSUB 16-column MFMA tiles per wave (N-tile = 16*NSUB).
-//    The wave keeps the A pair in registers across its NSUB tiles.
+//  * v_smfmac_f32_16x16x64_{f16,bf16} for 2:4 blocks (bk = 16): one
+//    instruction consumes four compressed 16x16 blocks.  Lane group g (lanes
+//    16g..16g+15) carries block g of the quad: 8 kept values and one 16-bit
+//    index word per row.  The matrix core pairs compressed value m of lane
+//    group g with B lane group 2*(g%2) + m/4, element 8*(g/2) + 4*((m%4)/2)
+//    + idx_m, so B lane group h needs the quad rows 8h..8h+7 (elements 0-7)
+//    and 32+8h..32+8h+7 (elements 8-15) when the quad's 64 B rows are laid
+//    out block by block.
+//
+//  * Row-major B is staged through LDS with 128-bit loads and read back as
+//    operands with ds_read_b64_tr_b16 (see k_spmm_lds).
 //
 //  * Workgroups are 4 waves; the workgroup index is remapped so that the 8
 //    XCDs (which receive workgroups round-robin) each get a contiguous range
@@ -37,9 +45,12 @@ constexpr int kWaves = 4;  // waves per workgroup
 
 struct Item {
   int32_t brow;
-  int32_t pbeg;
+  int32_t pbeg;  // dense range (BCSR positions, or positions in the dense list)
   int32_t pend;
   int32_t slot;  // -1: write C; >= 0: partial tile index in scratch
+  int32_t sbeg;  // 2:4 range (positions in the 2:4 list), mixed plans only
+  int32_t send;
+  int32_t pad0, pad1;
 };
 
 __device__ inline uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
@@ -52,10 +63,44 @@ __device__ inline uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
   return (bid % nx) * per + bid / nx;
 }
 
-__device__ inline _Float16 u16_as_h(uint16_t v) { return __builtin_bit_cast(_Float16, v); }
+// ---- element types: DT 0 = FP16, 1 = BF16; operands travel as raw words ---
+template <int DT>
+__device__ inline smat_f4 mma(uint4 a, uint4 b, smat_f4 c) {
+  if constexpr (DT == 0)
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(smat_h8, a), __builtin_bit_cast(smat_h8, b),
+                                                  c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(smat_b8, a), __builtin_bit_cast(smat_b8, b),
+                                                   c, 0, 0, 0);
+}
+
+struct U8 {
+  uint4 lo, hi;
+};
+
+template <int DT>
+__device__ inline smat_f4 smma(uint4 a, U8 b, int idx, smat_f4 c) {
+  if constexpr (DT == 0)
+    return __builtin_amdgcn_smfmac_f32_16x16x64_f16(__builtin_bit_cast(smat_h8, a), __builtin_bit_cast(smat_h16, b),
+                                                    c, idx, 0, 0);
+  else
+    return __builtin_amdgcn_smfmac_f32_16x16x64_bf16(__builtin_bit_cast(smat_b8, a),
+                                                     __builtin_bit_cast(smat_b16, b), c, idx, 0, 0);
+}
+
+// Two 4 x 16-bit transposed LDS reads -> 8 k-values of one B column.
+__device__ inline uint4 tr_pair(const uint16_t* lo_addr, const uint16_t* hi_addr) {
+  const smat_s4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((__attribute__((address_space(3))) smat_s4*)(lo_addr));
+  const smat_s4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((__attribute__((address_space(3))) smat_s4*)(hi_addr));
+  // Reinterpret the two 64-bit results as one operand.  (Extracting the 16-bit
+  // elements one by one was lowered to a splat of element 0 by this compiler;
+  // the whole-vector cast maps the loaded registers straight onto the operand.)
+  const smat_s4 both[2] = {lo, hi};
+  return __builtin_bit_cast(uint4, both);
+}
 
 template <int BLAYOUT>
-__device__ inline uint16_t load_b(const uint16_t* __restrict__ B, int64_t ldb, int64_t k, int64_t n) {
+__device__ inline uint32_t load_b(const uint16_t* __restrict__ B, int64_t ldb, int64_t k, int64_t n) {
   if (BLAYOUT == SMAT_B_ROW_MAJOR) return B[k * ldb + n];
   return B[n * ldb + k];
 }
@@ -103,10 +148,34 @@ __device__ inline uint4 load_a(const uint16_t* __restrict__ values, int32_t p, i
   }
 }
 
+// D layout: lane holds rows 4*grp + i, column 16*j + r16 of its tile.
+template <int NSUB, bool OUT_F16>
+__device__ inline void store_tile(const smat_f4 (&acc)[NSUB], const Item& it, int64_t n0, int r16, int grp,
+                                  int64_t N, void* __restrict__ C, int64_t ldc, float* __restrict__ scratch) {
+#pragma unroll
+  for (int j = 0; j < NSUB; j++) {
+    const int64_t n = n0 + 16 * j + r16;
+    if (n >= N) continue;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int row = 4 * grp + i;
+      if (it.slot >= 0) {
+        scratch[((int64_t)it.slot * 16 + row) * N + n] = acc[j][i];
+      } else {
+        const int64_t crow = (int64_t)it.brow * 16 + row;
+        if (OUT_F16)
+          reinterpret_cast<_Float16*>(C)[crow * ldc + n] = (_Float16)acc[j][i];
+        else
+          reinterpret_cast<float*>(C)[crow * ldc + n] = acc[j][i];
+      }
+    }
+  }
+}
+
 // One wave: C[16 x 16*NSUB] tile of one work item; B read straight into the
 // MFMA operands (per-lane 16-bit gathers for row-major B; 16-byte loads for
 // the reference layout when BK >= 8).
-template <int BK, int NSUB, int BLAYOUT, bool OUT_F16>
+template <int BK, int NSUB, int BLAYOUT, int DT, bool OUT_F16>
 __global__ __launch_bounds__(256) void k_spmm_dense(
     const Item* __restrict__ items, int64_t n_items, int32_t ntiles,
     const int32_t* __restrict__ colidx, const uint16_t* __restrict__ values,
@@ -131,7 +200,7 @@ __global__ __launch_bounds__(256) void k_spmm_dense(
   constexpr int STEP = 32 / BK;  // blocks per MFMA step
 
   for (int32_t p = it.pbeg; p < it.pend; p += STEP) {
-    const smat_h8 a = __builtin_bit_cast(smat_h8, load_a<BK>(values, p, it.pend, r16, grp));
+    const uint4 a = load_a<BK>(values, p, it.pend, r16, grp);
     // B row (and validity) of each of this lane's 8 k-slots
     int32_t krow[8];
     bool kok[8];
@@ -145,48 +214,33 @@ __global__ __launch_bounds__(256) void k_spmm_dense(
 #pragma unroll
     for (int j = 0; j < NSUB; j++) {
       const int64_t n = n0 + 16 * j + r16;
-      smat_h8 b = smat_h8{0, 0, 0, 0, 0, 0, 0, 0};
+      uint4 b = uint4{0, 0, 0, 0};
       if (n < N) {
         if (BLAYOUT == SMAT_B_REFERENCE && BK >= 8 && (ldb % 8) == 0) {
-          if (kok[0]) b = __builtin_bit_cast(smat_h8, *reinterpret_cast<const uint4*>(B + n * ldb + krow[0]));
+          if (kok[0]) b = *reinterpret_cast<const uint4*>(B + n * ldb + krow[0]);
         } else {
+          uint32_t w[4];
 #pragma unroll
-          for (int e = 0; e < 8; e++)
-            if (kok[e]) b[e] = u16_as_h(load_b<BLAYOUT>(B, ldb, krow[e], n));
+          for (int q = 0; q < 4; q++) {
+            const uint32_t lo = kok[2 * q] ? load_b<BLAYOUT>(B, ldb, krow[2 * q], n) : 0u;
+            const uint32_t hi = kok[2 * q + 1] ? load_b<BLAYOUT>(B, ldb, krow[2 * q + 1], n) : 0u;
+            w[q] = lo | (hi << 16);
+          }
+          b = uint4{w[0], w[1], w[2], w[3]};
         }
       }
-      acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc[j], 0, 0, 0);
+      acc[j] = mma<DT>(a, b, acc[j]);
     }
   }
-
-  // D layout: lane holds rows 4*grp + i, column 16*j + r16.
-#pragma unroll
-  for (int j = 0; j < NSUB; j++) {
-    const int64_t n = n0 + 16 * j + r16;
-    if (n >= N) continue;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int row = 4 * grp + i;
-      if (it.slot >= 0) {
-        scratch[((int64_t)it.slot * 16 + row) * N + n] = acc[j][i];
-      } else {
-        const int64_t crow = (int64_t)it.brow * 16 + row;
-        if (OUT_F16)
-          reinterpret_cast<_Float16*>(C)[crow * ldc + n] = (_Float16)acc[j][i];
-        else
-          reinterpret_cast<float*>(C)[crow * ldc + n] = acc[j][i];
-      }
-    }
-  }
+  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, C, ldc, scratch);
 }
 
-// Row-major B through LDS.  Per MFMA step the 32 B rows it reduces over
-// (two 16x16 blocks or one 16x32 block) x the wave's 16*NSUB columns are
-// fetched with 128-bit loads (each row segment is contiguous in HBM), stored
-// to a per-wave LDS tile, and read back as MFMA B operands with
-// ds_read_b64_tr_b16: a 16-lane group reading a 4 x 16 sub-tile (lane L
-// addressing row L/4, columns 4*(L%4)..+3) receives column L of it, i.e.
-// B[k0..k0+3][n0+L] — exactly 4 of the 8 k-values the operand layout wants.
+// Row-major B through LDS.  Per MFMA step the 32 B rows it reduces over x the
+// wave's 16*NSUB columns are fetched with 128-bit loads (each row segment is
+// contiguous in HBM), stored to a per-wave LDS tile, and read back as MFMA B
+// operands with ds_read_b64_tr_b16: a 16-lane group reading a 4 x 16 sub-tile
+// (lane L addressing row L/4, columns 4*(L%4)..+3) receives column L of it,
+// i.e. B[k0..k0+3][n0+L] — 4 of the 8 k-values the operand layout wants.
 // Two such reads give a lane its 8 values.
 //
 // Pipeline (per wave, two register stages X/Y, loop unrolled by two so the
@@ -198,7 +252,7 @@ __global__ __launch_bounds__(256) void k_spmm_dense(
 // before the B loads, the in-order vmcnt wait for an index never drains the
 // other stage's B loads.  Waves never share LDS, so no barrier is needed: a
 // wave's LDS accesses complete in issue order.
-template <int BK, int NSUB, bool OUT_F16>
+template <int BK, int NSUB, int DT, bool OUT_F16>
 __global__ __launch_bounds__(256) void k_spmm_lds(
     const Item* __restrict__ items, int64_t n_items, int32_t ntiles,
     const int32_t* __restrict__ colidx, const uint16_t* __restrict__ values,
@@ -254,27 +308,16 @@ __global__ __launch_bounds__(256) void k_spmm_lds(
     }
     aa = load_a<BK>(values, p, pe, r16, grp);
   };
-  auto consume = [&](const uint4 (&bb)[NLD], const uint4& aa) -> smat_h8 {
+  auto consume = [&](const uint4 (&bb)[NLD]) {
 #pragma unroll
     for (int i = 0; i < NLD; i++)
       *reinterpret_cast<uint4*>(lds + (i * RPL + lrow) * PITCH + lch * 8) = bb[i];
-    return __builtin_bit_cast(smat_h8, aa);
   };
-  auto compute = [&](const smat_h8& a) {
+  auto compute = [&](const uint4 a) {
 #pragma unroll
     for (int j = 0; j < NSUB; j++) {
       const uint16_t* base = lds + (8 * grp + (r16 >> 2)) * PITCH + 16 * j + (r16 & 3) * 4;
-      const smat_s4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
-          (__attribute__((address_space(3))) smat_s4*)(base));
-      const smat_s4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
-          (__attribute__((address_space(3))) smat_s4*)(base + 4 * PITCH));
-      // Reinterpret the two 64-bit results as one operand.  (Extracting the
-      // 16-bit elements one by one was lowered to a splat of element 0 by
-      // this compiler; the whole-vector cast maps the loaded registers
-      // straight onto the MFMA operand.)
-      const smat_s4 both[2] = {lo, hi};
-      const smat_h8 b = __builtin_bit_cast(smat_h8, both);
-      acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc[j], 0, 0, 0);
+      acc[j] = mma<DT>(a, tr_pair(base, base + 4 * PITCH), acc[j]);
     }
   };
 
@@ -287,35 +330,140 @@ __global__ __launch_bounds__(256) void k_spmm_lds(
   issue(pb + STEP, cY, bY, aY);
   cY = cload(pb + 3 * STEP);
   for (int32_t p = pb; p < pe; p += 2 * STEP) {
-    smat_h8 a = consume(bX, aX);
+    consume(bX);
+    uint4 a = aX;
     issue(p + 2 * STEP, cX, bX, aX);
     cX = cload(p + 4 * STEP);
     compute(a);
     if (p + STEP >= pe) break;
-    a = consume(bY, aY);
+    consume(bY);
+    a = aY;
     issue(p + 3 * STEP, cY, bY, aY);
     cY = cload(p + 5 * STEP);
     compute(a);
   }
+  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, C, ldc, scratch);
+}
 
+// Mixed dense / 2:4 kernel for 16x16 blocks and row-major B.  Dense blocks
+// (ids dense_ids[pbeg..pend)) go through MFMA in pairs; 2:4 blocks (ids
+// sparse_ids[sbeg..send)) through SMFMAC in quads, from the compressed arrays
+// of smat_compress_24 (SMAT_24_SMFMAC layout, indexed by BCSR block).  Both
+// accumulate into the same FP32 tile.  One register stage of prefetch.
+template <int NSUB, int DT, bool OUT_F16>
+__global__ __launch_bounds__(256) void k_spmm_mixed(
+    const Item* __restrict__ items, int64_t n_items, int32_t ntiles, const int32_t* __restrict__ colidx,
+    const uint16_t* __restrict__ values, const int32_t* __restrict__ dense_ids,
+    const int32_t* __restrict__ sparse_ids, const uint16_t* __restrict__ values24,
+    const uint16_t* __restrict__ index24, const uint16_t* __restrict__ B, int64_t ldb, int64_t N,
+    void* __restrict__ C, int64_t ldc, float* __restrict__ scratch) {
+  constexpr int NT = 16 * NSUB;
+  constexpr int CPR = NT / 8;
+  constexpr int RPL = 64 / CPR;
+  constexpr int NLD_D = 32 / RPL;  // loads per dense pair (32 rows)
+  constexpr int NLD_S = 64 / RPL;  // loads per 2:4 quad (64 rows)
+  constexpr int PITCH = NT + 8;
+  __shared__ __attribute__((aligned(16))) uint16_t lds_all[kWaves * 64 * PITCH];
+
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t task = (int64_t)wg * kWaves + wave;
+  const int64_t item_id = task / ntiles;
+  if (item_id >= n_items) return;
+  const int nt = (int)(task % ntiles);
+  const Item it = items[item_id];
+  const int64_t n0 = (int64_t)nt * NT;
+  uint16_t* lds = lds_all + wave * 64 * PITCH;
+
+  const int r16 = lane & 15, grp = lane >> 4;
+  const int lrow = lane / CPR, lch = lane % CPR;
+  const int64_t ncol = n0 + lch * 8;
+  const bool col_ok = ncol < N;
+
+  smat_f4 acc[NSUB];
 #pragma unroll
-  for (int j = 0; j < NSUB; j++) {
-    const int64_t n = n0 + 16 * j + r16;
-    if (n >= N) continue;
+  for (int j = 0; j < NSUB; j++) acc[j] = smat_f4{0.f, 0.f, 0.f, 0.f};
+
+  // ---- dense pairs
+  {
+    const int32_t pe = it.pend;
+    uint4 bb[NLD_D], aa = uint4{0, 0, 0, 0};
+    auto issue = [&](int32_t p) {
+      if (p >= pe) return;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int row = 4 * grp + i;
-      if (it.slot >= 0) {
-        scratch[((int64_t)it.slot * 16 + row) * N + n] = acc[j][i];
-      } else {
-        const int64_t crow = (int64_t)it.brow * 16 + row;
-        if (OUT_F16)
-          reinterpret_cast<_Float16*>(C)[crow * ldc + n] = (_Float16)acc[j][i];
+      for (int i = 0; i < NLD_D; i++) {
+        const int row = i * RPL + lrow;
+        const int32_t q = p + row / 16;
+        if (q < pe && col_ok)
+          bb[i] = *reinterpret_cast<const uint4*>(B + (int64_t)(colidx[dense_ids[q]] + row % 16) * ldb + ncol);
         else
-          reinterpret_cast<float*>(C)[crow * ldc + n] = acc[j][i];
+          bb[i] = uint4{0, 0, 0, 0};
+      }
+      const int32_t q = p + (lane >> 5);
+      aa = q < pe ? *reinterpret_cast<const uint4*>(values + (int64_t)dense_ids[q] * 256 + r16 * 16 + (grp & 1) * 8)
+                  : uint4{0, 0, 0, 0};
+    };
+    issue(it.pbeg);
+    for (int32_t p = it.pbeg; p < pe; p += 2) {
+#pragma unroll
+      for (int i = 0; i < NLD_D; i++)
+        *reinterpret_cast<uint4*>(lds + (i * RPL + lrow) * PITCH + lch * 8) = bb[i];
+      const uint4 a = aa;
+      issue(p + 2);
+#pragma unroll
+      for (int j = 0; j < NSUB; j++) {
+        const uint16_t* base = lds + (8 * grp + (r16 >> 2)) * PITCH + 16 * j + (r16 & 3) * 4;
+        acc[j] = mma<DT>(a, tr_pair(base, base + 4 * PITCH), acc[j]);
       }
     }
   }
+  // ---- 2:4 quads
+  {
+    const int32_t se = it.send;
+    uint4 bb[NLD_S], aa = uint4{0, 0, 0, 0};
+    int idx = 0;
+    auto issue = [&](int32_t p) {
+      if (p >= se) return;
+#pragma unroll
+      for (int i = 0; i < NLD_S; i++) {
+        const int row = i * RPL + lrow;  // 0..63: block row/16 of the quad
+        const int32_t q = p + row / 16;
+        if (q < se && col_ok)
+          bb[i] = *reinterpret_cast<const uint4*>(B + (int64_t)(colidx[sparse_ids[q]] + row % 16) * ldb + ncol);
+        else
+          bb[i] = uint4{0, 0, 0, 0};
+      }
+      const int32_t q = p + grp;  // lane group g carries block g of the quad
+      if (q < se) {
+        const int64_t id = sparse_ids[q];
+        aa = *reinterpret_cast<const uint4*>(values24 + (id * 16 + r16) * 8);
+        idx = index24[id * 16 + r16];
+      } else {
+        aa = uint4{0, 0, 0, 0};
+        idx = 0x4444;  // positions (0,1) in every group; values are zero
+      }
+    };
+    issue(it.sbeg);
+    for (int32_t p = it.sbeg; p < se; p += 4) {
+#pragma unroll
+      for (int i = 0; i < NLD_S; i++)
+        *reinterpret_cast<uint4*>(lds + (i * RPL + lrow) * PITCH + lch * 8) = bb[i];
+      const uint4 a = aa;
+      const int ix = idx;
+      issue(p + 4);
+#pragma unroll
+      for (int j = 0; j < NSUB; j++) {
+        const uint16_t* b0 = lds + (8 * grp + (r16 >> 2)) * PITCH + 16 * j + (r16 & 3) * 4;
+        const uint16_t* b1 = b0 + 32 * PITCH;
+        U8 b;
+        b.lo = tr_pair(b0, b0 + 4 * PITCH);
+        b.hi = tr_pair(b1, b1 + 4 * PITCH);
+        acc[j] = smma<DT>(a, b, ix, acc[j]);
+      }
+    }
+  }
+  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, C, ldc, scratch);
 }
 
 struct Reduce {
@@ -347,49 +495,81 @@ __global__ void k_reduce(const Reduce* __restrict__ red, int64_t n_red, int32_t 
 }
 
 // ---- planning -------------------------------------------------------------
-// Per block row: number of items (>= 1: empty rows still write zeros) and of
-// scratch slots (items if split, else 0).
-__global__ void k_plan_count(int64_t nbrows, const int32_t* __restrict__ rowptr, int32_t S,
-                             int32_t* __restrict__ n_items, int32_t* __restrict__ n_slots) {
+// A block row holds up to two lists: list 1 (dense, rp1) worked in steps of
+// step1 blocks and list 2 (2:4, rp2, may be absent) in steps of step2.  A row
+// whose total step count exceeds S is cut into k slices, each taking the
+// same share of both lists.  Empty rows still get one item (it writes zeros).
+__device__ inline int32_t ceil_div(int32_t a, int32_t b) { return (a + b - 1) / b; }
+
+__global__ void k_plan_count(int64_t nbrows, const int32_t* __restrict__ rp1, int32_t step1,
+                             const int32_t* __restrict__ rp2, int32_t step2, int32_t S,
+                             int32_t* __restrict__ n_items, int32_t* __restrict__ n_slots,
+                             int32_t* __restrict__ split_flag) {
   int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r > nbrows) return;
   if (r == nbrows) {
     n_items[r] = 0;
     n_slots[r] = 0;
+    split_flag[r] = 0;
     return;
   }
-  int32_t nb = rowptr[r + 1] - rowptr[r];
-  int32_t k = nb <= S ? 1 : (nb + S - 1) / S;
+  const int32_t n1 = rp1[r + 1] - rp1[r];
+  const int32_t n2 = rp2 ? rp2[r + 1] - rp2[r] : 0;
+  const int32_t steps = ceil_div(n1, step1) + ceil_div(n2, step2);
+  const int32_t k = steps <= S ? 1 : ceil_div(steps, S);
   n_items[r] = k;
   n_slots[r] = k > 1 ? k : 0;
+  split_flag[r] = k > 1 ? 1 : 0;
 }
 
-__global__ void k_plan_emit(int64_t nbrows, const int32_t* __restrict__ rowptr, int32_t S,
-                            const int32_t* __restrict__ item_off, const int32_t* __restrict__ slot_off,
-                            const int32_t* __restrict__ red_flag_off, Item* __restrict__ items,
-                            Reduce* __restrict__ red) {
+__global__ void k_plan_emit(int64_t nbrows, const int32_t* __restrict__ rp1, int32_t step1,
+                            const int32_t* __restrict__ rp2, int32_t step2, const int32_t* __restrict__ item_off,
+                            const int32_t* __restrict__ slot_off, const int32_t* __restrict__ red_off,
+                            Item* __restrict__ items, Reduce* __restrict__ red) {
   int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nbrows) return;
-  int32_t pb = rowptr[r], pe = rowptr[r + 1];
-  int32_t k = item_off[r + 1] - item_off[r];
-  int32_t base = item_off[r];
+  const int32_t b1 = rp1[r], e1 = rp1[r + 1];
+  const int32_t b2 = rp2 ? rp2[r] : 0, e2 = rp2 ? rp2[r + 1] : 0;
+  const int32_t k = item_off[r + 1] - item_off[r];
+  const int32_t base = item_off[r];
   if (k == 1) {
-    items[base] = Item{(int32_t)r, pb, pe, -1};
+    items[base] = Item{(int32_t)r, b1, e1, -1, b2, e2, 0, 0};
     return;
   }
-  int32_t sbase = slot_off[r];
+  const int32_t per1 = ceil_div(ceil_div(e1 - b1, step1), k) * step1;
+  const int32_t per2 = ceil_div(ceil_div(e2 - b2, step2), k) * step2;
+  const int32_t sbase = slot_off[r];
   for (int32_t q = 0; q < k; q++) {
-    int32_t b0 = pb + q * S;
-    int32_t b1 = b0 + S < pe ? b0 + S : pe;
-    items[base + q] = Item{(int32_t)r, b0, b1, sbase + q};
+    const int32_t d0 = min(b1 + q * per1, e1), d1 = min(d0 + per1, e1);
+    const int32_t s0 = min(b2 + q * per2, e2), s1 = min(s0 + per2, e2);
+    items[base + q] = Item{(int32_t)r, d0, d1, sbase + q, s0, s1, 0, 0};
   }
-  red[red_flag_off[r]] = Reduce{(int32_t)r, sbase, k, 0};
+  red[red_off[r]] = Reduce{(int32_t)r, sbase, k, 0};
 }
 
-__global__ void k_split_flag(int64_t nbrows, const int32_t* __restrict__ n_slots, int32_t* __restrict__ f) {
-  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r > nbrows) return;
-  f[r] = (r < nbrows && n_slots[r] > 0) ? 1 : 0;
+// Mixed plans: per-block flags and list positions of tag-2 and tag-1 blocks.
+__global__ void k_list_flags(int64_t nnzb, const int32_t* __restrict__ tags, int32_t* __restrict__ fd,
+                             int32_t* __restrict__ fs) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nnzb) return;
+  const int t = b < nnzb ? tags[b] : 0;
+  fd[b] = t == 2;
+  fs[b] = t == 1;
+}
+
+__global__ void k_list_emit(int64_t nnzb, int64_t nbrows, const int32_t* __restrict__ tags,
+                            const int32_t* __restrict__ rowptr, const int32_t* __restrict__ pd,
+                            const int32_t* __restrict__ ps, int32_t* __restrict__ dense_ids,
+                            int32_t* __restrict__ sparse_ids, int32_t* __restrict__ drp,
+                            int32_t* __restrict__ srp) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b <= nbrows) {
+    drp[b] = pd[rowptr[b]];
+    srp[b] = ps[rowptr[b]];
+  }
+  if (b >= nnzb) return;
+  if (tags[b] == 2) dense_ids[pd[b]] = (int32_t)b;
+  else if (tags[b] == 1) sparse_ids[ps[b]] = (int32_t)b;
 }
 
 inline unsigned blocks_for(int64_t n, int t) {
@@ -404,62 +584,171 @@ struct smat_plan_t {
   int64_t N = 0;
   smat_spmm_opts opts{};
   int32_t nsub = 4;
+  bool mixed = false;
   int64_t n_items = 0, n_red = 0, n_slots = 0;
-  smat::DevBuf items, red, scratch;
+  int64_t n_dense = 0, n_sparse = 0;
+  smat::DevBuf items, red, scratch, dense_ids, sparse_ids;
 };
 
 namespace {
 
-template <int BK, int NSUB, int BLAYOUT, bool OUT_F16>
-void launch_dense(const smat_plan_t* P, const uint16_t* B, int64_t ldb, void* C, int64_t ldc,
-                  hipStream_t s) {
-  const int32_t ntiles = (int32_t)((P->N + 16 * NSUB - 1) / (16 * NSUB));
-  const int64_t tasks = P->n_items * ntiles;
-  const unsigned grid = blocks_for(tasks, kWaves);
-  hipLaunchKernelGGL((k_spmm_dense<BK, NSUB, BLAYOUT, OUT_F16>), grid, 64 * kWaves, 0, s,
-                     P->items.as<Item>(), P->n_items, ntiles, P->A.colidx, P->A.values, B, ldb, P->N,
-                     C, ldc, P->scratch.as<float>());
+inline int32_t ntiles_of(const smat_plan_t* P, int nsub) { return (int32_t)((P->N + 16 * nsub - 1) / (16 * nsub)); }
+
+template <int BK, int NSUB, int BLAYOUT, int DT, bool OUT_F16>
+void launch_dense(const smat_plan_t* P, const uint16_t* B, int64_t ldb, void* C, int64_t ldc, hipStream_t s) {
+  const int32_t ntiles = ntiles_of(P, NSUB);
+  hipLaunchKernelGGL((k_spmm_dense<BK, NSUB, BLAYOUT, DT, OUT_F16>), blocks_for(P->n_items * ntiles, kWaves),
+                     64 * kWaves, 0, s, P->items.as<Item>(), P->n_items, ntiles, P->A.colidx, P->A.values, B,
+                     ldb, P->N, C, ldc, P->scratch.as<float>());
 }
 
-template <int BK, int NSUB, bool OUT_F16>
+template <int BK, int NSUB, int DT, bool OUT_F16>
 void launch_lds(const smat_plan_t* P, const uint16_t* B, int64_t ldb, void* C, int64_t ldc, hipStream_t s) {
-  const int32_t ntiles = (int32_t)((P->N + 16 * NSUB - 1) / (16 * NSUB));
-  const int64_t tasks = P->n_items * ntiles;
-  const unsigned grid = blocks_for(tasks, kWaves);
-  hipLaunchKernelGGL((k_spmm_lds<BK, NSUB, OUT_F16>), grid, 64 * kWaves, 0, s, P->items.as<Item>(),
-                     P->n_items, ntiles, P->A.colidx, P->A.values, B, ldb, P->N, C, ldc,
+  const int32_t ntiles = ntiles_of(P, NSUB);
+  hipLaunchKernelGGL((k_spmm_lds<BK, NSUB, DT, OUT_F16>), blocks_for(P->n_items * ntiles, kWaves), 64 * kWaves,
+                     0, s, P->items.as<Item>(), P->n_items, ntiles, P->A.colidx, P->A.values, B, ldb, P->N, C,
+                     ldc, P->scratch.as<float>());
+}
+
+template <int NSUB, int DT, bool OUT_F16>
+void launch_mixed(const smat_plan_t* P, const uint16_t* v24, const uint16_t* i24, const uint16_t* B, int64_t ldb,
+                  void* C, int64_t ldc, hipStream_t s) {
+  const int32_t ntiles = ntiles_of(P, NSUB);
+  hipLaunchKernelGGL((k_spmm_mixed<NSUB, DT, OUT_F16>), blocks_for(P->n_items * ntiles, kWaves), 64 * kWaves, 0,
+                     s, P->items.as<Item>(), P->n_items, ntiles, P->A.colidx, P->A.values,
+                     P->dense_ids.as<int32_t>(), P->sparse_ids.as<int32_t>(), v24, i24, B, ldb, P->N, C, ldc,
                      P->scratch.as<float>());
 }
 
-template <int BK, int BLAYOUT, bool OUT_F16>
-void dispatch_nsub(const smat_plan_t* P, const uint16_t* B, int64_t ldb, void* C, int64_t ldc,
-                   hipStream_t s) {
+template <int BK, int BLAYOUT, int DT, bool OUT_F16>
+void dispatch_nsub(const smat_plan_t* P, const uint16_t* B, int64_t ldb, void* C, int64_t ldc, hipStream_t s) {
   const bool lds_ok = (P->N % 8 == 0) && (ldb % 8 == 0) && ((reinterpret_cast<uintptr_t>(B) & 15) == 0);
   if (BLAYOUT == SMAT_B_ROW_MAJOR && P->opts.b_staging != SMAT_BSTAGE_DIRECT && lds_ok) {
     switch (P->nsub) {
-      case 1: launch_lds<BK, 1, OUT_F16>(P, B, ldb, C, ldc, s); return;
-      case 2: launch_lds<BK, 2, OUT_F16>(P, B, ldb, C, ldc, s); return;
-      default: launch_lds<BK, 4, OUT_F16>(P, B, ldb, C, ldc, s); return;
+      case 1: launch_lds<BK, 1, DT, OUT_F16>(P, B, ldb, C, ldc, s); return;
+      case 2: launch_lds<BK, 2, DT, OUT_F16>(P, B, ldb, C, ldc, s); return;
+      default: launch_lds<BK, 4, DT, OUT_F16>(P, B, ldb, C, ldc, s); return;
     }
   }
   switch (P->nsub) {
-    case 1: launch_dense<BK, 1, BLAYOUT, OUT_F16>(P, B, ldb, C, ldc, s); break;
-    case 2: launch_dense<BK, 2, BLAYOUT, OUT_F16>(P, B, ldb, C, ldc, s); break;
-    default: launch_dense<BK, 4, BLAYOUT, OUT_F16>(P, B, ldb, C, ldc, s); break;
+    case 1: launch_dense<BK, 1, BLAYOUT, DT, OUT_F16>(P, B, ldb, C, ldc, s); break;
+    case 2: launch_dense<BK, 2, BLAYOUT, DT, OUT_F16>(P, B, ldb, C, ldc, s); break;
+    default: launch_dense<BK, 4, BLAYOUT, DT, OUT_F16>(P, B, ldb, C, ldc, s); break;
   }
 }
 
-template <int BK>
-void dispatch_layout(const smat_plan_t* P, const uint16_t* B, int64_t ldb, void* C, int64_t ldc,
-                     hipStream_t s) {
+template <int BK, int DT>
+void dispatch_layout(const smat_plan_t* P, const uint16_t* B, int64_t ldb, void* C, int64_t ldc, hipStream_t s) {
   const bool f16 = P->opts.out_dtype == SMAT_F16;
   if (P->opts.b_layout == SMAT_B_REFERENCE) {
-    if (f16) dispatch_nsub<BK, SMAT_B_REFERENCE, true>(P, B, ldb, C, ldc, s);
-    else dispatch_nsub<BK, SMAT_B_REFERENCE, false>(P, B, ldb, C, ldc, s);
+    if (f16) dispatch_nsub<BK, SMAT_B_REFERENCE, DT, true>(P, B, ldb, C, ldc, s);
+    else dispatch_nsub<BK, SMAT_B_REFERENCE, DT, false>(P, B, ldb, C, ldc, s);
   } else {
-    if (f16) dispatch_nsub<BK, SMAT_B_ROW_MAJOR, true>(P, B, ldb, C, ldc, s);
-    else dispatch_nsub<BK, SMAT_B_ROW_MAJOR, false>(P, B, ldb, C, ldc, s);
+    if (f16) dispatch_nsub<BK, SMAT_B_ROW_MAJOR, DT, true>(P, B, ldb, C, ldc, s);
+    else dispatch_nsub<BK, SMAT_B_ROW_MAJOR, DT, false>(P, B, ldb, C, ldc, s);
   }
+}
+
+template <int DT>
+void dispatch_bk(const smat_plan_t* P, const uint16_t* B, int64_t ldb, void* C, int64_t ldc, hipStream_t s) {
+  switch (P->A.bk) {
+    case 1: dispatch_layout<1, DT>(P, B, ldb, C, ldc, s); break;
+    case 2: dispatch_layout<2, DT>(P, B, ldb, C, ldc, s); break;
+    case 4: dispatch_layout<4, DT>(P, B, ldb, C, ldc, s); break;
+    case 8: dispatch_layout<8, DT>(P, B, ldb, C, ldc, s); break;
+    case 16: dispatch_layout<16, DT>(P, B, ldb, C, ldc, s); break;
+    default: dispatch_layout<32, DT>(P, B, ldb, C, ldc, s); break;
+  }
+}
+
+template <int DT>
+void dispatch_mixed(const smat_plan_t* P, const uint16_t* v24, const uint16_t* i24, const uint16_t* B, int64_t ldb,
+                    void* C, int64_t ldc, hipStream_t s) {
+  const bool f16 = P->opts.out_dtype == SMAT_F16;
+  switch (P->nsub) {
+    case 1: f16 ? launch_mixed<1, DT, true>(P, v24, i24, B, ldb, C, ldc, s)
+                : launch_mixed<1, DT, false>(P, v24, i24, B, ldb, C, ldc, s); break;
+    case 2: f16 ? launch_mixed<2, DT, true>(P, v24, i24, B, ldb, C, ldc, s)
+                : launch_mixed<2, DT, false>(P, v24, i24, B, ldb, C, ldc, s); break;
+    default: f16 ? launch_mixed<4, DT, true>(P, v24, i24, B, ldb, C, ldc, s)
+                 : launch_mixed<4, DT, false>(P, v24, i24, B, ldb, C, ldc, s); break;
+  }
+}
+
+// Exclusive scan helper over n int32 with a reusable temp buffer.
+smat_status exscan(smat::DevBuf& tmp, const int32_t* in, int32_t* out, int64_t n, hipStream_t s) {
+  size_t tb = 0;
+  SMAT_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, (int)n, s));
+  if (tb > tmp.bytes) SMAT_TRY(tmp.alloc(tb));
+  tb = tmp.bytes;
+  SMAT_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, in, out, (int)n, s));
+  return SMAT_OK;
+}
+
+smat_status build_items(smat_plan_t* P, const int32_t* rp1, int32_t step1, const int32_t* rp2, int32_t step2,
+                        int32_t S, hipStream_t s) {
+  const int64_t nbrows = P->A.nbrows, nb1 = nbrows + 1;
+  smat::DevBuf cnt_i, cnt_s, flag, off_i, off_s, off_f, tmp;
+  SMAT_TRY(cnt_i.alloc(nb1 * 4));
+  SMAT_TRY(cnt_s.alloc(nb1 * 4));
+  SMAT_TRY(flag.alloc(nb1 * 4));
+  SMAT_TRY(off_i.alloc(nb1 * 4));
+  SMAT_TRY(off_s.alloc(nb1 * 4));
+  SMAT_TRY(off_f.alloc(nb1 * 4));
+  hipLaunchKernelGGL(k_plan_count, blocks_for(nb1, 256), 256, 0, s, nbrows, rp1, step1, rp2, step2, S,
+                     cnt_i.as<int32_t>(), cnt_s.as<int32_t>(), flag.as<int32_t>());
+  SMAT_LAUNCH_CHECK();
+  SMAT_TRY(exscan(tmp, cnt_i.as<int32_t>(), off_i.as<int32_t>(), nb1, s));
+  SMAT_TRY(exscan(tmp, cnt_s.as<int32_t>(), off_s.as<int32_t>(), nb1, s));
+  SMAT_TRY(exscan(tmp, flag.as<int32_t>(), off_f.as<int32_t>(), nb1, s));
+  int32_t tot[3] = {0, 0, 0};
+  SMAT_HIP_TRY(hipMemcpyAsync(&tot[0], off_i.as<int32_t>() + nbrows, 4, hipMemcpyDeviceToHost, s));
+  SMAT_HIP_TRY(hipMemcpyAsync(&tot[1], off_s.as<int32_t>() + nbrows, 4, hipMemcpyDeviceToHost, s));
+  SMAT_HIP_TRY(hipMemcpyAsync(&tot[2], off_f.as<int32_t>() + nbrows, 4, hipMemcpyDeviceToHost, s));
+  SMAT_HIP_TRY(hipStreamSynchronize(s));
+  P->n_items = tot[0];
+  P->n_slots = tot[1];
+  P->n_red = tot[2];
+  SMAT_TRY(P->items.alloc(P->n_items * sizeof(Item)));
+  SMAT_TRY(P->red.alloc(P->n_red * sizeof(Reduce)));
+  SMAT_TRY(P->scratch.alloc(P->n_slots * 16 * P->N * sizeof(float)));
+  hipLaunchKernelGGL(k_plan_emit, blocks_for(nbrows, 256), 256, 0, s, nbrows, rp1, step1, rp2, step2,
+                     off_i.as<int32_t>(), off_s.as<int32_t>(), off_f.as<int32_t>(), P->items.as<Item>(),
+                     P->red.as<Reduce>());
+  SMAT_LAUNCH_CHECK();
+  SMAT_HIP_TRY(hipStreamSynchronize(s));
+  return SMAT_OK;
+}
+
+// Mixed plans: dense and 2:4 block-id lists grouped by block row, then items.
+smat_status build_mixed(smat_plan_t* P, int32_t S, hipStream_t s) {
+  const smat_bcsr_desc& A = P->A;
+  const int64_t n = A.nnzb, nb1 = A.nbrows + 1;
+  smat::DevBuf fd, fs, pd, ps, drp, srp, tmp;
+  SMAT_TRY(fd.alloc((n + 1) * 4));
+  SMAT_TRY(fs.alloc((n + 1) * 4));
+  SMAT_TRY(pd.alloc((n + 1) * 4));
+  SMAT_TRY(ps.alloc((n + 1) * 4));
+  SMAT_TRY(drp.alloc(nb1 * 4));
+  SMAT_TRY(srp.alloc(nb1 * 4));
+  hipLaunchKernelGGL(k_list_flags, blocks_for(n + 1, 256), 256, 0, s, n, A.tags, fd.as<int32_t>(), fs.as<int32_t>());
+  SMAT_LAUNCH_CHECK();
+  SMAT_TRY(exscan(tmp, fd.as<int32_t>(), pd.as<int32_t>(), n + 1, s));
+  SMAT_TRY(exscan(tmp, fs.as<int32_t>(), ps.as<int32_t>(), n + 1, s));
+  int32_t cnt[2] = {0, 0};
+  SMAT_HIP_TRY(hipMemcpyAsync(&cnt[0], pd.as<int32_t>() + n, 4, hipMemcpyDeviceToHost, s));
+  SMAT_HIP_TRY(hipMemcpyAsync(&cnt[1], ps.as<int32_t>() + n, 4, hipMemcpyDeviceToHost, s));
+  SMAT_HIP_TRY(hipStreamSynchronize(s));
+  P->n_dense = cnt[0];
+  P->n_sparse = cnt[1];
+  SMAT_TRY(P->dense_ids.alloc(P->n_dense * 4));
+  SMAT_TRY(P->sparse_ids.alloc(P->n_sparse * 4));
+  const int64_t m = n > A.nbrows ? n : A.nbrows + 1;
+  hipLaunchKernelGGL(k_list_emit, blocks_for(m, 256), 256, 0, s, n, A.nbrows, A.tags, A.rowptr, pd.as<int32_t>(),
+                     ps.as<int32_t>(), P->dense_ids.as<int32_t>(), P->sparse_ids.as<int32_t>(),
+                     drp.as<int32_t>(), srp.as<int32_t>());
+  SMAT_LAUNCH_CHECK();
+  return build_items(P, drp.as<int32_t>(), 2, srp.as<int32_t>(), 4, S, s);
 }
 
 }  // namespace
@@ -477,11 +766,16 @@ smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat
   smat_spmm_opts o{};
   if (opts) o = *opts;
   if (o.out_dtype == 0) o.out_dtype = SMAT_F32;
+  if (o.in_dtype == 0) o.in_dtype = SMAT_F16;
   SMAT_REQUIRE(o.out_dtype == SMAT_F32 || o.out_dtype == SMAT_F16, "smat_spmm_plan_create: out_dtype");
+  SMAT_REQUIRE(o.in_dtype == SMAT_F16 || o.in_dtype == SMAT_BF16, "smat_spmm_plan_create: in_dtype");
   SMAT_REQUIRE(o.b_layout == SMAT_B_ROW_MAJOR || o.b_layout == SMAT_B_REFERENCE,
                "smat_spmm_plan_create: b_layout");
-  SMAT_REQUIRE(o.kernel == SMAT_KERNEL_AUTO || o.kernel == SMAT_KERNEL_DENSE,
-               "smat_spmm_plan_create: kernel %d not available in this build", o.kernel);
+  SMAT_REQUIRE(o.kernel == SMAT_KERNEL_AUTO || o.kernel == SMAT_KERNEL_DENSE || o.kernel == SMAT_KERNEL_MIXED24,
+               "smat_spmm_plan_create: unknown kernel %d", o.kernel);
+  if (o.kernel == SMAT_KERNEL_MIXED24)
+    SMAT_REQUIRE(A->bk == 16 && o.b_layout == SMAT_B_ROW_MAJOR,
+                 "smat_spmm_plan_create: the 2:4 kernel needs bk = 16 and row-major B");
   SMAT_REQUIRE(o.n_tile == 0 || o.n_tile == 16 || o.n_tile == 32 || o.n_tile == 64,
                "smat_spmm_plan_create: n_tile must be 0, 16, 32 or 64");
   SMAT_REQUIRE(o.b_staging >= SMAT_BSTAGE_AUTO && o.b_staging <= SMAT_BSTAGE_LDS,
@@ -492,88 +786,54 @@ smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat
   P->A = *A;
   P->N = N;
   P->opts = o;
-  int nt = o.n_tile ? o.n_tile : (N >= 64 ? 64 : (N > 16 ? 32 : 16));
+  P->mixed = o.kernel == SMAT_KERNEL_MIXED24;
+  const int nt = o.n_tile ? o.n_tile : (N >= 64 ? 64 : (N > 16 ? 32 : 16));
   P->nsub = nt / 16;
-  int32_t S = o.max_blocks_per_item;
+  const int32_t step = 32 / A->bk;  // blocks per MFMA step
+  int32_t S = o.max_blocks_per_item;  // in blocks
   if (S <= 0) {
     // aim for >= 64k wave tasks, never below 64 blocks per slice
-    int64_t ntl = (N + nt - 1) / nt;
-    int64_t target_items = (65536 + ntl - 1) / ntl;
-    int64_t s_auto = (A->nnzb + target_items - 1) / (target_items > 0 ? target_items : 1);
+    const int64_t ntl = (N + nt - 1) / nt;
+    const int64_t target_items = (65536 + ntl - 1) / ntl;
+    const int64_t s_auto = (A->nnzb + target_items - 1) / target_items;
     S = (int32_t)(s_auto < 64 ? 64 : (s_auto > 4096 ? 4096 : s_auto));
   }
-  {
-    const int32_t step = 32 / A->bk;  // blocks per MFMA step: keep steps inside one slice
-    S = (S + step - 1) / step * step;
-  }
-  auto fail = [&](smat_status st) {
+  smat_status st;
+  if (P->mixed)
+    st = build_mixed(P, (S + 1) / 2, s);  // S in MFMA steps (a dense pair or a 2:4 quad each)
+  else
+    st = build_items(P, A->rowptr, step, nullptr, 1, (S + step - 1) / step, s);
+  if (st != SMAT_OK) {
     delete P;
     return st;
-  };
-  const int64_t nb1 = A->nbrows + 1;
-  smat::DevBuf cnt_i, cnt_s, off_i, off_s, flag, off_f, tmp;
-  smat_status st;
-  if ((st = cnt_i.alloc(nb1 * 4)) || (st = cnt_s.alloc(nb1 * 4)) || (st = off_i.alloc(nb1 * 4)) ||
-      (st = off_s.alloc(nb1 * 4)) || (st = flag.alloc(nb1 * 4)) || (st = off_f.alloc(nb1 * 4)))
-    return fail(st);
-  hipLaunchKernelGGL(k_plan_count, blocks_for(nb1, 256), 256, 0, s, A->nbrows, A->rowptr, S,
-                     cnt_i.as<int32_t>(), cnt_s.as<int32_t>());
-  hipLaunchKernelGGL(k_split_flag, blocks_for(nb1, 256), 256, 0, s, A->nbrows, cnt_s.as<int32_t>(),
-                     flag.as<int32_t>());
-  size_t tb = 0;
-  if (hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt_i.as<int32_t>(), off_i.as<int32_t>(), (int)nb1, s) !=
-      hipSuccess)
-    return fail(smat::set_error(SMAT_ERR_HIP, "plan scan sizing failed"));
-  if ((st = tmp.alloc(tb))) return fail(st);
-  if (hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, cnt_i.as<int32_t>(), off_i.as<int32_t>(), (int)nb1, s) !=
-          hipSuccess ||
-      hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, cnt_s.as<int32_t>(), off_s.as<int32_t>(), (int)nb1, s) !=
-          hipSuccess ||
-      hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, flag.as<int32_t>(), off_f.as<int32_t>(), (int)nb1, s) !=
-          hipSuccess)
-    return fail(smat::set_error(SMAT_ERR_HIP, "plan scan failed"));
-  int32_t tot[3] = {0, 0, 0};
-  if (hipMemcpyAsync(&tot[0], off_i.as<int32_t>() + A->nbrows, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
-      hipMemcpyAsync(&tot[1], off_s.as<int32_t>() + A->nbrows, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
-      hipMemcpyAsync(&tot[2], off_f.as<int32_t>() + A->nbrows, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
-      hipStreamSynchronize(s) != hipSuccess)
-    return fail(smat::set_error(SMAT_ERR_HIP, "plan readback failed"));
-  P->n_items = tot[0];
-  P->n_slots = tot[1];
-  P->n_red = tot[2];
-  if ((st = P->items.alloc(P->n_items * sizeof(Item))) || (st = P->red.alloc(P->n_red * sizeof(Reduce))) ||
-      (st = P->scratch.alloc(P->n_slots * 16 * N * sizeof(float))))
-    return fail(st);
-  hipLaunchKernelGGL(k_plan_emit, blocks_for(A->nbrows, 256), 256, 0, s, A->nbrows, A->rowptr, S,
-                     off_i.as<int32_t>(), off_s.as<int32_t>(), off_f.as<int32_t>(), P->items.as<Item>(),
-                     P->red.as<Reduce>());
-  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
-    return fail(smat::set_error(SMAT_ERR_HIP, "plan emit failed"));
+  }
   *out = P;
   return SMAT_OK;
 }
 
 smat_status smat_spmm_execute(smat_plan P, const uint16_t* values24, const void* index24, const uint16_t* B,
                               int64_t ldb, void* C, int64_t ldc, smat_stream stream) {
-  (void)values24;
-  (void)index24;
   SMAT_REQUIRE(P && B && C, "smat_spmm_execute: null argument");
   SMAT_REQUIRE(ldc >= P->N, "smat_spmm_execute: ldc < N");
   if (P->opts.b_layout == SMAT_B_ROW_MAJOR)
     SMAT_REQUIRE(ldb >= P->N, "smat_spmm_execute: ldb < N for row-major B");
   else
     SMAT_REQUIRE(ldb >= P->A.K, "smat_spmm_execute: ldb < K for reference-layout B");
-  if (P->opts.b_staging == SMAT_BSTAGE_LDS && P->opts.b_layout == SMAT_B_ROW_MAJOR)
-    SMAT_REQUIRE(P->N % 8 == 0 && ldb % 8 == 0 && (reinterpret_cast<uintptr_t>(B) & 15) == 0,
-                 "smat_spmm_execute: LDS staging needs N %% 8 == 0, ldb %% 8 == 0 and a 16-byte aligned B");
+  const bool aligned = P->N % 8 == 0 && ldb % 8 == 0 && (reinterpret_cast<uintptr_t>(B) & 15) == 0;
+  if ((P->opts.b_staging == SMAT_BSTAGE_LDS && P->opts.b_layout == SMAT_B_ROW_MAJOR) || P->mixed)
+    SMAT_REQUIRE(aligned, "smat_spmm_execute: LDS staging needs N %% 8 == 0, ldb %% 8 == 0 and a 16-byte aligned B");
+  if (P->mixed)
+    SMAT_REQUIRE(values24 && index24 && (reinterpret_cast<uintptr_t>(values24) & 15) == 0,
+                 "smat_spmm_execute: the 2:4 kernel needs 16-byte aligned values24 and index24");
   hipStream_t s = smat::to_hip(stream);
-  switch (P->A.bk) {
-    case 1: dispatch_layout<1>(P, B, ldb, C, ldc, s); break;
-    case 2: dispatch_layout<2>(P, B, ldb, C, ldc, s); break;
-    case 4: dispatch_layout<4>(P, B, ldb, C, ldc, s); break;
-    case 8: dispatch_layout<8>(P, B, ldb, C, ldc, s); break;
-    case 16: dispatch_layout<16>(P, B, ldb, C, ldc, s); break;
-    default: dispatch_layout<32>(P, B, ldb, C, ldc, s); break;
+  const bool bf16 = P->opts.in_dtype == SMAT_BF16;
+  if (P->mixed) {
+    const uint16_t* i24 = static_cast<const uint16_t*>(index24);
+    if (bf16) dispatch_mixed<1>(P, values24, i24, B, ldb, C, ldc, s);
+    else dispatch_mixed<0>(P, values24, i24, B, ldb, C, ldc, s);
+  } else {
+    if (bf16) dispatch_bk<1>(P, B, ldb, C, ldc, s);
+    else dispatch_bk<0>(P, B, ldb, C, ldc, s);
   }
   if (P->n_red > 0) {
     const int32_t chunks = (int32_t)((16 * P->N + 255) / 256);

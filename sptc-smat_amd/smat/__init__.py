@@ -46,7 +46,7 @@ class BcsrDesc(ctypes.Structure):
 class SpmmOpts(ctypes.Structure):
     _fields_ = [
         ("kernel", _i32), ("out_dtype", _i32), ("b_layout", _i32), ("n_tile", _i32),
-        ("max_blocks_per_item", _i32), ("b_staging", _i32), ("reserved", _i32 * 2),
+        ("max_blocks_per_item", _i32), ("b_staging", _i32), ("in_dtype", _i32), ("reserved", _i32),
     ]
 
 
@@ -223,6 +223,14 @@ def copy_to_host(dst_host: int, src_dev: int, nbytes: int) -> None:
         raise RuntimeError(f"hipMemcpy D2H failed: {ret}")
 
 
+def copy_to_device(dst_dev: int, src_host: int, nbytes: int) -> None:
+    """Synchronous host-to-device copy into library-owned memory (hipMemcpy)."""
+    copy_to_host(0, 0, 0)  # loads the runtime binding
+    ret = _hip.hipMemcpy(dst_dev, src_host, nbytes, 1)  # hipMemcpyHostToDevice
+    if ret != 0:
+        raise RuntimeError(f"hipMemcpy H2D failed: {ret}")
+
+
 @dataclass
 class PlanInfo:
     n_items: int
@@ -233,10 +241,10 @@ class PlanInfo:
 class SpmmPlan:
     def __init__(self, A: Bcsr | BcsrDesc, N: int, *, kernel: int = SMAT_KERNEL_AUTO,
                  out_dtype: int = SMAT_F32, b_layout: int = SMAT_B_ROW_MAJOR, n_tile: int = 0,
-                 max_blocks_per_item: int = 0, b_staging: int = 0, stream=None):
+                 max_blocks_per_item: int = 0, b_staging: int = 0, in_dtype: int = SMAT_F16, stream=None):
         desc = A.desc if isinstance(A, Bcsr) else A
         self._keep = A
-        o = SpmmOpts(kernel, out_dtype, b_layout, n_tile, max_blocks_per_item, b_staging)
+        o = SpmmOpts(kernel, out_dtype, b_layout, n_tile, max_blocks_per_item, b_staging, in_dtype)
         h = _p()
         check(lib().smat_spmm_plan_create(ctypes.byref(desc), N, ctypes.byref(o), stream_ptr(stream),
                                           ctypes.byref(h)))

@@ -133,6 +133,17 @@ def dense_b(K: int, N: int, seed: int = 7) -> np.ndarray:
     return rng.uniform(-1.0, 1.0, size=(K, N)).astype(np.float16).view(np.uint16)
 
 
+def f32_to_bf16_bits(x: np.ndarray) -> np.ndarray:
+    """FP32 -> BF16 bits, round to nearest even (finite inputs)."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    r = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return r.astype(np.uint16)
+
+
+def bf16_bits_to_f32(b: np.ndarray) -> np.ndarray:
+    return (np.ascontiguousarray(b, np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
 def csr_to_dense(M, K, rowptr, colidx, values_u16) -> np.ndarray:
     d = np.zeros((M, K), dtype=np.float32)
     rows = np.repeat(np.arange(M), np.diff(rowptr))

@@ -127,6 +127,45 @@ smat_status smat_bcsr_filter(const smat_bcsr_desc* A, int64_t* s_nnzb_host,
                              uint16_t* d_values, smat_stream stream);
 
 /* ------------------------------------------------------------------------ */
+/* Row reordering (Jaccard "Keeper" grouping, the SMaT reorder pipeline)    */
+/* ------------------------------------------------------------------------ */
+/*
+ * Replaces BlockingEngine::GetGrouping with blocking algorithm 5
+ * (IterativeBlockingKeeper, preprocess/src/general/blocking.cpp:431-547,
+ * selected by `Matrix_Blocking -a 5`, src/reorder.py:131): groups rows of a
+ * pattern (columns sorted inside each row) into groups of at most max_rows
+ * by block-level Jaccard distance over column blocks of width col_block.
+ * Bit-exact with the reference, including its counters
+ * (counters_host = [comparison_counter, merge_counter]).  The distance
+ * evaluation runs on the GPU (rowptr/colidx, device); the order-dependent
+ * bookkeeping runs on the host, which also needs the pattern
+ * (rowptr_host/colidx_host).  grouping_host receives `rows` group ids: the
+ * seed row of a complete group, seed + rows for an incomplete one.
+ */
+smat_status smat_reorder_grouping(int64_t rows, const int32_t* rowptr, const int32_t* colidx,
+                                  const int32_t* rowptr_host, const int32_t* colidx_host, float tau,
+                                  int64_t col_block, int64_t max_rows, int64_t* grouping_host,
+                                  int64_t* counters_host, smat_stream stream);
+
+/* get_fixed_size_grouping (preprocess/src/general/utilities.cpp:45-54). Host arrays. */
+smat_status smat_fixed_size_grouping(int64_t rows, const int64_t* grouping_host, int64_t row_block,
+                                     int64_t* out_host);
+
+/* get_permutation (utilities.cpp:8-20): the row order CSR::reorder applies. Host arrays. */
+smat_status smat_grouping_to_permutation(int64_t rows, const int64_t* grouping_host, int64_t* perm_host);
+
+/*
+ * CSR::reorder / permute_rows (csr.cpp:65-74, 97-105) on the GPU: row r of
+ * the output is row perm[r] of the input (perm: device int64[rows]); values
+ * may be NULL (pattern only).  Unlike the reference pipeline, which writes
+ * the reordered matrix as a pattern and sets every value to 1
+ * (Matrix_Blocking.cpp:46, reorder.py:39-40), values travel with their rows.
+ */
+smat_status smat_permute_csr_rows(int64_t rows, const int32_t* rowptr, const int32_t* colidx,
+                                  const uint16_t* values, const int64_t* perm, int32_t* out_rowptr,
+                                  int32_t* out_colidx, uint16_t* out_values, smat_stream stream);
+
+/* ------------------------------------------------------------------------ */
 /* 2:4 compression                                                          */
 /* ------------------------------------------------------------------------ */
 /* Byte layouts of compressed 2:4 blocks. */

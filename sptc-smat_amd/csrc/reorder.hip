@@ -1,0 +1,376 @@
+// Jaccard row reordering (SPARTA "Keeper", the grouping of the SMaT reorder
+// pipeline: src/reorder.py:131 runs Matrix_Blocking with -a 5 -b 8 -B 16 -F 1).
+// Bit-exact with IterativeBlockingKeeper
+// (preprocess/src/general/blocking.cpp:431-547), get_fixed_size_grouping and
+// get_permutation (preprocess/src/general/utilities.cpp:8-54) and
+// CSR::permute_rows (csr.cpp:65-74).
+//
+// Work split.  Seeds are sequential (each seed's outcome decides the next
+// seed and which rows are still free).  Inside a seed the candidates j > i
+// are independent until the first merge, so the GPU evaluates one round of
+// the seed at a time:
+//   * k_scan: for every candidate of [j0, rows) the prune test (seed-start
+//     distances; a candidate's distance is only written when it is visited),
+//     the "already grouped" test and, for eligible rows, the Jaccard
+//     distance to the current group pattern (block-level, count_zeros = 1,
+//     double division cast to float as blocking.cpp:974 does);
+//   * a block-wide scan compacts the compared candidates, in row order, to
+//     (row, distance) pairs and finds the first one with distance <= tau;
+//   * k_commit applies the visit side effects of [j0, merge] (pruned rows get
+//     distance -1, compared rows their distance) once the host knows where
+//     the round stopped.
+// The host replays the order-dependent part on the compacted list: the
+// top-up candidate set with the reference's exact truncation
+// (`advance(best.end(), k)` followed by `erase(it, end())`, whose effect
+// depends on the shape of libstdc++'s red-black tree — the same operation on
+// the same library reproduces it), merge_rows on the pattern, the top-up and
+// the group numbering.  That part is pointer chasing through a set of
+// thousands of elements per seed, latency-bound on a GPU lane; measured set
+// sizes are in the design notes.
+#include <algorithm>
+#include <numeric>
+#include <set>
+#include <utility>
+#include <vector>
+
+#include <hipcub/hipcub.hpp>
+
+#include "smat_internal.h"
+
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr uint8_t ST_SKIP = 0, ST_PRUNED = 1, ST_COMPARED = 2;
+
+// JaccardDistanceGroup(pattern, gA, row, 1, block) over sorted column lists.
+__device__ inline float jaccard(const int32_t* __restrict__ A, int32_t nA, int64_t gA,
+                                const int32_t* __restrict__ Bv, int32_t nB, int64_t block) {
+  if (nA == 0 && nB == 0) return 0.f;
+  if (nA == 0 || nB == 0) return 1.f;
+  const int64_t addA = 1, addB = gA;  // count_zeros: |A\B| weighted by g_B (=1), |B\A| by g_A
+  int32_t i = 0, j = 0;
+  int64_t count = 0, bA = 0, bB = 0;
+  while (i < nA && j < nB) {
+    const int64_t pa = A[i] / block, pb = Bv[j] / block;
+    if (pa < pb) {
+      count += addA;
+      bA++;
+      while (i < nA && A[i] / block == pa) i++;
+    } else if (pa > pb) {
+      count += addB;
+      bB++;
+      while (j < nB && Bv[j] / block == pb) j++;
+    } else {
+      bA++;
+      bB++;
+      while (i < nA && A[i] / block == pa) i++;
+      while (j < nB && Bv[j] / block == pb) j++;
+    }
+  }
+  while (i < nA) {
+    const int64_t pa = A[i] / block;
+    count += addA;
+    bA++;
+    while (i < nA && A[i] / block == pa) i++;
+  }
+  while (j < nB) {
+    const int64_t pb = Bv[j] / block;
+    count += addB;
+    bB++;
+    while (j < nB && Bv[j] / block == pb) j++;
+  }
+  return (float)((2.0 * (double)count) / (double)(bA * gA + bB * 1 + count));
+}
+
+// One thread per candidate j in [j0, rows).  Writes status[j], dist[j] and the
+// first merging row (atomicMin on *first_merge).
+__global__ void k_scan(int64_t rows, int64_t seed, int64_t j0, const int32_t* __restrict__ rowptr,
+                       const int32_t* __restrict__ colidx, const int32_t* __restrict__ pattern, int32_t plen,
+                       int64_t gsize, float tau, int64_t block, const float* __restrict__ d,
+                       const uint8_t* __restrict__ grouped, uint8_t* __restrict__ status, float* __restrict__ dist,
+                       int64_t* __restrict__ first_merge) {
+  const int64_t j = j0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= rows) return;
+  const float di = d[seed], dj = d[j];
+  uint8_t st;
+  if (di != -1.f && dj != -1.f && fabsf(di - dj) > tau) {
+    st = ST_PRUNED;
+  } else if (grouped[j]) {
+    st = ST_SKIP;
+  } else {
+    const int32_t b = rowptr[j], e = rowptr[j + 1];
+    const float x = jaccard(pattern, plen, gsize, colidx + b, e - b, block);
+    dist[j] = x;
+    st = ST_COMPARED;
+    if (x <= tau) atomicMin(reinterpret_cast<unsigned long long*>(first_merge), (unsigned long long)j);
+  }
+  status[j] = st;
+}
+
+__global__ void k_flag_compared(int64_t n, int64_t j0, const uint8_t* __restrict__ status, int32_t* __restrict__ f) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > n) return;
+  f[t] = (t < n && status[j0 + t] == ST_COMPARED) ? 1 : 0;
+}
+
+__global__ void k_compact(int64_t n, int64_t j0, const int32_t* __restrict__ f, const int32_t* __restrict__ pos,
+                          const float* __restrict__ dist, int64_t* __restrict__ out_j, float* __restrict__ out_d) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || !f[t]) return;
+  out_j[pos[t]] = j0 + t;
+  out_d[pos[t]] = dist[j0 + t];
+}
+
+// Visit side effects for [j0, jend): pruned -> -1, compared -> its distance.
+__global__ void k_commit(int64_t j0, int64_t jend, const uint8_t* __restrict__ status,
+                         const float* __restrict__ dist, float* __restrict__ d) {
+  const int64_t j = j0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= jend) return;
+  const uint8_t st = status[j];
+  if (st == ST_PRUNED) d[j] = -1.f;
+  else if (st == ST_COMPARED) d[j] = dist[j];
+}
+
+__global__ void k_mark(const int64_t* __restrict__ rows_list, int32_t n, uint8_t* __restrict__ grouped) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) grouped[rows_list[t]] = 1;
+}
+
+__global__ void k_init(int64_t rows, float* __restrict__ d, uint8_t* __restrict__ grouped) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= rows) return;
+  d[j] = (j == 0) ? -1.f : 0.f;  // the reference's VLA initialiser {-1} sets element 0 only
+  grouped[j] = 0;
+}
+
+// Row permutation of a CSR: new row r is old row perm[r].
+__global__ void k_perm_lengths(int64_t rows, const int64_t* __restrict__ perm, const int32_t* __restrict__ rowptr,
+                               int32_t* __restrict__ len) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > rows) return;
+  len[r] = r < rows ? rowptr[perm[r] + 1] - rowptr[perm[r]] : 0;
+}
+
+__global__ void k_perm_gather(int64_t rows, const int64_t* __restrict__ perm, const int32_t* __restrict__ rowptr,
+                              const int32_t* __restrict__ colidx, const uint16_t* __restrict__ val,
+                              const int32_t* __restrict__ new_rowptr, int32_t* __restrict__ out_col,
+                              uint16_t* __restrict__ out_val) {
+  // one 64-lane group per row
+  const int64_t r = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
+  if (r >= rows) return;
+  const int32_t src = rowptr[perm[r]], n = rowptr[perm[r] + 1] - src, dst = new_rowptr[r];
+  for (int32_t k = lane; k < n; k += 64) {
+    out_col[dst + k] = colidx[src + k];
+    if (val) out_val[dst + k] = val[src + k];
+  }
+}
+
+inline unsigned nblk(int64_t n, int t) {
+  const int64_t g = (n + t - 1) / t;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+
+// merge_rows (utilities.cpp:145-173) on host column lists: not a plain
+// union — when lower_bound reaches the end of A the loop stops, dropping A's
+// remaining elements, and B's tail is appended.
+std::vector<int32_t> merge_rows(const std::vector<int32_t>& A, const int32_t* B, int64_t nB) {
+  std::vector<int32_t> out;
+  out.reserve(A.size() + (size_t)nB);
+  auto i = A.begin();
+  int64_t j = 0;
+  while (j < nB) {
+    const int32_t bv = B[j];
+    auto ni = std::lower_bound(i, A.end(), bv);
+    if (ni == A.end()) break;
+    out.insert(out.end(), i, ni);
+    out.push_back(bv);
+    if (*ni == bv) ++ni;
+    i = ni;
+    j++;
+  }
+  out.insert(out.end(), B + j, B + nB);
+  return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+smat_status smat_reorder_grouping(int64_t rows, const int32_t* rowptr, const int32_t* colidx,
+                                  const int32_t* rowptr_host, const int32_t* colidx_host, float tau,
+                                  int64_t col_block, int64_t max_rows, int64_t* grouping_host,
+                                  int64_t* counters_host, smat_stream stream) {
+  SMAT_REQUIRE(rows > 0 && rowptr && colidx && rowptr_host && colidx_host && grouping_host,
+               "smat_reorder_grouping: null argument or empty matrix");
+  SMAT_REQUIRE(col_block > 0 && max_rows > 0, "smat_reorder_grouping: col_block and max_rows must be positive");
+  SMAT_REQUIRE(tau >= 0.f, "smat_reorder_grouping: tau must be >= 0");
+  hipStream_t s = smat::to_hip(stream);
+  smat::DevBuf d, grouped, status, dist, flag, pos, outj, outd, first, pat, tmp, marks;
+  SMAT_TRY(d.alloc(rows * 4));
+  SMAT_TRY(grouped.alloc(rows));
+  SMAT_TRY(status.alloc(rows));
+  SMAT_TRY(dist.alloc(rows * 4));
+  SMAT_TRY(flag.alloc((rows + 1) * 4));
+  SMAT_TRY(pos.alloc((rows + 1) * 4));
+  SMAT_TRY(outj.alloc(rows * 8));
+  SMAT_TRY(outd.alloc(rows * 4));
+  SMAT_TRY(first.alloc(8));
+  SMAT_TRY(marks.alloc(max_rows * 8));
+  size_t tb = 0;
+  SMAT_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, flag.as<int32_t>(), pos.as<int32_t>(), (int)(rows + 1), s));
+  SMAT_TRY(tmp.alloc(tb));
+  hipLaunchKernelGGL(k_init, nblk(rows, 256), 256, 0, s, rows, d.as<float>(), grouped.as<uint8_t>());
+  SMAT_LAUNCH_CHECK();
+
+  std::vector<int64_t> grouping((size_t)rows, -1);
+  std::vector<int64_t> hj;
+  std::vector<float> hd;
+  int64_t comparisons = 0, merges = 0;
+  size_t pat_cap = 0;
+
+  for (int64_t i = 0; i < rows; i++) {
+    if (grouping[i] != -1) continue;
+    const int64_t group = i + rows;
+    std::set<std::pair<float, int64_t>> best;
+    std::vector<int64_t> merged{i};
+    std::vector<int32_t> pattern(colidx_host + rowptr_host[i], colidx_host + rowptr_host[i + 1]);
+    int64_t gsize = 1;
+    grouping[i] = group;
+    {
+      const int64_t ii = i;
+      SMAT_HIP_TRY(hipMemcpyAsync(marks.p, &ii, 8, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_mark, 1, 64, 0, s, marks.as<int64_t>(), 1, grouped.as<uint8_t>());
+    }
+    int64_t j0 = i + 1;
+    while (j0 < rows && gsize != max_rows) {
+      // upload the pattern and scan [j0, rows)
+      if (pattern.size() + 1 > pat_cap) {
+        pat_cap = 2 * (pattern.size() + 1);
+        SMAT_TRY(pat.alloc(pat_cap * 4));
+      }
+      if (!pattern.empty())
+        SMAT_HIP_TRY(hipMemcpyAsync(pat.p, pattern.data(), pattern.size() * 4, hipMemcpyHostToDevice, s));
+      const int64_t n = rows - j0;
+      const int64_t none = INT64_MAX;
+      SMAT_HIP_TRY(hipMemcpyAsync(first.p, &none, 8, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_scan, nblk(n, kScanThreads), kScanThreads, 0, s, rows, i, j0, rowptr, colidx,
+                         pat.as<int32_t>(), (int32_t)pattern.size(), gsize, tau, col_block, d.as<float>(),
+                         grouped.as<uint8_t>(), status.as<uint8_t>(), dist.as<float>(), first.as<int64_t>());
+      hipLaunchKernelGGL(k_flag_compared, nblk(n + 1, 256), 256, 0, s, n, j0, status.as<uint8_t>(),
+                         flag.as<int32_t>());
+      size_t tbb = tmp.bytes;
+      SMAT_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.p, tbb, flag.as<int32_t>(), pos.as<int32_t>(), (int)(n + 1), s));
+      hipLaunchKernelGGL(k_compact, nblk(n, 256), 256, 0, s, n, j0, flag.as<int32_t>(), pos.as<int32_t>(),
+                         dist.as<float>(), outj.as<int64_t>(), outd.as<float>());
+      SMAT_LAUNCH_CHECK();
+      int32_t ncmp = 0;
+      int64_t jm = none;
+      SMAT_HIP_TRY(hipMemcpyAsync(&ncmp, pos.as<int32_t>() + n, 4, hipMemcpyDeviceToHost, s));
+      SMAT_HIP_TRY(hipMemcpyAsync(&jm, first.p, 8, hipMemcpyDeviceToHost, s));
+      SMAT_HIP_TRY(hipStreamSynchronize(s));
+      // compared candidates strictly before the merge (or all of them)
+      hj.resize((size_t)ncmp);
+      hd.resize((size_t)ncmp);
+      if (ncmp > 0) {
+        SMAT_HIP_TRY(hipMemcpyAsync(hj.data(), outj.p, (size_t)ncmp * 8, hipMemcpyDeviceToHost, s));
+        SMAT_HIP_TRY(hipMemcpyAsync(hd.data(), outd.p, (size_t)ncmp * 4, hipMemcpyDeviceToHost, s));
+        SMAT_HIP_TRY(hipStreamSynchronize(s));
+      }
+      for (int32_t q = 0; q < ncmp && hj[q] < jm; q++) {
+        comparisons++;
+        best.insert(std::make_pair(hd[q], hj[q]));
+        const size_t keep = (size_t)(max_rows - (int64_t)merged.size());
+        if (best.size() > keep) {
+          auto it = best.end();  // blocking.cpp:505-507
+          std::advance(it, keep);
+          best.erase(it, best.end());
+        }
+      }
+      const int64_t jend = jm == none ? rows : jm + 1;
+      hipLaunchKernelGGL(k_commit, nblk(jend - j0, 256), 256, 0, s, j0, jend, status.as<uint8_t>(),
+                         dist.as<float>(), d.as<float>());
+      SMAT_LAUNCH_CHECK();
+      if (jm == none) break;
+      // merge row jm
+      comparisons++;
+      merges++;
+      grouping[jm] = group;
+      merged.push_back(jm);
+      pattern = merge_rows(pattern, colidx_host + rowptr_host[jm], rowptr_host[jm + 1] - rowptr_host[jm]);
+      gsize++;
+      SMAT_HIP_TRY(hipMemcpyAsync(marks.p, &jm, 8, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_mark, 1, 64, 0, s, marks.as<int64_t>(), 1, grouped.as<uint8_t>());
+      j0 = jm + 1;
+    }
+    // top-up from the candidate set (blocking.cpp:513-521)
+    std::vector<int64_t> topped;
+    if (gsize < max_rows) {
+      for (auto it = best.begin(); it != best.end() && gsize != max_rows; ++it) {
+        grouping[it->second] = group;
+        merged.push_back(it->second);
+        topped.push_back(it->second);
+        gsize++;
+      }
+    }
+    if (gsize == max_rows)
+      for (int64_t idx : merged) grouping[idx] -= rows;  // complete groups first
+    if (!topped.empty()) {
+      SMAT_HIP_TRY(hipMemcpyAsync(marks.p, topped.data(), topped.size() * 8, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_mark, 1, 64, 0, s, marks.as<int64_t>(), (int32_t)topped.size(), grouped.as<uint8_t>());
+    }
+    SMAT_HIP_TRY(hipStreamSynchronize(s));  // `topped` lives on the host stack
+  }
+  std::copy(grouping.begin(), grouping.end(), grouping_host);
+  if (counters_host) {
+    counters_host[0] = comparisons;
+    counters_host[1] = merges;
+  }
+  return SMAT_OK;
+}
+
+// get_permutation (utilities.cpp:8-20): iota stably-unspecified sorted by key
+// with std::sort and an int comparator, exactly as the reference does.
+smat_status smat_grouping_to_permutation(int64_t rows, const int64_t* grouping_host, int64_t* perm_host) {
+  SMAT_REQUIRE(rows >= 0 && grouping_host && perm_host, "smat_grouping_to_permutation: null argument");
+  std::vector<int64_t> v((size_t)rows);
+  std::iota(v.begin(), v.end(), 0);
+  auto comp = [&](int a, int b) { return grouping_host[a] < grouping_host[b]; };
+  std::sort(v.begin(), v.end(), comp);
+  std::copy(v.begin(), v.end(), perm_host);
+  return SMAT_OK;
+}
+
+// get_fixed_size_grouping (utilities.cpp:45-54).
+smat_status smat_fixed_size_grouping(int64_t rows, const int64_t* grouping_host, int64_t row_block,
+                                     int64_t* out_host) {
+  SMAT_REQUIRE(rows >= 0 && grouping_host && out_host && row_block > 0, "smat_fixed_size_grouping: bad argument");
+  std::vector<int64_t> perm((size_t)rows);
+  smat_grouping_to_permutation(rows, grouping_host, perm.data());
+  for (int64_t i = 0; i < rows; i++) out_host[perm[i]] = i / row_block;
+  return SMAT_OK;
+}
+
+// CSR::reorder / permute_rows (csr.cpp:65-74, 97-105) on the GPU: new row r is
+// old row perm[r]; the output arrays are caller-allocated (nnz entries).
+smat_status smat_permute_csr_rows(int64_t rows, const int32_t* rowptr, const int32_t* colidx,
+                                  const uint16_t* values, const int64_t* perm, int32_t* out_rowptr,
+                                  int32_t* out_colidx, uint16_t* out_values, smat_stream stream) {
+  SMAT_REQUIRE(rows > 0 && rowptr && colidx && perm && out_rowptr && out_colidx,
+               "smat_permute_csr_rows: null argument");
+  hipStream_t s = smat::to_hip(stream);
+  smat::DevBuf len, tmp;
+  SMAT_TRY(len.alloc((rows + 1) * 4));
+  hipLaunchKernelGGL(k_perm_lengths, nblk(rows + 1, 256), 256, 0, s, rows, perm, rowptr, len.as<int32_t>());
+  size_t tb = 0;
+  SMAT_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, len.as<int32_t>(), out_rowptr, (int)(rows + 1), s));
+  SMAT_TRY(tmp.alloc(tb));
+  SMAT_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, len.as<int32_t>(), out_rowptr, (int)(rows + 1), s));
+  hipLaunchKernelGGL(k_perm_gather, nblk(rows, 4), 256, 0, s, rows, perm, rowptr, colidx, values, out_rowptr,
+                     out_colidx, out_values);
+  SMAT_LAUNCH_CHECK();
+  SMAT_HIP_TRY(hipStreamSynchronize(s));
+  return SMAT_OK;
+}
+
+}  // extern "C"

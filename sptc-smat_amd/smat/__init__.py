@@ -72,6 +72,10 @@ SIGNATURES = {
     "smat_spmm_execute": (_i32, [_p, _p, _p, _p, _i64, _p, _i64, _p]),
     "smat_spmm_plan_destroy": (_i32, [_p]),
     "smat_spmm_plan_info": (_i32, [_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "smat_reorder_grouping": (_i32, [_i64, _p, _p, _p, _p, ctypes.c_float, _i64, _i64, _p, _p, _p]),
+    "smat_fixed_size_grouping": (_i32, [_i64, _p, _i64, _p]),
+    "smat_grouping_to_permutation": (_i32, [_i64, _p, _p]),
+    "smat_permute_csr_rows": (_i32, [_i64, _p, _p, _p, _p, _p, _p, _p, _p]),
     "mmaOBTKernel": (None, [_p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
     "mmaOBTSKernel": (None, [_p, _p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
     "preprocessing_mmaSTKernel": (None, [_p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
@@ -276,3 +280,38 @@ class SpmmPlan:
             self.close()
         except Exception:
             pass
+
+
+def reorder_grouping(rowptr, colidx, tau: float, col_block: int, max_rows: int, stream=None):
+    """Keeper (algorithm 5) grouping of a host CSR pattern (int32 numpy
+    arrays, columns sorted per row); returns (grouping int64[rows],
+    [comparisons, merges])."""
+    import numpy as np
+    import torch
+    rp = np.ascontiguousarray(rowptr, np.int32)
+    ci = np.ascontiguousarray(colidx, np.int32)
+    rows = rp.size - 1
+    d_rp = torch.from_numpy(rp).cuda()
+    d_ci = torch.from_numpy(ci if ci.size else np.zeros(1, np.int32)).cuda()
+    g = np.zeros(rows, np.int64)
+    cnt = np.zeros(2, np.int64)
+    check(lib().smat_reorder_grouping(rows, d_rp.data_ptr(), d_ci.data_ptr(), rp.ctypes.data,
+                                      (ci if ci.size else np.zeros(1, np.int32)).ctypes.data, float(tau),
+                                      col_block, max_rows, g.ctypes.data, cnt.ctypes.data, stream_ptr(stream)))
+    return g, cnt
+
+
+def fixed_size_grouping(grouping, row_block: int):
+    import numpy as np
+    g = np.ascontiguousarray(grouping, np.int64)
+    out = np.zeros_like(g)
+    check(lib().smat_fixed_size_grouping(g.size, g.ctypes.data, row_block, out.ctypes.data))
+    return out
+
+
+def grouping_to_permutation(grouping):
+    import numpy as np
+    g = np.ascontiguousarray(grouping, np.int64)
+    out = np.zeros_like(g)
+    check(lib().smat_grouping_to_permutation(g.size, g.ctypes.data, out.ctypes.data))
+    return out

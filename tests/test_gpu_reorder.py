@@ -1,0 +1,84 @@
+"""GPU Keeper reordering vs the compiled reference's golden vectors and the
+oracle, bit-exact; GPU row permutation vs numpy."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_ffi as of
+from smat import gen
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def test_keeper_golden_cases(golden_dir):
+    _need_gpu()
+    import smat
+    d = np.load(os.path.join(golden_dir, "keeper_golden.npz"))
+    cases = [str(c) for c in d["cases"]]
+    for key in cases:
+        name, params = key.split("__")
+        t, b, B = params.split("_")
+        tau, cb, mr = float(t[1:]), int(b[1:]), int(B[1:])
+        rp, ci = d[f"{name}__rowptr"].astype(np.int32), d[f"{name}__colidx"].astype(np.int32)
+        print(f"case {key}", flush=True)
+        g, cnt = smat.reorder_grouping(rp, ci, np.float32(tau), cb, mr)
+        assert np.array_equal(g, d[key + "__grouping"]), key
+        assert np.array_equal(cnt, d[key + "__counters"]), key
+        fsg = smat.fixed_size_grouping(g, mr)
+        assert np.array_equal(fsg, d[key + "__fixed"]), key
+        assert np.array_equal(smat.grouping_to_permutation(fsg), d[key + "__perm"]), key
+
+
+@pytest.mark.parametrize("scale,tau,cb", [(12, 0.5, 16), (12, 0.1, 8), (11, 0.7, 16)])
+def test_keeper_vs_oracle_rmat(scale, tau, cb):
+    _need_gpu()
+    import smat
+    M, K, rp, ci, _ = gen.rmat(scale=scale, edge_factor=16, seed=200 + scale)
+    rows = 1 << scale
+    rp = rp[: rows + 1]
+    print(f"rmat scale {scale} tau {tau} cb {cb}", flush=True)
+    g, cnt = smat.reorder_grouping(rp, ci, np.float32(tau), cb, 16)
+    eg, ecnt = of.keeper(rp.astype(np.int64), ci.astype(np.int64), np.float32(tau), cb, 16)
+    assert np.array_equal(cnt, ecnt)
+    assert np.array_equal(g, eg)
+
+
+def test_keeper_band():
+    _need_gpu()
+    import smat
+    M, K, rp, ci, _ = gen.band(n=2048, hb=40, p=0.4, seed=77)
+    g, cnt = smat.reorder_grouping(rp, ci, np.float32(0.3), 16, 16)
+    eg, ecnt = of.keeper(rp.astype(np.int64), ci.astype(np.int64), np.float32(0.3), 16, 16)
+    assert np.array_equal(cnt, ecnt) and np.array_equal(g, eg)
+
+
+def test_permute_rows_on_gpu():
+    _need_gpu()
+    import smat
+    M, K, rp, ci, v = gen.rmat(scale=10, edge_factor=8, seed=5)
+    rng = np.random.default_rng(1)
+    perm = rng.permutation(M).astype(np.int64)
+    d_rp, d_ci = torch.from_numpy(rp).cuda(), torch.from_numpy(ci).cuda()
+    d_v = torch.from_numpy(v.view(np.int16)).cuda()
+    d_perm = torch.from_numpy(perm).cuda()
+    o_rp = torch.empty(M + 1, dtype=torch.int32, device="cuda")
+    o_ci = torch.empty_like(d_ci)
+    o_v = torch.empty_like(d_v)
+    smat.check(smat.lib().smat_permute_csr_rows(M, d_rp.data_ptr(), d_ci.data_ptr(), d_v.data_ptr(),
+                                                d_perm.data_ptr(), o_rp.data_ptr(), o_ci.data_ptr(),
+                                                o_v.data_ptr(), smat.stream_ptr()))
+    torch.cuda.synchronize()
+    lens = np.diff(rp)[perm]
+    erp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    eci = np.concatenate([ci[rp[r]:rp[r + 1]] for r in perm])
+    ev = np.concatenate([v[rp[r]:rp[r + 1]] for r in perm])
+    assert np.array_equal(o_rp.cpu().numpy(), erp)
+    assert np.array_equal(o_ci.cpu().numpy(), eci)
+    assert np.array_equal(o_v.cpu().numpy().view(np.uint16), ev)

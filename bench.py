@@ -46,7 +46,8 @@ def parse():
     ap.add_argument("--kernel", default="dense", choices=["dense", "mixed24"])
     ap.add_argument("--dtype", default="f16", choices=["f16", "bf16"])
     ap.add_argument("--single", action="store_true", help="time only --bk/--n-tile (no sweep, no plain leg)")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and preprocessing legs")
+    ap.add_argument("--pre-scale", type=int, default=14, help="R-MAT scale of the reorder comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline duration")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run a device copy of this many MiB (HBM counter calibration) and exit")
@@ -108,6 +109,33 @@ def cpu_baseline(M, rp, ci, vals_u16, N, target_s):
     return {"value": round(gflops, 4), "unit": "useful GFLOP/s", "cores": 1, "kind": kind,
             "sample": f"CSR::multiply (float, column-major B/C) on the leading {r}x{r} submatrix "
                       f"({nnz} nnz) x {N} columns, {t:.2f} s"}
+
+
+def preprocess_leg(scale: int, tau: float = 0.5):
+    """Jaccard Keeper reordering (16-row groups, 16-column blocks) of an R-MAT
+    sample: the GPU implementation vs the reference's own CPU code
+    (IterativeBlockingKeeper compiled into oracle/_ref), on one core.  The two
+    groupings must be identical."""
+    import numpy as np
+    import smat
+    from smat import gen
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ffi as of
+    M, K, rp, ci, _ = gen.rmat(scale=scale, seed=1002)
+    rows = 1 << scale
+    rp = rp[: rows + 1]
+    t0 = time.perf_counter()
+    g, cnt = smat.reorder_grouping(rp, ci, np.float32(tau), 16, 16)
+    t_gpu = time.perf_counter() - t0
+    out = {"workload": f"R-MAT scale {scale} ({rows} rows, {int(ci.size)} nnz), Keeper tau={tau}, 16x16",
+           "gpu_s": round(t_gpu, 3), "comparisons": int(cnt[0]), "merges": int(cnt[1])}
+    if of.ref_available():
+        eg, ecnt, t_ref = of.keeper(rp.astype(np.int64), ci.astype(np.int64), np.float32(tau), 16, 16,
+                                    cols=K, use_ref=True)
+        out.update({"reference_cpu_s": round(t_ref, 3), "cores": 1,
+                    "speedup": round(t_ref / t_gpu, 2),
+                    "identical": bool(np.array_equal(g, eg) and np.array_equal(cnt, ecnt))})
+    return out
 
 
 def make_workload(args):
@@ -276,8 +304,13 @@ def main():
         tuned = measure(best["bk"], 0 if best["n_tile"] == "auto" else best["n_tile"], best["kernel"],
                         args.steps, args.warmup)
 
-    cpu = None
+    cpu = pre = None
     if not args.no_cpu:
+        try:
+            pre = preprocess_leg(args.pre_scale)
+            log(f"reorder: {pre}")
+        except Exception as ex:  # a report, never the measurement
+            pre = {"error": str(ex)}
         try:
             # the reference CPU path computes in float; it is fed the same values
             v16 = v if args.dtype == "f16" else ctx.vals_f64.astype(np.float16).view(np.uint16)
@@ -311,6 +344,7 @@ def main():
                      "alg_bytes_per_call": tuned["alg_bytes"], "event_ms_avg": tuned["event_ms"]},
         "sweep": [{k: s[k] for k in ("bk", "n_tile", "kernel", "event_ms", "useful_gflops")} for s in sweep],
         "cpu_baseline": cpu,
+        "preprocess": pre,
     }
     print(json.dumps(out))
 

@@ -39,6 +39,31 @@ void DevBuf::release() {
   bytes = 0;
 }
 
+smat_status HostBuf::alloc(size_t n) {
+  release();
+  if (n == 0) n = 256;
+  hipError_t e = hipHostMalloc(&p, n, hipHostMallocMapped);
+  if (e != hipSuccess) {
+    p = nullptr;
+    (void)hipGetLastError();
+    return set_error(SMAT_ERR_ALLOC, "hipHostMalloc(%zu) failed: %s", n, hipGetErrorString(e));
+  }
+  e = hipHostGetDevicePointer(&d, p, 0);
+  if (e != hipSuccess) {
+    (void)hipHostFree(p);
+    p = d = nullptr;
+    return set_error(SMAT_ERR_HIP, "hipHostGetDevicePointer failed: %s", hipGetErrorString(e));
+  }
+  bytes = n;
+  return SMAT_OK;
+}
+
+void HostBuf::release() {
+  if (p) (void)hipHostFree(p);
+  p = d = nullptr;
+  bytes = 0;
+}
+
 }  // namespace smat
 
 extern "C" {

@@ -82,44 +82,97 @@ __device__ inline float jaccard(const int32_t* __restrict__ A, int32_t nA, int64
   return (float)((2.0 * (double)count) / (double)(bA * gA + bB * 1 + count));
 }
 
-// One thread per candidate j in [j0, rows).  Writes status[j], dist[j] and the
-// first merging row (atomicMin on *first_merge).
-__global__ void k_scan(int64_t rows, int64_t seed, int64_t j0, const int32_t* __restrict__ rowptr,
-                       const int32_t* __restrict__ colidx, const int32_t* __restrict__ pattern, int32_t plen,
-                       int64_t gsize, float tau, int64_t block, const float* __restrict__ d,
-                       const uint8_t* __restrict__ grouped, uint8_t* __restrict__ status, float* __restrict__ dist,
-                       int64_t* __restrict__ first_merge) {
-  const int64_t j = j0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= rows) return;
-  const float di = d[seed], dj = d[j];
-  uint8_t st;
-  if (di != -1.f && dj != -1.f && fabsf(di - dj) > tau) {
-    st = ST_PRUNED;
-  } else if (grouped[j]) {
-    st = ST_SKIP;
-  } else {
-    const int32_t b = rowptr[j], e = rowptr[j + 1];
-    const float x = jaccard(pattern, plen, gsize, colidx + b, e - b, block);
-    dist[j] = x;
-    st = ST_COMPARED;
-    if (x <= tau) atomicMin(reinterpret_cast<unsigned long long*>(first_merge), (unsigned long long)j);
+// Round, kernel 1 (kScanThreads candidates per block): status, distance,
+// block-local exclusive position of each compared candidate, per-block
+// count, and the first merging row (atomicMin).
+__global__ __launch_bounds__(kScanThreads) void k_round_scan(
+    int64_t rows, int64_t seed, int64_t j0, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+    const int32_t* __restrict__ pattern, int32_t plen, int64_t gsize, float tau, int64_t block,
+    const float* __restrict__ d, const uint8_t* __restrict__ grouped, uint8_t* __restrict__ status,
+    float* __restrict__ dist, int32_t* __restrict__ local_pos, int32_t* __restrict__ block_count,
+    unsigned long long* __restrict__ first_merge) {
+  __shared__ int32_t wsum[kScanThreads / 64];
+  const int t = threadIdx.x;
+  const int64_t j = j0 + (int64_t)blockIdx.x * kScanThreads + t;
+  int flag = 0;
+  if (j < rows) {
+    const float di = d[seed], dj = d[j];
+    uint8_t st;
+    if (di != -1.f && dj != -1.f && fabsf(di - dj) > tau) {
+      st = ST_PRUNED;
+    } else if (grouped[j]) {
+      st = ST_SKIP;
+    } else {
+      const int32_t b = rowptr[j], e = rowptr[j + 1];
+      const float x = jaccard(pattern, plen, gsize, colidx + b, e - b, block);
+      dist[j] = x;
+      st = ST_COMPARED;
+      flag = 1;
+      if (x <= tau) atomicMin(first_merge, (unsigned long long)j);
+    }
+    status[j] = st;
   }
-  status[j] = st;
+  // block-local exclusive prefix of the compared flags (wave ballots + LDS)
+  const int lane = t & 63, w = t >> 6;
+  const unsigned long long bal = __ballot(flag);
+  const int in_wave = __popcll(bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+  if (lane == 0) wsum[w] = __popcll(bal);
+  __syncthreads();
+  int before = 0, total = 0;
+#pragma unroll
+  for (int q = 0; q < kScanThreads / 64; q++) {
+    before += (q < w) ? wsum[q] : 0;
+    total += wsum[q];
+  }
+  if (flag) local_pos[j] = before + in_wave;
+  if (t == 0) block_count[blockIdx.x] = total;
 }
 
-__global__ void k_flag_compared(int64_t n, int64_t j0, const uint8_t* __restrict__ status, int32_t* __restrict__ f) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t > n) return;
-  f[t] = (t < n && status[j0 + t] == ST_COMPARED) ? 1 : 0;
+// Round, kernel 2 (one block): block offsets, total and first merge ->
+// header in mapped host memory; resets first_merge for the next round.
+__global__ __launch_bounds__(1024) void k_round_offsets(int32_t nb, const int32_t* __restrict__ block_count,
+                                                        int32_t* __restrict__ block_off,
+                                                        unsigned long long* __restrict__ first_merge,
+                                                        int64_t* __restrict__ hdr) {
+  __shared__ int32_t part[1024];
+  const int t = threadIdx.x;
+  int32_t carry = 0;
+  for (int32_t base = 0; base < nb; base += 1024) {
+    const int32_t v = (base + t < nb) ? block_count[base + t] : 0;
+    part[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
+      const int32_t x = (t >= off) ? part[t - off] : 0;
+      __syncthreads();
+      part[t] += x;
+      __syncthreads();
+    }
+    if (base + t < nb) block_off[base + t] = carry + part[t] - v;
+    carry += part[1023];
+    __syncthreads();
+  }
+  if (t == 0) {
+    hdr[0] = carry;
+    hdr[1] = (int64_t)*first_merge;
+    *first_merge = (unsigned long long)INT64_MAX;
+  }
 }
 
-__global__ void k_compact(int64_t n, int64_t j0, const int32_t* __restrict__ f, const int32_t* __restrict__ pos,
-                          const float* __restrict__ dist, int64_t* __restrict__ out_j, float* __restrict__ out_d) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n || !f[t]) return;
-  out_j[pos[t]] = j0 + t;
-  out_d[pos[t]] = dist[j0 + t];
+// Round, kernel 3: compared candidates in row order -> mapped host memory.
+__global__ __launch_bounds__(kScanThreads) void k_round_write(int64_t rows, int64_t j0,
+                                                              const uint8_t* __restrict__ status,
+                                                              const float* __restrict__ dist,
+                                                              const int32_t* __restrict__ local_pos,
+                                                              const int32_t* __restrict__ block_off,
+                                                              int32_t* __restrict__ out_j, float* __restrict__ out_d) {
+  const int64_t j = j0 + (int64_t)blockIdx.x * kScanThreads + threadIdx.x;
+  if (j >= rows || status[j] != ST_COMPARED) return;
+  const int32_t q = block_off[blockIdx.x] + local_pos[j];
+  out_j[q] = (int32_t)j;
+  out_d[q] = dist[j];
 }
+
+__global__ void k_reset_first(unsigned long long* first_merge) { *first_merge = (unsigned long long)INT64_MAX; }
 
 // Visit side effects for [j0, jend): pruned -> -1, compared -> its distance.
 __global__ void k_commit(int64_t j0, int64_t jend, const uint8_t* __restrict__ status,
@@ -193,6 +246,46 @@ std::vector<int32_t> merge_rows(const std::vector<int32_t>& A, const int32_t* B,
   return out;
 }
 
+// Fixed-size node pool for the top-up set.  The set keeps libstdc++'s
+// red-black tree algorithm (so its shape, which the truncation depends on,
+// is unchanged); only node allocation skips malloc.
+struct NodePool {
+  std::vector<void*> free_;
+  std::vector<std::vector<char>> chunks_;
+  size_t size_ = 0;
+  void* get(size_t sz) {
+    if (size_ == 0) size_ = sz;
+    if (sz != size_) return ::operator new(sz);
+    if (free_.empty()) {
+      chunks_.emplace_back(size_ * 4096);
+      char* b = chunks_.back().data();
+      for (size_t k = 0; k < 4096; k++) free_.push_back(b + k * size_);
+    }
+    void* p = free_.back();
+    free_.pop_back();
+    return p;
+  }
+  void put(void* p, size_t sz) {
+    if (sz != size_) ::operator delete(p);
+    else free_.push_back(p);
+  }
+};
+
+template <class T>
+struct PoolAlloc {
+  using value_type = T;
+  NodePool* pool;
+  explicit PoolAlloc(NodePool* p) : pool(p) {}
+  template <class U>
+  PoolAlloc(const PoolAlloc<U>& o) : pool(o.pool) {}
+  T* allocate(size_t n) { return static_cast<T*>(pool->get(n * sizeof(T))); }
+  void deallocate(T* p, size_t n) { pool->put(p, n * sizeof(T)); }
+  template <class U>
+  bool operator==(const PoolAlloc<U>& o) const { return pool == o.pool; }
+  template <class U>
+  bool operator!=(const PoolAlloc<U>& o) const { return pool != o.pool; }
+};
+
 }  // namespace
 
 extern "C" {
@@ -203,83 +296,99 @@ smat_status smat_reorder_grouping(int64_t rows, const int32_t* rowptr, const int
                                   int64_t* counters_host, smat_stream stream) {
   SMAT_REQUIRE(rows > 0 && rowptr && colidx && rowptr_host && colidx_host && grouping_host,
                "smat_reorder_grouping: null argument or empty matrix");
+  SMAT_REQUIRE(rows < INT32_MAX, "smat_reorder_grouping: too many rows");
   SMAT_REQUIRE(col_block > 0 && max_rows > 0, "smat_reorder_grouping: col_block and max_rows must be positive");
   SMAT_REQUIRE(tau >= 0.f, "smat_reorder_grouping: tau must be >= 0");
   hipStream_t s = smat::to_hip(stream);
-  smat::DevBuf d, grouped, status, dist, flag, pos, outj, outd, first, pat, tmp, marks;
+  const int64_t nb_max = (rows + kScanThreads - 1) / kScanThreads;
+  smat::DevBuf d, grouped, status, dist, lpos, bcount, boff, first, pat;
   SMAT_TRY(d.alloc(rows * 4));
   SMAT_TRY(grouped.alloc(rows));
   SMAT_TRY(status.alloc(rows));
   SMAT_TRY(dist.alloc(rows * 4));
-  SMAT_TRY(flag.alloc((rows + 1) * 4));
-  SMAT_TRY(pos.alloc((rows + 1) * 4));
-  SMAT_TRY(outj.alloc(rows * 8));
-  SMAT_TRY(outd.alloc(rows * 4));
+  SMAT_TRY(lpos.alloc(rows * 4));
+  SMAT_TRY(bcount.alloc(nb_max * 4));
+  SMAT_TRY(boff.alloc(nb_max * 4));
   SMAT_TRY(first.alloc(8));
-  SMAT_TRY(marks.alloc(max_rows * 8));
-  size_t tb = 0;
-  SMAT_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, flag.as<int32_t>(), pos.as<int32_t>(), (int)(rows + 1), s));
-  SMAT_TRY(tmp.alloc(tb));
+  // Pinned, device-mapped host buffers: the round kernels write the compared
+  // list and its header straight into host memory (one sync per round);
+  // marks hold the rows to flag as grouped (each row is written once).
+  smat::HostBuf h_j, h_d, h_hdr, h_marks, h_pat;
+  SMAT_TRY(h_j.alloc(rows * 4));
+  SMAT_TRY(h_d.alloc(rows * 4));
+  SMAT_TRY(h_hdr.alloc(16));
+  SMAT_TRY(h_marks.alloc(rows * 8));
   hipLaunchKernelGGL(k_init, nblk(rows, 256), 256, 0, s, rows, d.as<float>(), grouped.as<uint8_t>());
+  hipLaunchKernelGGL(k_reset_first, 1, 1, 0, s, first.as<unsigned long long>());
   SMAT_LAUNCH_CHECK();
 
+  const int32_t* oj = h_j.as<int32_t>();
+  const float* od = h_d.as<float>();
+  volatile int64_t* hdr = h_hdr.as<int64_t>();
+  int64_t* marks = h_marks.as<int64_t>();
+  int64_t marks_used = 0;
+  auto mark = [&](const int64_t* list, int32_t n) {
+    std::copy(list, list + n, marks + marks_used);
+    hipLaunchKernelGGL(k_mark, nblk(n, 64), 64, 0, s, h_marks.dev<int64_t>() + marks_used, n,
+                       grouped.as<uint8_t>());
+    marks_used += n;
+  };
+
   std::vector<int64_t> grouping((size_t)rows, -1);
-  std::vector<int64_t> hj;
-  std::vector<float> hd;
   int64_t comparisons = 0, merges = 0;
   size_t pat_cap = 0;
+  NodePool pool;
+  using BestSet = std::set<std::pair<float, int64_t>, std::less<std::pair<float, int64_t>>,
+                           PoolAlloc<std::pair<float, int64_t>>>;
 
   for (int64_t i = 0; i < rows; i++) {
     if (grouping[i] != -1) continue;
     const int64_t group = i + rows;
-    std::set<std::pair<float, int64_t>> best;
+    BestSet best{std::less<std::pair<float, int64_t>>(), PoolAlloc<std::pair<float, int64_t>>(&pool)};
     std::vector<int64_t> merged{i};
     std::vector<int32_t> pattern(colidx_host + rowptr_host[i], colidx_host + rowptr_host[i + 1]);
     int64_t gsize = 1;
     grouping[i] = group;
-    {
-      const int64_t ii = i;
-      SMAT_HIP_TRY(hipMemcpyAsync(marks.p, &ii, 8, hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(k_mark, 1, 64, 0, s, marks.as<int64_t>(), 1, grouped.as<uint8_t>());
-    }
+    mark(&i, 1);
     int64_t j0 = i + 1;
+    bool pattern_dirty = true;
     while (j0 < rows && gsize != max_rows) {
-      // upload the pattern and scan [j0, rows)
-      if (pattern.size() + 1 > pat_cap) {
-        pat_cap = 2 * (pattern.size() + 1);
-        SMAT_TRY(pat.alloc(pat_cap * 4));
+      if (pattern_dirty) {
+        // the pattern goes through a pinned staging area so the copy is asynchronous
+        if (pattern.size() + 1 > pat_cap) {
+          pat_cap = 2 * (pattern.size() + 1);
+          SMAT_TRY(pat.alloc(pat_cap * 4));
+          SMAT_TRY(h_pat.alloc(pat_cap * 4));
+        }
+        std::copy(pattern.begin(), pattern.end(), h_pat.as<int32_t>());
+        if (!pattern.empty())
+          SMAT_HIP_TRY(hipMemcpyAsync(pat.p, h_pat.p, pattern.size() * 4, hipMemcpyHostToDevice, s));
+        pattern_dirty = false;
       }
-      if (!pattern.empty())
-        SMAT_HIP_TRY(hipMemcpyAsync(pat.p, pattern.data(), pattern.size() * 4, hipMemcpyHostToDevice, s));
       const int64_t n = rows - j0;
-      const int64_t none = INT64_MAX;
-      SMAT_HIP_TRY(hipMemcpyAsync(first.p, &none, 8, hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(k_scan, nblk(n, kScanThreads), kScanThreads, 0, s, rows, i, j0, rowptr, colidx,
-                         pat.as<int32_t>(), (int32_t)pattern.size(), gsize, tau, col_block, d.as<float>(),
-                         grouped.as<uint8_t>(), status.as<uint8_t>(), dist.as<float>(), first.as<int64_t>());
-      hipLaunchKernelGGL(k_flag_compared, nblk(n + 1, 256), 256, 0, s, n, j0, status.as<uint8_t>(),
-                         flag.as<int32_t>());
-      size_t tbb = tmp.bytes;
-      SMAT_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.p, tbb, flag.as<int32_t>(), pos.as<int32_t>(), (int)(n + 1), s));
-      hipLaunchKernelGGL(k_compact, nblk(n, 256), 256, 0, s, n, j0, flag.as<int32_t>(), pos.as<int32_t>(),
-                         dist.as<float>(), outj.as<int64_t>(), outd.as<float>());
+      const int32_t nb = (int32_t)((n + kScanThreads - 1) / kScanThreads);
+      hipLaunchKernelGGL(k_round_scan, nb, kScanThreads, 0, s, rows, i, j0, rowptr, colidx, pat.as<int32_t>(),
+                         (int32_t)pattern.size(), gsize, tau, col_block, d.as<float>(), grouped.as<uint8_t>(),
+                         status.as<uint8_t>(), dist.as<float>(), lpos.as<int32_t>(), bcount.as<int32_t>(),
+                         first.as<unsigned long long>());
+      hipLaunchKernelGGL(k_round_offsets, 1, 1024, 0, s, nb, bcount.as<int32_t>(), boff.as<int32_t>(),
+                         first.as<unsigned long long>(), h_hdr.dev<int64_t>());
+      hipLaunchKernelGGL(k_round_write, nb, kScanThreads, 0, s, rows, j0, status.as<uint8_t>(), dist.as<float>(),
+                         lpos.as<int32_t>(), boff.as<int32_t>(), h_j.dev<int32_t>(), h_d.dev<float>());
       SMAT_LAUNCH_CHECK();
-      int32_t ncmp = 0;
-      int64_t jm = none;
-      SMAT_HIP_TRY(hipMemcpyAsync(&ncmp, pos.as<int32_t>() + n, 4, hipMemcpyDeviceToHost, s));
-      SMAT_HIP_TRY(hipMemcpyAsync(&jm, first.p, 8, hipMemcpyDeviceToHost, s));
       SMAT_HIP_TRY(hipStreamSynchronize(s));
+      const int64_t ncmp = hdr[0];
+      const int64_t jm = hdr[1];
       // compared candidates strictly before the merge (or all of them)
-      hj.resize((size_t)ncmp);
-      hd.resize((size_t)ncmp);
-      if (ncmp > 0) {
-        SMAT_HIP_TRY(hipMemcpyAsync(hj.data(), outj.p, (size_t)ncmp * 8, hipMemcpyDeviceToHost, s));
-        SMAT_HIP_TRY(hipMemcpyAsync(hd.data(), outd.p, (size_t)ncmp * 4, hipMemcpyDeviceToHost, s));
-        SMAT_HIP_TRY(hipStreamSynchronize(s));
-      }
-      for (int32_t q = 0; q < ncmp && hj[q] < jm; q++) {
+      for (int64_t q = 0; q < ncmp && oj[q] < jm; q++) {
         comparisons++;
-        best.insert(std::make_pair(hd[q], hj[q]));
+        const std::pair<float, int64_t> key(od[q], (int64_t)oj[q]);
+        // A key above the current maximum lands as the right child of the
+        // rightmost node whether or not the end() hint is given, so the hint
+        // only skips the descent: the tree, and the truncation below, stay
+        // exactly those of a plain insert.
+        if (!best.empty() && *best.rbegin() < key) best.insert(best.end(), key);
+        else best.insert(key);
         const size_t keep = (size_t)(max_rows - (int64_t)merged.size());
         if (best.size() > keep) {
           auto it = best.end();  // blocking.cpp:505-507
@@ -287,20 +396,20 @@ smat_status smat_reorder_grouping(int64_t rows, const int32_t* rowptr, const int
           best.erase(it, best.end());
         }
       }
-      const int64_t jend = jm == none ? rows : jm + 1;
+      const int64_t jend = jm == INT64_MAX ? rows : jm + 1;
       hipLaunchKernelGGL(k_commit, nblk(jend - j0, 256), 256, 0, s, j0, jend, status.as<uint8_t>(),
                          dist.as<float>(), d.as<float>());
       SMAT_LAUNCH_CHECK();
-      if (jm == none) break;
+      if (jm == INT64_MAX) break;
       // merge row jm
       comparisons++;
       merges++;
       grouping[jm] = group;
       merged.push_back(jm);
       pattern = merge_rows(pattern, colidx_host + rowptr_host[jm], rowptr_host[jm + 1] - rowptr_host[jm]);
+      pattern_dirty = true;
       gsize++;
-      SMAT_HIP_TRY(hipMemcpyAsync(marks.p, &jm, 8, hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(k_mark, 1, 64, 0, s, marks.as<int64_t>(), 1, grouped.as<uint8_t>());
+      mark(&jm, 1);
       j0 = jm + 1;
     }
     // top-up from the candidate set (blocking.cpp:513-521)
@@ -315,12 +424,9 @@ smat_status smat_reorder_grouping(int64_t rows, const int32_t* rowptr, const int
     }
     if (gsize == max_rows)
       for (int64_t idx : merged) grouping[idx] -= rows;  // complete groups first
-    if (!topped.empty()) {
-      SMAT_HIP_TRY(hipMemcpyAsync(marks.p, topped.data(), topped.size() * 8, hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(k_mark, 1, 64, 0, s, marks.as<int64_t>(), (int32_t)topped.size(), grouped.as<uint8_t>());
-    }
-    SMAT_HIP_TRY(hipStreamSynchronize(s));  // `topped` lives on the host stack
+    if (!topped.empty()) mark(topped.data(), (int32_t)topped.size());
   }
+  SMAT_HIP_TRY(hipStreamSynchronize(s));
   std::copy(grouping.begin(), grouping.end(), grouping_host);
   if (counters_host) {
     counters_host[0] = comparisons;

@@ -27,9 +27,24 @@ struct DevBuf {
   T* as() const { return static_cast<T*>(p); }
 };
 
+// Pinned host buffer (hipHostMalloc), mapped into the device address space:
+// kernels may write it directly; the host reads it after a stream sync.
+struct HostBuf {
+  void* p = nullptr;
+  void* d = nullptr;  // device-side address of the same memory
+  size_t bytes = 0;
+  smat_status alloc(size_t n);
+  void release();
+  ~HostBuf() { release(); }
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+  template <typename T>
+  T* dev() const { return static_cast<T*>(d); }
+};
+
 }  // namespace smat
 
-#define SMAT_HIP_TRY(expr)                                                        \
+#define SMAT_HIP_TRY(expr)                                                      \
   do {                                                                            \
     hipError_t e_ = (expr);                                                       \
     if (e_ != hipSuccess)                                                         \

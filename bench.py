@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--kernel", default="dense", choices=["dense", "mixed24"])
     ap.add_argument("--dtype", default="f16", choices=["f16", "bf16"])
     ap.add_argument("--single", action="store_true", help="time only --bk/--n-tile (no sweep, no plain leg)")
+    ap.add_argument("--no-ladder", action="store_true", help="skip the precision ladder (F16/BF16/FP8/split)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and preprocessing legs")
     ap.add_argument("--pre-scale", type=int, default=14, help="R-MAT scale of the reorder comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline duration")
@@ -139,6 +140,17 @@ def preprocess_leg(scale: int, tau: float = 0.5):
 
 
 def make_workload(args):
+    """((M, K, rowptr, colidx, FP16 bits, FP32 values), description)."""
+    import numpy as np
+    from smat import gen
+    if args.workload == "powerlaw":
+        return gen.power_law(n=131072, seed=1004, f32=True), (
+            "power-law n=131072, Zipf(1.5) rows <= 4096, N(0,1) values, 1% x1e3")
+    (M, K, rp, ci, v), desc = _make16(args)
+    return (M, K, rp, ci, v, v.view(np.float16).astype(np.float32)), desc
+
+
+def _make16(args):
     from smat import gen
     if args.workload == "rmat":
         return gen.rmat(scale=args.scale, seed=1002), (
@@ -148,7 +160,7 @@ def make_workload(args):
     if args.workload == "clustered":
         return gen.block_clustered(n=65536, tiles=12000, seed=1003), (
             "block-clustered n=65536, 12000 16x16 tiles, every 1x4 group pruned to 2 of 4")
-    return gen.power_law(n=131072, seed=1004), "power-law n=131072, Zipf(1.5) rows <= 4096, 1% x1e3 outliers"
+    raise ValueError(args.workload)
 
 
 class Run:
@@ -215,21 +227,6 @@ class Run:
         tags = self.A.arrays()[3].numpy()
         return int((tags == 2).sum()), int((tags == 1).sum())
 
-    def spot_check(self, rp, ci, v):
-        """64 random rows of C against a float64 product (outside the timing)."""
-        import numpy as np
-        ctx = self.ctx
-        rng = np.random.default_rng(0)
-        rows = rng.choice(ctx.M, size=min(64, ctx.M), replace=False)
-        Bh = ctx.B_host
-        Ch = ctx.C.cpu().numpy()
-        num = den = 0.0
-        for r in rows:
-            ref = ctx.vals_f64[rp[r]:rp[r + 1]] @ Bh[ci[rp[r]:rp[r + 1]]]
-            num += float(np.sum((Ch[r] - ref) ** 2))
-            den += float(np.sum(ref ** 2))
-        return (num / den) ** 0.5 if den > 0 else 0.0
-
     def close(self):
         self.plan.close()
         self.A.close()
@@ -237,6 +234,56 @@ class Run:
 
 class Ctx:
     pass
+
+
+def precision_ladder(ctx, v32, Bf, bk, steps, warmup, log):
+    """The tuned block width in every precision mode, from the FP32 inputs:
+    pack A (preprocessing) and B (reported separately), time the SpMM, and the
+    relative Frobenius error against the FP64 product of the FP32 inputs
+    (sums formed in FP64 on the GPU)."""
+    import torch
+    import smat
+    dev = ctx.C.device
+    A = smat.Bcsr(ctx.M, ctx.K, ctx.d_rp, ctx.d_ci, ctx.d_v, bk=bk)
+    x = smat.scatter_values(A, ctx.d_rp, ctx.d_ci, torch.from_numpy(v32).to(dev))
+    Bd = torch.from_numpy(Bf).to(dev)
+    d = A.desc
+    nnzb = int(d.nnzb)
+    out = []
+    for name, dt in (("f16", smat.SMAT_F16), ("bf16", smat.SMAT_BF16), ("fp8_e4m3", smat.SMAT_FP8_E4M3),
+                     ("split_f16", smat.SMAT_SPLIT_F16)):
+        a, sa = smat.pack_a(A, x, dt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b, sb = smat.pack_b(Bd, dt)
+        torch.cuda.synchronize()
+        pack_b_ms = (time.perf_counter() - t0) * 1e3
+        plan = smat.SpmmPlan(A, ctx.N, in_dtype=dt)
+        call = lambda: plan.execute(b, ctx.C, a_values=a, a_scale=sa, b_scale=sb)  # noqa: E731
+        for _ in range(warmup):
+            call()
+        stream = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(steps):
+            call()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        err = smat.rel_fro_error(ctx.C, ctx.R_32)
+        es, narr = smat.elem_bytes(dt), 2 if dt == smat.SMAT_SPLIT_F16 else 1
+        scales = 4 * (ctx.M // 16 + ctx.N) if dt in (smat.SMAT_FP8_E4M3, smat.SMAT_SPLIT_F16) else 0
+        alg = (nnzb * 16 * bk + ctx.K * ctx.N) * es * narr + 4 * nnzb + 4 * (ctx.M // 16 + 1) + ctx.M * ctx.N * 4 + scales
+        rec = {"mode": name, "bk": bk, "event_ms": round(ms, 4),
+               "useful_gflops": round(2.0 * ctx.nnz * ctx.N / (ms * 1e-3) / 1e9, 2),
+               "achieved_gbs": round(alg / (ms * 1e-3) / 1e9, 1), "rel_fro_err_vs_fp64": err,
+               "pack_b_ms": round(pack_b_ms, 3)}
+        log(f"ladder {rec}")
+        out.append(rec)
+        plan.close()
+    A.close()
+    return out
 
 
 def main():
@@ -252,8 +299,9 @@ def main():
     import torch
     from smat import gen
 
+    import smat
     t0 = time.time()
-    (M, K, rp, ci, v), desc = make_workload(args)
+    (M, K, rp, ci, v, v32), desc = make_workload(args)
     if args.dtype == "bf16":
         v = gen.f32_to_bf16_bits(v.view(np.float16).astype(np.float32))
     ctx = Ctx()
@@ -273,13 +321,25 @@ def main():
     ctx.B = torch.from_numpy(np.ascontiguousarray(Bbits).view(np.int16)).to(dev)
     ctx.C = torch.empty((M, ctx.N), dtype=torch.float32, device=dev)
 
+    # FP64 references on the host (scipy): the product of the rounded inputs
+    # the kernels consume, and of the FP32 inputs (the precision ladder's truth).
+    import scipy.sparse as sp
+    t1 = time.time()
+    A64 = sp.csr_matrix((ctx.vals_f64, ci, rp), shape=(M, K))
+    ctx.R_in = torch.from_numpy(np.ascontiguousarray(A64 @ ctx.B_host)).to(dev)
+    A32 = sp.csr_matrix((v32.astype(np.float64), ci, rp), shape=(M, K))
+    ctx.R_32 = torch.from_numpy(np.ascontiguousarray(A32 @ Bf.astype(np.float64))).to(dev)
+    del A64, A32
+    log(f"FP64 references in {time.time() - t1:.1f}s")
+
     def measure(bk, nt, kernel, steps, warmup):
         r = Run(ctx, bk, nt, kernel).time(steps, warmup)
-        err = r.spot_check(rp, ci, v)
+        err = smat.rel_fro_error(ctx.C, ctx.R_in)
         st = r.stats()
-        st["rel_fro_err_spot"] = err
+        st["rel_fro_err"] = err
+        st["rel_fro_err_vs_fp32_inputs"] = smat.rel_fro_error(ctx.C, ctx.R_32)
         log(f"bk={bk:2d} n_tile={nt or 'auto':>4} {kernel}: {st['event_ms']:.3f} ms, "
-            f"{st['useful_gflops']:.1f} useful GFLOP/s, spot err {err:.2e}")
+            f"{st['useful_gflops']:.1f} useful GFLOP/s, rel err {err:.2e}")
         r.close()
         if not err < 1e-5:
             print(json.dumps({"error": f"SpMM result check failed for bk={bk}: rel error {err}"}))
@@ -303,6 +363,10 @@ def main():
         # re-time the winner with the full step count
         tuned = measure(best["bk"], 0 if best["n_tile"] == "auto" else best["n_tile"], best["kernel"],
                         args.steps, args.warmup)
+
+    ladder = None
+    if not args.no_ladder:
+        ladder = precision_ladder(ctx, v32, Bf, tuned["bk"], args.steps, args.warmup, log)
 
     cpu = pre = None
     if not args.no_cpu:
@@ -335,7 +399,8 @@ def main():
                    "tuning": "exhaustive sweep over block width x N-tile" if not args.single else "fixed"},
         "plain": plain,
         "speedup_vs_plain": round(tuned["useful_gflops"] / plain["useful_gflops"], 3) if plain else None,
-        "rel_fro_err_spot": tuned["rel_fro_err_spot"],
+        "rel_fro_err": tuned["rel_fro_err"],
+        "rel_fro_err_vs_fp32_inputs": tuned["rel_fro_err_vs_fp32_inputs"],
         "stored_block_gflops": tuned["stored_gflops"],
         "build_ms": tuned["build_ms"],
         "roofline": {"bound": "hbm", "achieved": tuned["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -343,6 +408,7 @@ def main():
                      "kernel": "SpMM call (main kernel + split-row reduction)",
                      "alg_bytes_per_call": tuned["alg_bytes"], "event_ms_avg": tuned["event_ms"]},
         "sweep": [{k: s[k] for k in ("bk", "n_tile", "kernel", "event_ms", "useful_gflops")} for s in sweep],
+        "precision_ladder": ladder,
         "cpu_baseline": cpu,
         "preprocess": pre,
     }

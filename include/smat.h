@@ -58,6 +58,14 @@ typedef void* smat_stream;
 #define SMAT_F16 1
 #define SMAT_BF16 2
 #define SMAT_F32 3
+/* OCP FP8 e4m3 (finite, max 448), one byte per element.  A carries one FP32
+   scale per block row, B one per column: C = a_scale[row/16] * b_scale[n] *
+   sum_k qA * qB.  See smat_pack_a / smat_pack_b. */
+#define SMAT_FP8_E4M3 4
+/* Split FP16: every operand x (pre-scaled by a power of two, per block row of
+   A and per column of B) is stored as hi = fp16(x), lo = fp16(x - hi); the
+   kernel accumulates Ahi*Bhi + Ahi*Blo + Alo*Bhi in FP32. */
+#define SMAT_SPLIT_F16 5
 
 /* Layout of the dense operand B (K x N). */
 #define SMAT_B_ROW_MAJOR 0   /* B[k*ldb + n] */
@@ -101,6 +109,16 @@ smat_status smat_bcsr_from_csr(int64_t M, int64_t K, int64_t nnz,
                                smat_stream stream, smat_bcsr* out);
 smat_status smat_bcsr_describe(smat_bcsr A, smat_bcsr_desc* out);
 smat_status smat_bcsr_destroy(smat_bcsr A);
+
+/*
+ * Writes FP32 values of a CSR with the pattern A was built from into A's block
+ * layout: out_values[nnzb * 16 * bk] (zeros where no entry falls).  The
+ * reference keeps half values only (matrix.h:757-775); this is what the FP8
+ * and split precision modes quantise from (smat_pack_a).
+ */
+smat_status smat_bcsr_scatter_values(const smat_bcsr_desc* A, const int32_t* csr_rowptr,
+                                     const int32_t* csr_colidx, const float* csr_values,
+                                     float* out_values, smat_stream stream);
 
 /*
  * Re-tag every block with the exact 2:4 rule (SMAT_TAG_EXACT24) in place.
@@ -204,11 +222,29 @@ typedef struct {
   int32_t n_tile;          /* columns per wave: 16, 32 or 64; 0 = auto */
   int32_t max_blocks_per_item; /* split block rows longer than this; 0 = auto */
   int32_t b_staging;       /* SMAT_BSTAGE_* (row-major B only) */
-  int32_t in_dtype;        /* element type of A values and B: SMAT_F16 (0 = F16) or SMAT_BF16 */
+  int32_t in_dtype;        /* SMAT_F16 (0 = F16), SMAT_BF16, SMAT_FP8_E4M3 or SMAT_SPLIT_F16 */
   int32_t reserved;
 } smat_spmm_opts;
 
 typedef struct smat_plan_t* smat_plan;
+
+/*
+ * Operands of one SpMM call (smat_spmm_execute_ex).  a_values NULL means the
+ * BCSR's own values.  For SMAT_FP8_E4M3 and SMAT_SPLIT_F16 plans, a_values /
+ * a_scale come from smat_pack_a and b / b_scale from smat_pack_b (split
+ * arrays hold hi then lo: A's lo starts nnzb*16*bk elements in, B's K*ldb).
+ */
+typedef struct {
+  const void* a_values;
+  const float* a_scale;        /* [nbrows]; FP8 and split only */
+  const void* b;               /* K x N */
+  const float* b_scale;        /* [N]; FP8 and split only */
+  int64_t ldb;                 /* elements */
+  void* c;                     /* M x N row-major, overwritten */
+  int64_t ldc;
+  const uint16_t* values24;    /* SMAT_KERNEL_MIXED24 only */
+  const void* index24;
+} smat_spmm_operands;
 
 /*
  * Builds the work decomposition (on the GPU) and the scratch for split rows.
@@ -227,7 +263,41 @@ smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N,
 smat_status smat_spmm_execute(smat_plan plan, const uint16_t* values24, const void* index24,
                               const uint16_t* B, int64_t ldb, void* C, int64_t ldc,
                               smat_stream stream);
+/* Same as smat_spmm_execute with every operand explicit; the only entry for
+ * FP8 and split plans. */
+smat_status smat_spmm_execute_ex(smat_plan plan, const smat_spmm_operands* ops, smat_stream stream);
 smat_status smat_spmm_plan_destroy(smat_plan plan);
+
+/* ------------------------------------------------------------------------ */
+/* Precision modes: operand packing and the FP64 error reduction           */
+/* ------------------------------------------------------------------------ */
+/*
+ * FP32 block values (smat_bcsr_scatter_values) -> operand A of `dtype`:
+ *  SMAT_F16 / SMAT_BF16: uint16[nnzb*16*bk], round to nearest even;
+ *  SMAT_FP8_E4M3: uint8[nnzb*16*bk] = e4m3(x / s), s = a_scale[r] =
+ *    max|x| over block row r / 448 (1 for an all-zero row), round to nearest
+ *    even, clamped to +-448;
+ *  SMAT_SPLIT_F16: uint16[2*nnzb*16*bk] (hi, then lo) of x / s with
+ *    s = a_scale[r] = 2^(e-14), e the exponent of max|x| (max|x|/s in
+ *    [2^14, 2^15)), 1 for an all-zero row.
+ */
+smat_status smat_pack_a(const smat_bcsr_desc* A, const float* values_f32, int32_t dtype,
+                        void* a_out, float* a_scale, smat_stream stream);
+/*
+ * FP32 B (K x N, row-major, leading dimension ldb) -> operand B of `dtype`,
+ * row-major with leading dimension N; per-column scales as smat_pack_a uses
+ * per block row (b_scale[N], FP8 and split only).  Split output: hi [K*N]
+ * then lo [K*N].
+ */
+smat_status smat_pack_b(const float* B, int64_t K, int64_t N, int64_t ldb, int32_t dtype,
+                        void* b_out, float* b_scale, smat_stream stream);
+/*
+ * Sums in FP64 on the GPU: out[0] = sum (C - R)^2, out[1] = sum R^2 over a
+ * rows x N window; C FP32 (ldc), R FP64 (ldr).  out: device double[2];
+ * work: device double[2 * 1024].  rel. Frobenius error = sqrt(out0 / out1).
+ */
+smat_status smat_error_sums(const float* C, int64_t ldc, const double* R, int64_t ldr, int64_t rows,
+                            int64_t N, double* out, double* work, smat_stream stream);
 
 /* Number of work items and split rows of a plan (for reports and tests). */
 smat_status smat_spmm_plan_info(smat_plan plan, int64_t* n_items, int64_t* n_split_rows,

@@ -39,6 +39,15 @@ _SIG = {
     "or_get_permutation": (None, [_i64, _p, _p]),
     "or_fixed_size_grouping": (None, [_i64, _p, _i64, _p]),
     "or_collect_blocking_info": (None, [_i64, _i64, _p, _p, _p, _i64, _p, ctypes.POINTER(_f32)]),
+    "or_e4m3_decode": (_f32, [ctypes.c_uint8]),
+    "or_e4m3_encode": (ctypes.c_uint8, [_f32]),
+    "or_f2bf": (ctypes.c_uint16, [_f32]),
+    "or_bf2f": (_f32, [ctypes.c_uint16]),
+    "or_scale_of": (_f32, [_f32, ctypes.c_int]),
+    "or_pack_a": (None, [_i64, ctypes.c_int, _p, _p, ctypes.c_int, _p, _p]),
+    "or_pack_b": (None, [_i64, _i64, _i64, _p, ctypes.c_int, _p, _p]),
+    "or_spmm_bcsr_scaled_f64": (None, [_i64, ctypes.c_int, _p, _p, ctypes.c_int, _p, _p, _i64, _i64, _p, _p,
+                                       _p, _i64]),
 }
 
 _REF_SIG = {
@@ -267,3 +276,38 @@ def read_edgelist(text: str, weighted: bool = True):
     np.add.at(rowptr, r + 1, 1)
     rowptr = np.cumsum(rowptr)
     return nrows, int(max(cols)) + 1, rowptr, np.array(cols, np.int64), np.array(vals, np.float32)
+
+
+# ---- precision modes ---------------------------------------------------------
+F16, BF16, FP8, SPLIT = 1, 2, 4, 5
+
+
+def pack_a(bk, rowptr, values_f32, dt):
+    rowptr = np.ascontiguousarray(rowptr, np.int32)
+    x = np.ascontiguousarray(values_f32, np.float32)
+    nbrows = rowptr.size - 1
+    n = int(rowptr[-1]) * 16 * bk
+    out = np.zeros(max(2 * n if dt == SPLIT else n, 1), np.uint8 if dt == FP8 else np.uint16)
+    scale = np.ones(max(nbrows, 1), np.float32)
+    lib().or_pack_a(nbrows, bk, P(rowptr), P(x) if x.size else None, dt, P(out), P(scale))
+    return out[: (2 * n if dt == SPLIT else n)], scale[:nbrows]
+
+
+def pack_b(B_f32, dt):
+    B = np.ascontiguousarray(B_f32, np.float32)
+    K, N = B.shape
+    out = np.zeros(2 * K * N if dt == SPLIT else K * N, np.uint8 if dt == FP8 else np.uint16)
+    scale = np.ones(N, np.float32)
+    lib().or_pack_b(K, N, N, P(B), dt, P(out), P(scale))
+    return out, scale
+
+
+def spmm_scaled_f64(bk, rowptr, colidx, dt, a, a_scale, K, N, b, b_scale):
+    rowptr = np.ascontiguousarray(rowptr, np.int32)
+    colidx = np.ascontiguousarray(colidx, np.int32)
+    nbrows = rowptr.size - 1
+    C = np.zeros((nbrows * 16, N), np.float64)
+    lib().or_spmm_bcsr_scaled_f64(nbrows, bk, P(rowptr), P(colidx) if colidx.size else None, dt,
+                                  P(np.ascontiguousarray(a)), None if a_scale is None else P(a_scale), K, N,
+                                  P(np.ascontiguousarray(b)), None if b_scale is None else P(b_scale), P(C), N)
+    return C

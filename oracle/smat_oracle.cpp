@@ -29,6 +29,17 @@
 //                              reference test holds their outputs); pinned only
 //                              against FP64 within FP16-accumulation error
 //   or_csr_multiply_f32        dense brute force (FP32)
+//   or_e4m3_encode/decode      all 254 finite codes round-trip; every midpoint
+//                              between neighbouring codes ties to the even code;
+//                              hand values (448 = 0x7e, 1 = 0x38, 2^-9 = 0x01)
+//   or_f2bf                    hand values, ties-to-even cases
+//   or_pack_a / or_pack_b      scale closed forms (FP8 max element -> +-448,
+//                              split max in [2^14, 2^15)); hi + lo reconstructs
+//                              x / s to 2^-22 relative
+//   or_spmm_bcsr_scaled_f64    reduces to or_spmm_bcsr_f64 for unit scales and
+//                              FP16 operands; dense brute force
+// (The precision modes have no reference counterpart: the reference computes
+// in FP16 only, mmio_highlevel.h:608 and ptx.h:20-41.)
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -604,6 +615,160 @@ void or_collect_blocking_info(intT rows, intT cols, const intT* rowptr,
   out[1] = nzblocks;
   out[2] = longest;
   *avg_height = ((float)total_h) / nzblocks;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Precision modes (no reference counterpart; restated from the definitions in
+// include/smat.h: OCP FP8 e4m3, BF16 and FP16 round-to-nearest-even).
+// ---------------------------------------------------------------------------
+extern "C" {
+
+float or_e4m3_decode(uint8_t v) {
+  const int s = v >> 7, e = (v >> 3) & 15, m = v & 7;
+  double r;
+  if (e == 0) r = std::ldexp((double)m, -9);
+  else if (e == 15 && m == 7) return NAN;
+  else r = std::ldexp(1.0 + m / 8.0, e - 7);
+  return (float)(s ? -r : r);
+}
+
+// Round to nearest even, saturating at +-448 (the largest finite e4m3).
+uint8_t or_e4m3_encode(float x) {
+  if (std::isnan(x)) return 0x7f;
+  const uint8_t sign = std::signbit(x) ? 0x80 : 0;
+  double a = std::fabs((double)x);
+  if (a >= 448.0) return sign | 0x7e;
+  if (a < std::ldexp(1.0, -6)) {  // subnormal: multiples of 2^-9; code 8 is 2^-6
+    const double q = std::nearbyint(a * 512.0);
+    return sign | (uint8_t)q;
+  }
+  int e;
+  const double m = std::frexp(a, &e);  // a = m * 2^e, m in [0.5, 1)
+  const int ex = e - 1;                // a = (2m) * 2^ex, 2m in [1, 2)
+  const double q = std::nearbyint((2.0 * m - 1.0) * 8.0);  // 0..8 (8 carries)
+  return sign | (uint8_t)(((ex + 7) << 3) + (int)q);
+}
+
+uint16_t or_f2bf(float x) {
+  uint32_t f;
+  std::memcpy(&f, &x, 4);
+  if ((f & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((f >> 16) | 0x40);
+  const uint32_t r = f + 0x7fffu + ((f >> 16) & 1u);
+  return (uint16_t)(r >> 16);
+}
+
+float or_bf2f(uint16_t b) {
+  const uint32_t f = (uint32_t)b << 16;
+  float out;
+  std::memcpy(&out, &f, 4);
+  return out;
+}
+
+// dtype codes as in smat.h: 1 F16, 2 BF16, 4 FP8 e4m3, 5 split F16
+float or_scale_of(float amax, int dt) {
+  if (!(amax > 0.f)) return 1.f;
+  if (dt == 4) return amax / 448.f;
+  int e;
+  (void)std::frexp(amax, &e);
+  return std::ldexp(1.f, e - 15);
+}
+
+// Operand A of smat_pack_a from FP32 block values (block rows by rowptr).
+void or_pack_a(int64_t nbrows, int bk, const int32_t* rowptr, const float* x, int dt, void* out,
+               float* scale) {
+  const int64_t n = (int64_t)rowptr[nbrows] * 16 * bk;
+  for (int64_t r = 0; r < nbrows; r++) {
+    const int64_t b = (int64_t)rowptr[r] * 16 * bk, e = (int64_t)rowptr[r + 1] * 16 * bk;
+    if (dt == 1 || dt == 2) {
+      for (int64_t i = b; i < e; i++)
+        static_cast<uint16_t*>(out)[i] = dt == 1 ? f2h(x[i]) : or_f2bf(x[i]);
+      continue;
+    }
+    float m = 0.f;
+    for (int64_t i = b; i < e; i++) m = std::max(m, std::fabs(x[i]));
+    const float s = or_scale_of(m, dt);
+    scale[r] = s;
+    for (int64_t i = b; i < e; i++) {
+      const float y = x[i] / s;
+      if (dt == 4) {
+        static_cast<uint8_t*>(out)[i] = or_e4m3_encode(y);
+      } else {
+        const uint16_t hi = f2h(y);
+        static_cast<uint16_t*>(out)[i] = hi;
+        static_cast<uint16_t*>(out)[n + i] = f2h(y - h2f(hi));
+      }
+    }
+  }
+}
+
+// Operand B of smat_pack_b (row-major K x N, leading dimension N).
+void or_pack_b(int64_t K, int64_t N, int64_t ldb, const float* B, int dt, void* out, float* scale) {
+  std::vector<float> s((size_t)N, 1.f);
+  if (dt == 4 || dt == 5)
+    for (int64_t n = 0; n < N; n++) {
+      float m = 0.f;
+      for (int64_t k = 0; k < K; k++) m = std::max(m, std::fabs(B[k * ldb + n]));
+      s[n] = or_scale_of(m, dt);
+      scale[n] = s[n];
+    }
+  for (int64_t k = 0; k < K; k++)
+    for (int64_t n = 0; n < N; n++) {
+      const float v = B[k * ldb + n];
+      const int64_t i = k * N + n;
+      if (dt == 1) static_cast<uint16_t*>(out)[i] = f2h(v);
+      else if (dt == 2) static_cast<uint16_t*>(out)[i] = or_f2bf(v);
+      else if (dt == 4) static_cast<uint8_t*>(out)[i] = or_e4m3_encode(v / s[n]);
+      else {
+        const float y = v / s[n];
+        const uint16_t hi = f2h(y);
+        static_cast<uint16_t*>(out)[i] = hi;
+        static_cast<uint16_t*>(out)[K * N + i] = f2h(y - h2f(hi));
+      }
+    }
+}
+
+// What the kernels compute from packed operands, in FP64:
+//   C = a_scale[row/16] * b_scale[n] * sum over stored k of a(k) * b(k, n),
+// with a * b = hi*hi + hi*lo + lo*hi for split operands (lo*lo is not
+// formed).  Unscaled dtypes (F16, BF16) take scales of 1 (NULL allowed).
+void or_spmm_bcsr_scaled_f64(int64_t nbrows, int bk, const int32_t* rowptr, const int32_t* colidx, int dt,
+                             const void* a, const float* a_scale, int64_t K, int64_t N, const void* b,
+                             const float* b_scale, double* C, int64_t ldc) {
+  const int64_t na = (int64_t)rowptr[nbrows] * 16 * bk;
+  auto dec16 = [&](uint16_t v) -> double { return dt == 2 ? (double)or_bf2f(v) : (double)h2f(v); };
+  std::vector<double> acc((size_t)N);
+  for (int64_t br = 0; br < nbrows; br++)
+    for (int r = 0; r < 16; r++) {
+      std::fill(acc.begin(), acc.end(), 0.0);
+      for (int64_t p = rowptr[br]; p < rowptr[br + 1]; p++)
+        for (int c = 0; c < bk; c++) {
+          const int64_t ia = p * 16 * bk + r * bk + c;
+          const int64_t k = colidx[p] + c;
+          for (int64_t n = 0; n < N; n++) {
+            const int64_t ib = k * N + n;
+            double v;
+            if (dt == 4) {
+              v = (double)or_e4m3_decode(static_cast<const uint8_t*>(a)[ia]) *
+                  (double)or_e4m3_decode(static_cast<const uint8_t*>(b)[ib]);
+            } else if (dt == 5) {
+              const uint16_t* A16 = static_cast<const uint16_t*>(a);
+              const uint16_t* B16 = static_cast<const uint16_t*>(b);
+              const double ah = h2f(A16[ia]), al = h2f(A16[na + ia]);
+              const double bh = h2f(B16[ib]), bl = h2f(B16[K * N + ib]);
+              v = ah * bh + ah * bl + al * bh;
+            } else {
+              v = dec16(static_cast<const uint16_t*>(a)[ia]) * dec16(static_cast<const uint16_t*>(b)[ib]);
+            }
+            acc[n] += v;
+          }
+        }
+      for (int64_t n = 0; n < N; n++) {
+        const double sa = a_scale ? (double)a_scale[br] : 1.0, sb = b_scale ? (double)b_scale[n] : 1.0;
+        C[(br * 16 + r) * ldc + n] = acc[n] * sa * sb;
+      }
+    }
 }
 
 }  // extern "C"

@@ -235,13 +235,90 @@ __global__ __launch_bounds__(256) void k_spmm_dense(
   store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, C, ldc, scratch);
 }
 
+// Element codecs of the LDS kernel.  DT 0 = FP16, 1 = BF16 (A values and B
+// as given); DT 2 = FP8 e4m3 (one byte per element, A scaled per block row,
+// B per column); DT 3 = split FP16 (every operand is a pair hi + lo of FP16
+// arrays, lo stored after hi; C += Ahi*Bhi + Ahi*Blo + Alo*Bhi, with the
+// same power-of-two scales as FP8 uses).
+template <int DT>
+struct Codec {
+  static constexpr int ES = DT == 2 ? 1 : 2;  // bytes per element
+  static constexpr int NARR = DT == 3 ? 2 : 1;  // arrays per operand (hi, lo)
+  static constexpr bool SCALED = DT >= 2;
+};
+
+__device__ inline smat_f4 mma_fp8(uint2 a, uint2 b, smat_f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)__builtin_bit_cast(uint64_t, a),
+                                                    (long)__builtin_bit_cast(uint64_t, b), c, 0, 0, 0);
+}
+
+// One ds_read_b64_tr_b8: a 16-lane group reading an 8 x 16-byte sub-tile
+// (lane L addressing row L/2, bytes 8*(L%2)..+7) receives column L of it,
+// i.e. the 8 k-values of one B column (probed on gfx950).
+__device__ inline uint2 tr8(const uint8_t* addr) {
+  typedef int v2i __attribute__((ext_vector_type(2)));
+  const v2i v = __builtin_amdgcn_ds_read_tr8_b64_v2i32((__attribute__((address_space(3))) v2i*)(addr));
+  return __builtin_bit_cast(uint2, v);
+}
+
+// FP8 counterpart of load_a: 8 bytes (k-slots 8*grp .. 8*grp+7 of row r16)
+// gathered from the 32/BK blocks of the step starting at block p.
+template <int BK>
+__device__ inline uint2 load_a8(const uint8_t* __restrict__ values, int32_t p, int32_t pend, int r16, int grp) {
+  if constexpr (BK >= 8) {
+    const int32_t blk = p + (8 * grp) / BK;
+    if (blk >= pend) return uint2{0, 0};
+    return *reinterpret_cast<const uint2*>(values + (int64_t)blk * 16 * BK + r16 * BK + (8 * grp) % BK);
+  } else if constexpr (BK == 4) {
+    uint32_t w[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int32_t blk = p + 2 * grp + q;
+      w[q] = blk < pend ? *reinterpret_cast<const uint32_t*>(values + (int64_t)blk * 64 + r16 * 4) : 0u;
+    }
+    return uint2{w[0], w[1]};
+  } else if constexpr (BK == 2) {
+    uint32_t w[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int32_t blk = p + 4 * grp + q;
+      w[q] = blk < pend ? *reinterpret_cast<const uint16_t*>(values + (int64_t)blk * 32 + r16 * 2) : 0u;
+    }
+    return uint2{w[0] | (w[1] << 16), w[2] | (w[3] << 16)};
+  } else {
+    uint32_t w[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int32_t blk = p + 8 * grp + q;
+      w[q] = blk < pend ? values[(int64_t)blk * 16 + r16] : 0u;
+    }
+    return uint2{w[0] | (w[1] << 8) | (w[2] << 16) | (w[3] << 24), w[4] | (w[5] << 8) | (w[6] << 16) | (w[7] << 24)};
+  }
+}
+
+// A operand of one step for every codec (hi and, for split, lo).
+template <int BK, int DT>
+__device__ inline void load_a_any(const void* __restrict__ values, int64_t lo_off, int32_t p, int32_t pend,
+                                  int r16, int grp, uint4 (&a)[Codec<DT>::NARR]) {
+  if constexpr (DT == 2) {
+    const uint2 v = load_a8<BK>(static_cast<const uint8_t*>(values), p, pend, r16, grp);
+    a[0] = uint4{v.x, v.y, 0, 0};
+  } else {
+    const uint16_t* v = static_cast<const uint16_t*>(values);
+    a[0] = load_a<BK>(v, p, pend, r16, grp);
+    if constexpr (DT == 3) a[Codec<DT>::NARR - 1] = load_a<BK>(v + lo_off, p, pend, r16, grp);
+  }
+}
+
 // Row-major B through LDS.  Per MFMA step the 32 B rows it reduces over x the
 // wave's 16*NSUB columns are fetched with 128-bit loads (each row segment is
 // contiguous in HBM), stored to a per-wave LDS tile, and read back as MFMA B
-// operands with ds_read_b64_tr_b16: a 16-lane group reading a 4 x 16 sub-tile
-// (lane L addressing row L/4, columns 4*(L%4)..+3) receives column L of it,
-// i.e. B[k0..k0+3][n0+L] — 4 of the 8 k-values the operand layout wants.
-// Two such reads give a lane its 8 values.
+// operands with transposing LDS reads:
+//   * 16-bit elements, ds_read_b64_tr_b16: a 16-lane group reading a 4 x 16
+//     sub-tile (lane L addressing row L/4, columns 4*(L%4)..+3) receives
+//     column L of it, i.e. B[k0..k0+3][n0+L] — 4 of the 8 k-values the
+//     operand layout wants; two such reads give a lane its 8 values;
+//   * FP8, ds_read_b64_tr_b8: one read of an 8 x 16 sub-tile gives all 8.
 //
 // Pipeline (per wave, two register stages X/Y, loop unrolled by two so the
 // stage index is static): while step p is transposed and multiplied, the B
@@ -252,36 +329,55 @@ __global__ __launch_bounds__(256) void k_spmm_dense(
 // before the B loads, the in-order vmcnt wait for an index never drains the
 // other stage's B loads.  Waves never share LDS, so no barrier is needed: a
 // wave's LDS accesses complete in issue order.
-template <int BK, int NSUB, int DT, bool OUT_F16>
-__global__ __launch_bounds__(256) void k_spmm_lds(
-    const Item* __restrict__ items, int64_t n_items, int32_t ntiles,
-    const int32_t* __restrict__ colidx, const uint16_t* __restrict__ values,
-    const uint16_t* __restrict__ B, int64_t ldb, int64_t N, void* __restrict__ C, int64_t ldc,
-    float* __restrict__ scratch) {
-  constexpr int NT = 16 * NSUB;           // columns per wave
-  constexpr int CPR = NT / 8;             // 16-byte chunks per B row segment
-  constexpr int RPL = 64 / CPR;           // B rows per load instruction
-  constexpr int NLD = 32 / RPL;           // load instructions per MFMA step
-  constexpr int PITCH = NT + 8;           // LDS row pitch in halves (16-byte pad)
-  constexpr int STEP = 32 / BK;           // blocks per MFMA step
-  __shared__ __attribute__((aligned(16))) uint16_t lds_all[kWaves * 32 * PITCH];
+//
+// Scaled codecs multiply the FP32 tile by a_scale[block row] * b_scale[col]
+// before it is stored (split rows scale their partial tiles, so the slice
+// reduction sums scaled values).
+// Operands of one SpMM as the kernels see them (element offsets; for split
+// operands the lo array starts a_lo / b_lo elements after the hi array).
+struct Operands {
+  const int32_t* colidx;
+  const void* a;
+  int64_t a_lo;
+  const float* a_scale;
+  const void* b;
+  int64_t ldb;
+  int64_t b_lo;
+  const float* b_scale;
+  int64_t N;
+  void* c;
+  int64_t ldc;
+  float* scratch;
+};
 
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-  const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t task = (int64_t)wg * kWaves + wave;
-  const int64_t item_id = task / ntiles;
-  if (item_id >= n_items) return;
-  const int nt = (int)(task % ntiles);
-  const Item it = items[item_id];
-  const int64_t n0 = (int64_t)nt * NT;
-  uint16_t* lds = lds_all + wave * 32 * PITCH;
+// LDS bytes one wave of the LDS kernel needs.
+template <int NSUB, int DT>
+constexpr int lds_wave_bytes() {
+  return Codec<DT>::NARR * 32 * (16 * NSUB * Codec<DT>::ES + 16);
+}
+
+template <int BK, int NSUB, int DT, bool OUT_F16>
+__device__ __forceinline__ void lds_tile(const Item& it, int64_t n0, const Operands& op, uint8_t* __restrict__ lds,
+                                         int lane) {
+  using CD = Codec<DT>;
+  constexpr int ES = CD::ES, NARR = CD::NARR;
+  constexpr int NT = 16 * NSUB;           // columns per wave
+  constexpr int RB = NT * ES;             // bytes per B row segment
+  constexpr int CPR = RB / 16;            // 16-byte chunks per B row segment
+  constexpr int RPL = 64 / CPR;           // B rows per load instruction
+  static_assert(RPL <= 32, "a load instruction must not span more than one step");
+  constexpr int NLD = 32 / RPL;           // load instructions per MFMA step and array
+  constexpr int PITCH = RB + 16;          // LDS row pitch in bytes (16-byte pad)
+  constexpr int STEP = 32 / BK;           // blocks per MFMA step
+  const int32_t* __restrict__ colidx = op.colidx;
+  const uint8_t* __restrict__ B = static_cast<const uint8_t*>(op.b);
+  const int64_t ldb = op.ldb, N = op.N;
 
   const int r16 = lane & 15, grp = lane >> 4;
   // load coordinates: instruction i covers rows i*RPL .. i*RPL+RPL-1
   const int lrow = lane / CPR, lch = lane % CPR;
-  const int64_t ncol = n0 + lch * 8;
-  const bool col_ok = ncol < N;  // N % 8 == 0: a chunk is all in or all out
+  const int64_t ncol = n0 + lch * (16 / ES);
+  const bool col_ok = ncol < N;  // N is a multiple of the chunk: a chunk is all in or all out
 
   smat_f4 acc[NSUB];
 #pragma unroll
@@ -294,34 +390,53 @@ __global__ __launch_bounds__(256) void k_spmm_lds(
     return b < pe ? colidx[b] : 0;
   };
   // B rows and A values of step p (zeros past the end of the item)
-  auto issue = [&](int32_t p, int32_t cid, uint4 (&bb)[NLD], uint4& aa) {
+  auto issue = [&](int32_t p, int32_t cid, uint4 (&bb)[NARR][NLD], uint4 (&aa)[NARR]) {
     if (p >= pe) return;
 #pragma unroll
     for (int i = 0; i < NLD; i++) {
       const int row = i * RPL + lrow;  // 0..31 = k-slot
       const int boff = row / BK;
       const int32_t c = __shfl(cid, boff);
-      if (p + boff < pe && col_ok)
-        bb[i] = *reinterpret_cast<const uint4*>(B + (int64_t)(c + row % BK) * ldb + ncol);
-      else
-        bb[i] = uint4{0, 0, 0, 0};
-    }
-    aa = load_a<BK>(values, p, pe, r16, grp);
-  };
-  auto consume = [&](const uint4 (&bb)[NLD]) {
+      const bool ok = p + boff < pe && col_ok;
+      const int64_t off = ((int64_t)(c + row % BK) * ldb + ncol) * ES;
 #pragma unroll
-    for (int i = 0; i < NLD; i++)
-      *reinterpret_cast<uint4*>(lds + (i * RPL + lrow) * PITCH + lch * 8) = bb[i];
+      for (int h = 0; h < NARR; h++)
+        bb[h][i] = ok ? *reinterpret_cast<const uint4*>(B + off + h * op.b_lo * ES) : uint4{0, 0, 0, 0};
+    }
+    load_a_any<BK, DT>(op.a, op.a_lo, p, pe, r16, grp, aa);
   };
-  auto compute = [&](const uint4 a) {
+  auto consume = [&](const uint4 (&bb)[NARR][NLD]) {
+#pragma unroll
+    for (int h = 0; h < NARR; h++)
+#pragma unroll
+      for (int i = 0; i < NLD; i++)
+        *reinterpret_cast<uint4*>(lds + (h * 32 + i * RPL + lrow) * PITCH + lch * 16) = bb[h][i];
+  };
+  auto compute = [&](const uint4 (&a)[NARR]) {
 #pragma unroll
     for (int j = 0; j < NSUB; j++) {
-      const uint16_t* base = lds + (8 * grp + (r16 >> 2)) * PITCH + 16 * j + (r16 & 3) * 4;
-      acc[j] = mma<DT>(a, tr_pair(base, base + 4 * PITCH), acc[j]);
+      if constexpr (DT == 2) {
+        const uint8_t* base = lds + (8 * grp + (r16 >> 1)) * PITCH + 16 * j + (r16 & 1) * 8;
+        acc[j] = mma_fp8(uint2{a[0].x, a[0].y}, tr8(base), acc[j]);
+      } else {
+        const uint8_t* base = lds + (8 * grp + (r16 >> 2)) * PITCH + (16 * j + (r16 & 3) * 4) * 2;
+        const uint4 bh = tr_pair(reinterpret_cast<const uint16_t*>(base),
+                                 reinterpret_cast<const uint16_t*>(base + 4 * PITCH));
+        if constexpr (DT == 3) {
+          const uint8_t* lb = base + 32 * PITCH;
+          const uint4 bl = tr_pair(reinterpret_cast<const uint16_t*>(lb),
+                                   reinterpret_cast<const uint16_t*>(lb + 4 * PITCH));
+          acc[j] = mma<0>(a[1], bh, acc[j]);  // small terms first
+          acc[j] = mma<0>(a[0], bl, acc[j]);
+          acc[j] = mma<0>(a[0], bh, acc[j]);
+        } else {
+          acc[j] = mma<DT>(a[0], bh, acc[j]);
+        }
+      }
     }
   };
 
-  uint4 bX[NLD], bY[NLD], aX, aY;
+  uint4 bX[NARR][NLD], bY[NARR][NLD], aX[NARR], aY[NARR];
   const int32_t pb = it.pbeg;
   int32_t cX = cload(pb);
   int32_t cY = cload(pb + STEP);
@@ -330,19 +445,48 @@ __global__ __launch_bounds__(256) void k_spmm_lds(
   issue(pb + STEP, cY, bY, aY);
   cY = cload(pb + 3 * STEP);
   for (int32_t p = pb; p < pe; p += 2 * STEP) {
+    uint4 a[NARR];
     consume(bX);
-    uint4 a = aX;
+#pragma unroll
+    for (int h = 0; h < NARR; h++) a[h] = aX[h];
     issue(p + 2 * STEP, cX, bX, aX);
     cX = cload(p + 4 * STEP);
     compute(a);
     if (p + STEP >= pe) break;
     consume(bY);
-    a = aY;
+#pragma unroll
+    for (int h = 0; h < NARR; h++) a[h] = aY[h];
     issue(p + 3 * STEP, cY, bY, aY);
     cY = cload(p + 5 * STEP);
     compute(a);
   }
-  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, C, ldc, scratch);
+  if constexpr (CD::SCALED) {
+    const float sa = op.a_scale[it.brow];
+#pragma unroll
+    for (int j = 0; j < NSUB; j++) {
+      const int64_t n = n0 + 16 * j + r16;
+      const float sc = sa * (n < N ? op.b_scale[n] : 0.f);
+#pragma unroll
+      for (int i = 0; i < 4; i++) acc[j][i] *= sc;
+    }
+  }
+  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, op.c, op.ldc, op.scratch);
+}
+
+template <int BK, int NSUB, int DT, bool OUT_F16>
+__global__ __launch_bounds__(256) void k_spmm_lds(const Item* __restrict__ items, int64_t n_items, int32_t ntiles,
+                                                  Operands op) {
+  constexpr int WB = lds_wave_bytes<NSUB, DT>();
+  __shared__ __attribute__((aligned(16))) uint8_t lds_all[kWaves * WB];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t task = (int64_t)wg * kWaves + wave;
+  const int64_t item_id = task / ntiles;
+  if (item_id >= n_items) return;
+  const int nt = (int)(task % ntiles);
+  const Item it = items[item_id];
+  lds_tile<BK, NSUB, DT, OUT_F16>(it, (int64_t)nt * 16 * NSUB, op, lds_all + wave * WB, lane);
 }
 
 // Mixed dense / 2:4 kernel for 16x16 blocks and row-major B.  Dense blocks
@@ -595,83 +739,109 @@ namespace {
 inline int32_t ntiles_of(const smat_plan_t* P, int nsub) { return (int32_t)((P->N + 16 * nsub - 1) / (16 * nsub)); }
 
 template <int BK, int NSUB, int BLAYOUT, int DT, bool OUT_F16>
-void launch_dense(const smat_plan_t* P, const uint16_t* B, int64_t ldb, void* C, int64_t ldc, hipStream_t s) {
+void launch_dense(const smat_plan_t* P, const Operands& op, hipStream_t s) {
   const int32_t ntiles = ntiles_of(P, NSUB);
   hipLaunchKernelGGL((k_spmm_dense<BK, NSUB, BLAYOUT, DT, OUT_F16>), blocks_for(P->n_items * ntiles, kWaves),
-                     64 * kWaves, 0, s, P->items.as<Item>(), P->n_items, ntiles, P->A.colidx, P->A.values, B,
-                     ldb, P->N, C, ldc, P->scratch.as<float>());
+                     64 * kWaves, 0, s, P->items.as<Item>(), P->n_items, ntiles, op.colidx,
+                     static_cast<const uint16_t*>(op.a), static_cast<const uint16_t*>(op.b), op.ldb, op.N, op.c,
+                     op.ldc, op.scratch);
 }
 
 template <int BK, int NSUB, int DT, bool OUT_F16>
-void launch_lds(const smat_plan_t* P, const uint16_t* B, int64_t ldb, void* C, int64_t ldc, hipStream_t s) {
+void launch_lds(const smat_plan_t* P, const Operands& op, hipStream_t s) {
   const int32_t ntiles = ntiles_of(P, NSUB);
   hipLaunchKernelGGL((k_spmm_lds<BK, NSUB, DT, OUT_F16>), blocks_for(P->n_items * ntiles, kWaves), 64 * kWaves,
-                     0, s, P->items.as<Item>(), P->n_items, ntiles, P->A.colidx, P->A.values, B, ldb, P->N, C,
-                     ldc, P->scratch.as<float>());
+                     0, s, P->items.as<Item>(), P->n_items, ntiles, op);
 }
 
 template <int NSUB, int DT, bool OUT_F16>
-void launch_mixed(const smat_plan_t* P, const uint16_t* v24, const uint16_t* i24, const uint16_t* B, int64_t ldb,
-                  void* C, int64_t ldc, hipStream_t s) {
+void launch_mixed(const smat_plan_t* P, const uint16_t* v24, const uint16_t* i24, const Operands& op,
+                  hipStream_t s) {
   const int32_t ntiles = ntiles_of(P, NSUB);
   hipLaunchKernelGGL((k_spmm_mixed<NSUB, DT, OUT_F16>), blocks_for(P->n_items * ntiles, kWaves), 64 * kWaves, 0,
-                     s, P->items.as<Item>(), P->n_items, ntiles, P->A.colidx, P->A.values,
-                     P->dense_ids.as<int32_t>(), P->sparse_ids.as<int32_t>(), v24, i24, B, ldb, P->N, C, ldc,
-                     P->scratch.as<float>());
+                     s, P->items.as<Item>(), P->n_items, ntiles, op.colidx, static_cast<const uint16_t*>(op.a),
+                     P->dense_ids.as<int32_t>(), P->sparse_ids.as<int32_t>(), v24, i24,
+                     static_cast<const uint16_t*>(op.b), op.ldb, op.N, op.c, op.ldc, op.scratch);
+}
+
+inline bool lds_aligned(const smat_plan_t* P, const Operands& op, int elems_per_chunk) {
+  return P->N % elems_per_chunk == 0 && op.ldb % elems_per_chunk == 0 &&
+         (reinterpret_cast<uintptr_t>(op.b) & 15) == 0;
 }
 
 template <int BK, int BLAYOUT, int DT, bool OUT_F16>
-void dispatch_nsub(const smat_plan_t* P, const uint16_t* B, int64_t ldb, void* C, int64_t ldc, hipStream_t s) {
-  const bool lds_ok = (P->N % 8 == 0) && (ldb % 8 == 0) && ((reinterpret_cast<uintptr_t>(B) & 15) == 0);
-  if (BLAYOUT == SMAT_B_ROW_MAJOR && P->opts.b_staging != SMAT_BSTAGE_DIRECT && lds_ok) {
+void dispatch_nsub(const smat_plan_t* P, const Operands& op, hipStream_t s) {
+  if (BLAYOUT == SMAT_B_ROW_MAJOR && P->opts.b_staging != SMAT_BSTAGE_DIRECT && lds_aligned(P, op, 8)) {
     switch (P->nsub) {
-      case 1: launch_lds<BK, 1, DT, OUT_F16>(P, B, ldb, C, ldc, s); return;
-      case 2: launch_lds<BK, 2, DT, OUT_F16>(P, B, ldb, C, ldc, s); return;
-      default: launch_lds<BK, 4, DT, OUT_F16>(P, B, ldb, C, ldc, s); return;
+      case 1: launch_lds<BK, 1, DT, OUT_F16>(P, op, s); return;
+      case 2: launch_lds<BK, 2, DT, OUT_F16>(P, op, s); return;
+      default: launch_lds<BK, 4, DT, OUT_F16>(P, op, s); return;
     }
   }
   switch (P->nsub) {
-    case 1: launch_dense<BK, 1, BLAYOUT, DT, OUT_F16>(P, B, ldb, C, ldc, s); break;
-    case 2: launch_dense<BK, 2, BLAYOUT, DT, OUT_F16>(P, B, ldb, C, ldc, s); break;
-    default: launch_dense<BK, 4, BLAYOUT, DT, OUT_F16>(P, B, ldb, C, ldc, s); break;
+    case 1: launch_dense<BK, 1, BLAYOUT, DT, OUT_F16>(P, op, s); break;
+    case 2: launch_dense<BK, 2, BLAYOUT, DT, OUT_F16>(P, op, s); break;
+    default: launch_dense<BK, 4, BLAYOUT, DT, OUT_F16>(P, op, s); break;
   }
 }
 
 template <int BK, int DT>
-void dispatch_layout(const smat_plan_t* P, const uint16_t* B, int64_t ldb, void* C, int64_t ldc, hipStream_t s) {
+void dispatch_layout(const smat_plan_t* P, const Operands& op, hipStream_t s) {
   const bool f16 = P->opts.out_dtype == SMAT_F16;
   if (P->opts.b_layout == SMAT_B_REFERENCE) {
-    if (f16) dispatch_nsub<BK, SMAT_B_REFERENCE, DT, true>(P, B, ldb, C, ldc, s);
-    else dispatch_nsub<BK, SMAT_B_REFERENCE, DT, false>(P, B, ldb, C, ldc, s);
+    if (f16) dispatch_nsub<BK, SMAT_B_REFERENCE, DT, true>(P, op, s);
+    else dispatch_nsub<BK, SMAT_B_REFERENCE, DT, false>(P, op, s);
   } else {
-    if (f16) dispatch_nsub<BK, SMAT_B_ROW_MAJOR, DT, true>(P, B, ldb, C, ldc, s);
-    else dispatch_nsub<BK, SMAT_B_ROW_MAJOR, DT, false>(P, B, ldb, C, ldc, s);
+    if (f16) dispatch_nsub<BK, SMAT_B_ROW_MAJOR, DT, true>(P, op, s);
+    else dispatch_nsub<BK, SMAT_B_ROW_MAJOR, DT, false>(P, op, s);
   }
 }
 
-template <int DT>
-void dispatch_bk(const smat_plan_t* P, const uint16_t* B, int64_t ldb, void* C, int64_t ldc, hipStream_t s) {
+// FP8 and split codecs: row-major B through LDS only, FP32 C.  FP8 tiles are
+// at least 32 columns wide (a 16-byte load then spans at most 32 B rows).
+template <int BK, int DT>
+void dispatch_scaled(const smat_plan_t* P, const Operands& op, hipStream_t s) {
+  if (DT == 2) {
+    if (P->nsub == 4) launch_lds<BK, 4, 2, false>(P, op, s);
+    else launch_lds<BK, 2, 2, false>(P, op, s);
+  } else {
+    switch (P->nsub) {
+      case 1: launch_lds<BK, 1, 3, false>(P, op, s); break;
+      case 2: launch_lds<BK, 2, 3, false>(P, op, s); break;
+      default: launch_lds<BK, 4, 3, false>(P, op, s); break;
+    }
+  }
+}
+
+template <int BK>
+void dispatch_dt(const smat_plan_t* P, const Operands& op, hipStream_t s) {
+  switch (P->opts.in_dtype) {
+    case SMAT_BF16: dispatch_layout<BK, 1>(P, op, s); break;
+    case SMAT_FP8_E4M3: dispatch_scaled<BK, 2>(P, op, s); break;
+    case SMAT_SPLIT_F16: dispatch_scaled<BK, 3>(P, op, s); break;
+    default: dispatch_layout<BK, 0>(P, op, s); break;
+  }
+}
+
+void dispatch_bk(const smat_plan_t* P, const Operands& op, hipStream_t s) {
   switch (P->A.bk) {
-    case 1: dispatch_layout<1, DT>(P, B, ldb, C, ldc, s); break;
-    case 2: dispatch_layout<2, DT>(P, B, ldb, C, ldc, s); break;
-    case 4: dispatch_layout<4, DT>(P, B, ldb, C, ldc, s); break;
-    case 8: dispatch_layout<8, DT>(P, B, ldb, C, ldc, s); break;
-    case 16: dispatch_layout<16, DT>(P, B, ldb, C, ldc, s); break;
-    default: dispatch_layout<32, DT>(P, B, ldb, C, ldc, s); break;
+    case 1: dispatch_dt<1>(P, op, s); break;
+    case 2: dispatch_dt<2>(P, op, s); break;
+    case 4: dispatch_dt<4>(P, op, s); break;
+    case 8: dispatch_dt<8>(P, op, s); break;
+    case 16: dispatch_dt<16>(P, op, s); break;
+    default: dispatch_dt<32>(P, op, s); break;
   }
 }
 
 template <int DT>
-void dispatch_mixed(const smat_plan_t* P, const uint16_t* v24, const uint16_t* i24, const uint16_t* B, int64_t ldb,
-                    void* C, int64_t ldc, hipStream_t s) {
+void dispatch_mixed(const smat_plan_t* P, const uint16_t* v24, const uint16_t* i24, const Operands& op,
+                    hipStream_t s) {
   const bool f16 = P->opts.out_dtype == SMAT_F16;
   switch (P->nsub) {
-    case 1: f16 ? launch_mixed<1, DT, true>(P, v24, i24, B, ldb, C, ldc, s)
-                : launch_mixed<1, DT, false>(P, v24, i24, B, ldb, C, ldc, s); break;
-    case 2: f16 ? launch_mixed<2, DT, true>(P, v24, i24, B, ldb, C, ldc, s)
-                : launch_mixed<2, DT, false>(P, v24, i24, B, ldb, C, ldc, s); break;
-    default: f16 ? launch_mixed<4, DT, true>(P, v24, i24, B, ldb, C, ldc, s)
-                 : launch_mixed<4, DT, false>(P, v24, i24, B, ldb, C, ldc, s); break;
+    case 1: f16 ? launch_mixed<1, DT, true>(P, v24, i24, op, s) : launch_mixed<1, DT, false>(P, v24, i24, op, s); break;
+    case 2: f16 ? launch_mixed<2, DT, true>(P, v24, i24, op, s) : launch_mixed<2, DT, false>(P, v24, i24, op, s); break;
+    default: f16 ? launch_mixed<4, DT, true>(P, v24, i24, op, s) : launch_mixed<4, DT, false>(P, v24, i24, op, s); break;
   }
 }
 
@@ -768,7 +938,14 @@ smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat
   if (o.out_dtype == 0) o.out_dtype = SMAT_F32;
   if (o.in_dtype == 0) o.in_dtype = SMAT_F16;
   SMAT_REQUIRE(o.out_dtype == SMAT_F32 || o.out_dtype == SMAT_F16, "smat_spmm_plan_create: out_dtype");
-  SMAT_REQUIRE(o.in_dtype == SMAT_F16 || o.in_dtype == SMAT_BF16, "smat_spmm_plan_create: in_dtype");
+  SMAT_REQUIRE(o.in_dtype == SMAT_F16 || o.in_dtype == SMAT_BF16 || o.in_dtype == SMAT_FP8_E4M3 ||
+                   o.in_dtype == SMAT_SPLIT_F16,
+               "smat_spmm_plan_create: unknown in_dtype %d", o.in_dtype);
+  if (o.in_dtype == SMAT_FP8_E4M3 || o.in_dtype == SMAT_SPLIT_F16)
+    SMAT_REQUIRE(o.kernel != SMAT_KERNEL_MIXED24 && o.b_layout == SMAT_B_ROW_MAJOR && o.out_dtype == SMAT_F32 &&
+                     o.b_staging != SMAT_BSTAGE_DIRECT,
+                 "smat_spmm_plan_create: FP8 and split inputs need the dense kernel, row-major B staged "
+                 "through LDS and FP32 C");
   SMAT_REQUIRE(o.b_layout == SMAT_B_ROW_MAJOR || o.b_layout == SMAT_B_REFERENCE,
                "smat_spmm_plan_create: b_layout");
   SMAT_REQUIRE(o.kernel == SMAT_KERNEL_AUTO || o.kernel == SMAT_KERNEL_DENSE || o.kernel == SMAT_KERNEL_MIXED24,
@@ -787,7 +964,8 @@ smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat
   P->N = N;
   P->opts = o;
   P->mixed = o.kernel == SMAT_KERNEL_MIXED24;
-  const int nt = o.n_tile ? o.n_tile : (N >= 64 ? 64 : (N > 16 ? 32 : 16));
+  int nt = o.n_tile ? o.n_tile : (N >= 64 ? 64 : (N > 16 ? 32 : 16));
+  if (o.in_dtype == SMAT_FP8_E4M3 && nt < 32) nt = 32;  // FP8 tiles are at least 32 columns wide
   P->nsub = nt / 16;
   const int32_t step = 32 / A->bk;  // blocks per MFMA step
   int32_t S = o.max_blocks_per_item;  // in blocks
@@ -811,42 +989,71 @@ smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat
   return SMAT_OK;
 }
 
-smat_status smat_spmm_execute(smat_plan P, const uint16_t* values24, const void* index24, const uint16_t* B,
-                              int64_t ldb, void* C, int64_t ldc, smat_stream stream) {
-  SMAT_REQUIRE(P && B && C, "smat_spmm_execute: null argument");
+smat_status smat_spmm_execute_ex(smat_plan P, const smat_spmm_operands* ops, smat_stream stream) {
+  SMAT_REQUIRE(P && ops && ops->b && ops->c, "smat_spmm_execute: null argument");
+  const int64_t ldb = ops->ldb, ldc = ops->ldc;
+  const int dt = P->opts.in_dtype;
+  const bool scaled = dt == SMAT_FP8_E4M3 || dt == SMAT_SPLIT_F16;
   SMAT_REQUIRE(ldc >= P->N, "smat_spmm_execute: ldc < N");
   if (P->opts.b_layout == SMAT_B_ROW_MAJOR)
     SMAT_REQUIRE(ldb >= P->N, "smat_spmm_execute: ldb < N for row-major B");
   else
     SMAT_REQUIRE(ldb >= P->A.K, "smat_spmm_execute: ldb < K for reference-layout B");
-  const bool aligned = P->N % 8 == 0 && ldb % 8 == 0 && (reinterpret_cast<uintptr_t>(B) & 15) == 0;
-  if ((P->opts.b_staging == SMAT_BSTAGE_LDS && P->opts.b_layout == SMAT_B_ROW_MAJOR) || P->mixed)
-    SMAT_REQUIRE(aligned, "smat_spmm_execute: LDS staging needs N %% 8 == 0, ldb %% 8 == 0 and a 16-byte aligned B");
+  const void* a = ops->a_values ? ops->a_values : P->A.values;
+  SMAT_REQUIRE(a, "smat_spmm_execute: no A values");
+  Operands op{P->A.colidx, a, 0, ops->a_scale, ops->b, ldb, 0, ops->b_scale, P->N, ops->c, ldc,
+              P->scratch.as<float>()};
+  const int chunk = dt == SMAT_FP8_E4M3 ? 16 : 8;  // elements per 16-byte B load
+  const bool aligned = lds_aligned(P, op, chunk);
+  if ((P->opts.b_staging == SMAT_BSTAGE_LDS && P->opts.b_layout == SMAT_B_ROW_MAJOR) || P->mixed || scaled)
+    SMAT_REQUIRE(aligned, "smat_spmm_execute: LDS staging needs N and ldb multiples of %d and a 16-byte aligned B",
+                 chunk);
+  if (scaled) {
+    SMAT_REQUIRE(ops->a_scale && ops->b_scale, "smat_spmm_execute: FP8 and split operands need a_scale and b_scale");
+    SMAT_REQUIRE((reinterpret_cast<uintptr_t>(a) & 15) == 0, "smat_spmm_execute: A values must be 16-byte aligned");
+    if (dt == SMAT_SPLIT_F16) {
+      op.a_lo = P->A.nnzb * 16 * P->A.bk;  // [hi | lo], smat_pack_a
+      op.b_lo = P->A.K * ldb;               // [hi | lo], smat_pack_b
+    }
+  }
   if (P->mixed)
-    SMAT_REQUIRE(values24 && index24 && (reinterpret_cast<uintptr_t>(values24) & 15) == 0,
+    SMAT_REQUIRE(ops->values24 && ops->index24 && (reinterpret_cast<uintptr_t>(ops->values24) & 15) == 0,
                  "smat_spmm_execute: the 2:4 kernel needs 16-byte aligned values24 and index24");
   hipStream_t s = smat::to_hip(stream);
-  const bool bf16 = P->opts.in_dtype == SMAT_BF16;
   if (P->mixed) {
-    const uint16_t* i24 = static_cast<const uint16_t*>(index24);
-    if (bf16) dispatch_mixed<1>(P, values24, i24, B, ldb, C, ldc, s);
-    else dispatch_mixed<0>(P, values24, i24, B, ldb, C, ldc, s);
+    const uint16_t* i24 = static_cast<const uint16_t*>(ops->index24);
+    if (dt == SMAT_BF16) dispatch_mixed<1>(P, ops->values24, i24, op, s);
+    else dispatch_mixed<0>(P, ops->values24, i24, op, s);
   } else {
-    if (bf16) dispatch_bk<1>(P, B, ldb, C, ldc, s);
-    else dispatch_bk<0>(P, B, ldb, C, ldc, s);
+    dispatch_bk(P, op, s);
   }
   if (P->n_red > 0) {
     const int32_t chunks = (int32_t)((16 * P->N + 255) / 256);
     const unsigned g = (unsigned)(chunks * P->n_red);
     if (P->opts.out_dtype == SMAT_F16)
       hipLaunchKernelGGL(k_reduce<true>, g, 256, 0, s, P->red.as<Reduce>(), P->n_red, chunks,
-                         P->scratch.as<float>(), P->N, C, ldc);
+                         P->scratch.as<float>(), P->N, ops->c, ldc);
     else
       hipLaunchKernelGGL(k_reduce<false>, g, 256, 0, s, P->red.as<Reduce>(), P->n_red, chunks,
-                         P->scratch.as<float>(), P->N, C, ldc);
+                         P->scratch.as<float>(), P->N, ops->c, ldc);
   }
   SMAT_LAUNCH_CHECK();
   return SMAT_OK;
+}
+
+smat_status smat_spmm_execute(smat_plan P, const uint16_t* values24, const void* index24, const uint16_t* B,
+                              int64_t ldb, void* C, int64_t ldc, smat_stream stream) {
+  SMAT_REQUIRE(P, "smat_spmm_execute: null plan");
+  SMAT_REQUIRE(P->opts.in_dtype == SMAT_F16 || P->opts.in_dtype == SMAT_BF16,
+               "smat_spmm_execute: FP8 and split plans take their scales through smat_spmm_execute_ex");
+  smat_spmm_operands ops{};
+  ops.b = B;
+  ops.ldb = ldb;
+  ops.c = C;
+  ops.ldc = ldc;
+  ops.values24 = values24;
+  ops.index24 = index24;
+  return smat_spmm_execute_ex(P, &ops, stream);
 }
 
 smat_status smat_spmm_plan_destroy(smat_plan P) {

@@ -24,6 +24,7 @@ SMAT_ERR_NOT_24 = -4
 SMAT_ERR_UNSUPPORTED = -5
 
 SMAT_F16, SMAT_BF16, SMAT_F32 = 1, 2, 3
+SMAT_FP8_E4M3, SMAT_SPLIT_F16 = 4, 5
 SMAT_B_ROW_MAJOR, SMAT_B_REFERENCE = 0, 1
 SMAT_TAG_REFERENCE, SMAT_TAG_EXACT24 = 0, 1
 SMAT_24_REFERENCE_K16, SMAT_24_SMFMAC = 0, 1
@@ -40,6 +41,13 @@ class BcsrDesc(ctypes.Structure):
     _fields_ = [
         ("M", _i64), ("K", _i64), ("bm", _i32), ("bk", _i32), ("nbrows", _i64),
         ("nnzb", _i64), ("rowptr", _p), ("colidx", _p), ("values", _p), ("tags", _p),
+    ]
+
+
+class SpmmOperands(ctypes.Structure):
+    _fields_ = [
+        ("a_values", _p), ("a_scale", _p), ("b", _p), ("b_scale", _p), ("ldb", _i64), ("c", _p),
+        ("ldc", _i64), ("values24", _p), ("index24", _p),
     ]
 
 
@@ -70,7 +78,12 @@ SIGNATURES = {
     "smat_spmm_plan_create": (_i32, [ctypes.POINTER(BcsrDesc), _i64, ctypes.POINTER(SpmmOpts), _p,
                                      ctypes.POINTER(_p)]),
     "smat_spmm_execute": (_i32, [_p, _p, _p, _p, _i64, _p, _i64, _p]),
+    "smat_spmm_execute_ex": (_i32, [_p, ctypes.POINTER(SpmmOperands), _p]),
     "smat_spmm_plan_destroy": (_i32, [_p]),
+    "smat_bcsr_scatter_values": (_i32, [ctypes.POINTER(BcsrDesc), _p, _p, _p, _p, _p]),
+    "smat_pack_a": (_i32, [ctypes.POINTER(BcsrDesc), _p, _i32, _p, _p, _p]),
+    "smat_pack_b": (_i32, [_p, _i64, _i64, _i64, _i32, _p, _p, _p]),
+    "smat_error_sums": (_i32, [_p, _i64, _p, _i64, _i64, _i64, _p, _p, _p]),
     "smat_spmm_plan_info": (_i32, [_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "smat_reorder_grouping": (_i32, [_i64, _p, _p, _p, _p, ctypes.c_float, _i64, _i64, _p, _p, _p]),
     "smat_fixed_size_grouping": (_i32, [_i64, _p, _i64, _p]),
@@ -270,6 +283,18 @@ class SpmmPlan:
                                       ldc, stream_ptr(stream)))
         return C
 
+    def execute(self, B, C, *, a_values=None, a_scale=None, b_scale=None, ldb: int | None = None,
+                ldc: int | None = None, values24=None, index24=None, stream=None):
+        """smat_spmm_execute_ex: every operand explicit (FP8 / split plans)."""
+        if ldb is None:
+            ldb = self.N
+        if ldc is None:
+            ldc = C.shape[-1]
+        ops = SpmmOperands(ptr(a_values), ptr(a_scale), ptr(B), ptr(b_scale), ldb, ptr(C), ldc, ptr(values24),
+                           ptr(index24))
+        check(lib().smat_spmm_execute_ex(self.handle, ctypes.byref(ops), stream_ptr(stream)))
+        return C
+
     def close(self):
         if self.handle:
             lib().smat_spmm_plan_destroy(self.handle)
@@ -315,3 +340,59 @@ def grouping_to_permutation(grouping):
     out = np.zeros_like(g)
     check(lib().smat_grouping_to_permutation(g.size, g.ctypes.data, out.ctypes.data))
     return out
+
+
+# ---- precision modes --------------------------------------------------------
+def elem_bytes(dtype: int) -> int:
+    return 1 if dtype == SMAT_FP8_E4M3 else 2
+
+
+def scatter_values(A: Bcsr, csr_rowptr, csr_colidx, csr_values_f32, stream=None):
+    """FP32 values of a device CSR -> A's block layout (torch float32 tensor)."""
+    import torch
+    d = A.desc
+    out = torch.empty(max(d.nnzb * 16 * d.bk, 1), dtype=torch.float32, device="cuda")
+    check(lib().smat_bcsr_scatter_values(ctypes.byref(d), ptr(csr_rowptr), ptr(csr_colidx), ptr(csr_values_f32),
+                                         ptr(out), stream_ptr(stream)))
+    return out
+
+
+def pack_a(A: Bcsr, values_f32, dtype: int, stream=None):
+    """Operand A of `dtype` (uint8 / int16 torch tensor) and its per-block-row
+    scales (float32, FP8 and split only; None otherwise)."""
+    import torch
+    d = A.desc
+    n = d.nnzb * 16 * d.bk
+    cnt = 2 * n if dtype == SMAT_SPLIT_F16 else n
+    out = torch.empty(max(cnt, 8), dtype=torch.uint8 if dtype == SMAT_FP8_E4M3 else torch.int16, device="cuda")
+    scale = None
+    if dtype in (SMAT_FP8_E4M3, SMAT_SPLIT_F16):
+        scale = torch.empty(max(d.nbrows, 1), dtype=torch.float32, device="cuda")
+    check(lib().smat_pack_a(ctypes.byref(d), ptr(values_f32), dtype, ptr(out), ptr(scale), stream_ptr(stream)))
+    return out, scale
+
+
+def pack_b(B_f32, dtype: int, stream=None):
+    """FP32 B (K x N torch tensor) -> operand B of `dtype` and per-column scales."""
+    import torch
+    K, N = B_f32.shape
+    cnt = 2 * K * N if dtype == SMAT_SPLIT_F16 else K * N
+    out = torch.empty(cnt, dtype=torch.uint8 if dtype == SMAT_FP8_E4M3 else torch.int16, device="cuda")
+    scale = None
+    if dtype in (SMAT_FP8_E4M3, SMAT_SPLIT_F16):
+        scale = torch.empty(N, dtype=torch.float32, device="cuda")
+    check(lib().smat_pack_b(ptr(B_f32), K, N, B_f32.stride(0), dtype, ptr(out), ptr(scale), stream_ptr(stream)))
+    return out, scale
+
+
+def rel_fro_error(C, R, stream=None) -> float:
+    """||C - R||_F / ||R||_F with the sums formed in FP64 on the GPU
+    (C float32, R float64, both rows x N device tensors)."""
+    import torch
+    rows, N = C.shape
+    out = torch.zeros(2, dtype=torch.float64, device="cuda")
+    work = torch.empty(2 * 1024, dtype=torch.float64, device="cuda")
+    check(lib().smat_error_sums(ptr(C), C.stride(0), ptr(R), R.stride(0), rows, N, ptr(out), ptr(work),
+                                stream_ptr(stream)))
+    e2, r2 = out.cpu().tolist()
+    return (e2 / r2) ** 0.5 if r2 > 0 else (0.0 if e2 == 0 else float("inf"))

@@ -26,14 +26,15 @@ def pad_to(x: int, m: int) -> int:
     return (x + m - 1) // m * m
 
 
-def _to_csr(n_rows: int, n_cols: int, rows: np.ndarray, cols: np.ndarray, vals: np.ndarray):
+def _to_csr(n_rows: int, n_cols: int, rows: np.ndarray, cols: np.ndarray, vals: np.ndarray, f32: bool = False):
     order = np.lexsort((cols, rows))
     rows, cols, vals = rows[order], cols[order], vals[order]
     M, K = pad_to(n_rows, 16), pad_to(n_cols, 32)
     counts = np.bincount(rows, minlength=M)
     rowptr = np.zeros(M + 1, dtype=np.int64)
     np.cumsum(counts, out=rowptr[1:])
-    return M, K, rowptr.astype(np.int32), cols.astype(np.int32), vals.astype(np.float16).view(np.uint16)
+    out = (M, K, rowptr.astype(np.int32), cols.astype(np.int32), vals.astype(np.float16).view(np.uint16))
+    return out + (vals.astype(np.float32),) if f32 else out
 
 
 def _uniform_vals(rng: np.random.Generator, n: int) -> np.ndarray:
@@ -102,8 +103,11 @@ def block_clustered(n: int = 65536, tiles: int = 12000, seed: int = 1003, prune:
 
 
 def power_law(n: int = 131072, s: float = 1.5, max_len: int = 4096, seed: int = 1004,
-              outlier_frac: float = 0.01, outlier_scale: float = 1e3):
-    """Returns the CSR of FP16 values plus the FP32 values (for split modes)."""
+              outlier_frac: float = 0.01, outlier_scale: float = 1e3, f32: bool = False):
+    """Power-law rows (Zipf(s) lengths, at most max_len, distinct uniform
+    columns), values N(0, 1) with a fraction scaled by outlier_scale.  Returns
+    the CSR with FP16 values, and with f32=True also the FP32 values (the
+    exact inputs of the precision modes)."""
     rng = np.random.default_rng(seed)
     lens = rng.zipf(s, size=n)
     lens = np.minimum(lens, max_len).astype(np.int64)
@@ -117,7 +121,7 @@ def power_law(n: int = 131072, s: float = 1.5, max_len: int = 4096, seed: int = 
     out = rng.random(rows.size) < outlier_frac
     vals[out] *= outlier_scale
     vals[vals == 0] = 1.0
-    return _to_csr(n, n, rows, cols, vals)
+    return _to_csr(n, n, rows, cols, vals, f32)
 
 
 def _distinct_small(rng: np.random.Generator, n: int, L: int) -> np.ndarray:

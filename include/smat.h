@@ -244,6 +244,9 @@ typedef struct {
   int64_t ldc;
   const uint16_t* values24;    /* SMAT_KERNEL_MIXED24 only */
   const void* index24;
+  const int32_t* c_rowmap;     /* NULL, or [M]: row r of A is row c_rowmap[r] of C (A's rows
+                                  reordered, e.g. by smat_reorder_grouping; C keeps the
+                                  original order) */
 } smat_spmm_operands;
 
 /*
@@ -298,6 +301,31 @@ smat_status smat_pack_b(const float* B, int64_t K, int64_t N, int64_t ldb, int32
  */
 smat_status smat_error_sums(const float* C, int64_t ldc, const double* R, int64_t ldr, int64_t rows,
                             int64_t N, double* out, double* work, smat_stream stream);
+
+/* ------------------------------------------------------------------------ */
+/* Batches: many SpMMs in one persistent launch                              */
+/* ------------------------------------------------------------------------ */
+/*
+ * A batch runs `count` planned SpMMs (each with its own block width, N-tile
+ * and operands) as one persistent kernel: a grid sized to fill the GPU
+ * draws (matrix, work item, N-tile) tasks from one device counter, then one
+ * kernel reduces the split rows of every matrix.  Plans must be dense-kernel
+ * plans with the same 16-bit input type (F16 or BF16), row-major B staged
+ * through LDS and FP32 C; the plans and every operand buffer must outlive the
+ * batch (their addresses are baked in).  No reference counterpart: the
+ * reference launches one kernel per matrix (tester.h:944-1011).
+ */
+typedef struct smat_batch_t* smat_batch;
+smat_status smat_batch_create(int32_t count, const smat_plan* plans, const smat_spmm_operands* ops,
+                              smat_stream stream, smat_batch* out);
+/* Captures the batch's launches (counter reset, queue kernel, split-row
+ * reduction; no memset nodes) into a HIP graph on `stream` (not the default
+ * stream); later smat_batch_execute calls replay the graph. */
+smat_status smat_batch_capture(smat_batch batch, smat_stream stream);
+smat_status smat_batch_execute(smat_batch batch, smat_stream stream);
+smat_status smat_batch_info(smat_batch batch, int64_t* tasks, int64_t* split_rows, int32_t* grid,
+                            int32_t* captured);
+smat_status smat_batch_destroy(smat_batch batch);
 
 /* Number of work items and split rows of a plan (for reports and tests). */
 smat_status smat_spmm_plan_info(smat_plan plan, int64_t* n_items, int64_t* n_split_rows,

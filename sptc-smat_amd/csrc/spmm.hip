@@ -37,6 +37,9 @@
 //    of tasks, keeping neighbouring block rows in one L2.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+#include <vector>
+
 #include "smat_internal.h"
 
 namespace {
@@ -148,10 +151,12 @@ __device__ inline uint4 load_a(const uint16_t* __restrict__ values, int32_t p, i
   }
 }
 
-// D layout: lane holds rows 4*grp + i, column 16*j + r16 of its tile.
+// D layout: lane holds rows 4*grp + i, column 16*j + r16 of its tile.  With a
+// row map (A's rows reordered), row r of the tile goes to C row rowmap[r].
 template <int NSUB, bool OUT_F16>
 __device__ inline void store_tile(const smat_f4 (&acc)[NSUB], const Item& it, int64_t n0, int r16, int grp,
-                                  int64_t N, void* __restrict__ C, int64_t ldc, float* __restrict__ scratch) {
+                                  int64_t N, void* __restrict__ C, int64_t ldc, float* __restrict__ scratch,
+                                  const int32_t* __restrict__ rowmap) {
 #pragma unroll
   for (int j = 0; j < NSUB; j++) {
     const int64_t n = n0 + 16 * j + r16;
@@ -162,7 +167,8 @@ __device__ inline void store_tile(const smat_f4 (&acc)[NSUB], const Item& it, in
       if (it.slot >= 0) {
         scratch[((int64_t)it.slot * 16 + row) * N + n] = acc[j][i];
       } else {
-        const int64_t crow = (int64_t)it.brow * 16 + row;
+        int64_t crow = (int64_t)it.brow * 16 + row;
+        if (rowmap) crow = rowmap[crow];
         if (OUT_F16)
           reinterpret_cast<_Float16*>(C)[crow * ldc + n] = (_Float16)acc[j][i];
         else
@@ -180,7 +186,7 @@ __global__ __launch_bounds__(256) void k_spmm_dense(
     const Item* __restrict__ items, int64_t n_items, int32_t ntiles,
     const int32_t* __restrict__ colidx, const uint16_t* __restrict__ values,
     const uint16_t* __restrict__ B, int64_t ldb, int64_t N, void* __restrict__ C, int64_t ldc,
-    float* __restrict__ scratch) {
+    float* __restrict__ scratch, const int32_t* __restrict__ rowmap) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
@@ -232,7 +238,7 @@ __global__ __launch_bounds__(256) void k_spmm_dense(
       acc[j] = mma<DT>(a, b, acc[j]);
     }
   }
-  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, C, ldc, scratch);
+  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, C, ldc, scratch, rowmap);
 }
 
 // Element codecs of the LDS kernel.  DT 0 = FP16, 1 = BF16 (A values and B
@@ -348,6 +354,7 @@ struct Operands {
   void* c;
   int64_t ldc;
   float* scratch;
+  const int32_t* rowmap;  // NULL, or the C row of each A row
 };
 
 // LDS bytes one wave of the LDS kernel needs.
@@ -470,7 +477,7 @@ __device__ __forceinline__ void lds_tile(const Item& it, int64_t n0, const Opera
       for (int i = 0; i < 4; i++) acc[j][i] *= sc;
     }
   }
-  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, op.c, op.ldc, op.scratch);
+  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, op.c, op.ldc, op.scratch, op.rowmap);
 }
 
 template <int BK, int NSUB, int DT, bool OUT_F16>
@@ -500,7 +507,7 @@ __global__ __launch_bounds__(256) void k_spmm_mixed(
     const uint16_t* __restrict__ values, const int32_t* __restrict__ dense_ids,
     const int32_t* __restrict__ sparse_ids, const uint16_t* __restrict__ values24,
     const uint16_t* __restrict__ index24, const uint16_t* __restrict__ B, int64_t ldb, int64_t N,
-    void* __restrict__ C, int64_t ldc, float* __restrict__ scratch) {
+    void* __restrict__ C, int64_t ldc, float* __restrict__ scratch, const int32_t* __restrict__ rowmap) {
   constexpr int NT = 16 * NSUB;
   constexpr int CPR = NT / 8;
   constexpr int RPL = 64 / CPR;
@@ -607,7 +614,7 @@ __global__ __launch_bounds__(256) void k_spmm_mixed(
       }
     }
   }
-  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, C, ldc, scratch);
+  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, C, ldc, scratch, rowmap);
 }
 
 struct Reduce {
@@ -620,7 +627,8 @@ struct Reduce {
 // Sums the partial tiles of each split block row in slice order.
 template <bool OUT_F16>
 __global__ void k_reduce(const Reduce* __restrict__ red, int64_t n_red, int32_t chunks,
-                         const float* __restrict__ scratch, int64_t N, void* __restrict__ C, int64_t ldc) {
+                         const float* __restrict__ scratch, int64_t N, void* __restrict__ C, int64_t ldc,
+                         const int32_t* __restrict__ rowmap) {
   const int64_t r = blockIdx.x / chunks;
   if (r >= n_red) return;
   const Reduce rd = red[r];
@@ -630,11 +638,147 @@ __global__ void k_reduce(const Reduce* __restrict__ red, int64_t n_red, int32_t 
     const int64_t row = i / N, n = i % N;
     float s = 0.f;
     for (int q = 0; q < rd.slot_cnt; q++) s += scratch[((int64_t)(rd.slot_beg + q) * 16 + row) * N + n];
-    const int64_t crow = (int64_t)rd.brow * 16 + row;
+    int64_t crow = (int64_t)rd.brow * 16 + row;
+    if (rowmap) crow = rowmap[crow];
     if (OUT_F16)
       reinterpret_cast<_Float16*>(C)[crow * ldc + n] = (_Float16)s;
     else
       reinterpret_cast<float*>(C)[crow * ldc + n] = s;
+  }
+}
+
+// ---- batched persistent launch -----------------------------------------------
+// Many SpMMs (each with its own plan: block width, N-tile, work items) in one
+// persistent launch per N-tile width present.  Every wave of a grid sized to
+// fill the GPU once takes the next tasks (matrix, work item, N-tile) from a
+// device counter, finds their matrix by binary search over the matrices'
+// first task indices, and runs the LDS tile body of that matrix's block
+// width.  Uneven matrices thus share one dynamic queue instead of one launch
+// (and one tail) each.  The kernel is instantiated per tile width because one
+// instance holding every (block width, tile width) body would take the
+// register budget of the largest and halve the occupancy of the rest.
+struct BatchMat {
+  Operands op;
+  const Item* items;
+  int64_t task_beg;  // global index of this matrix's first task
+  int32_t ntiles;
+  int32_t kind;      // block-width code 0..5 for 1, 2, 4, 8, 16, 32
+};
+
+struct BatchRed {
+  int32_t mat;
+  int32_t brow;
+  int32_t slot_beg;
+  int32_t slot_cnt;
+};
+
+__global__ void k_batch_reset(unsigned long long* __restrict__ counters) {
+  if (threadIdx.x < 24) counters[threadIdx.x] = 0ull;  // 3 tile widths x 8 XCD ranges
+}
+
+// Wave-uniform copies (scalar registers) of values every lane holds equally.
+// readfirstlane returns a signed int: each half goes through uint32_t before
+// widening, or a low word with bit 31 set would sign-extend into the high word.
+__device__ inline int64_t uni(int64_t v) {
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  return (int64_t)(((uint64_t)hi << 32) | (uint64_t)lo);
+}
+template <typename T>
+__device__ inline T* uni(T* p) {
+  return reinterpret_cast<T*>(uni((int64_t)reinterpret_cast<uintptr_t>(p)));
+}
+__device__ inline Operands uni(const Operands& o) {
+  Operands u;
+  u.colidx = uni(o.colidx);
+  u.a = uni(o.a);
+  u.a_lo = uni(o.a_lo);
+  u.a_scale = uni(o.a_scale);
+  u.b = uni(o.b);
+  u.ldb = uni(o.ldb);
+  u.b_lo = uni(o.b_lo);
+  u.b_scale = uni(o.b_scale);
+  u.N = uni(o.N);
+  u.c = uni(o.c);
+  u.ldc = uni(o.ldc);
+  u.scratch = uni(o.scratch);
+  u.rowmap = uni(o.rowmap);
+  return u;
+}
+
+template <int DT, int NS>
+__device__ __forceinline__ void batch_tile(int bkc, const Item& it, int nt, const Operands& op, uint8_t* lds,
+                                           int lane) {
+  const int64_t n0 = (int64_t)nt * 16 * NS;
+  switch (bkc) {
+    case 0: lds_tile<1, NS, DT, false>(it, n0, op, lds, lane); break;
+    case 1: lds_tile<2, NS, DT, false>(it, n0, op, lds, lane); break;
+    case 2: lds_tile<4, NS, DT, false>(it, n0, op, lds, lane); break;
+    case 3: lds_tile<8, NS, DT, false>(it, n0, op, lds, lane); break;
+    case 4: lds_tile<16, NS, DT, false>(it, n0, op, lds, lane); break;
+    default: lds_tile<32, NS, DT, false>(it, n0, op, lds, lane); break;
+  }
+}
+
+// Persistent queue kernel.  The group's tasks are cut into 8 contiguous
+// ranges, one per XCD (workgroup i runs on XCD i % 8), each with its own
+// counter: neighbouring block rows then share an L2, and the atomics spread
+// over 8 addresses.  A wave draws `chunk` consecutive tasks per atomic from
+// its XCD's range, finds the first task's matrix by binary search and walks
+// forward through the chunk; when its range is drained it takes chunks from
+// the other ranges in turn.  Exit: counters only grow, so after at most one
+// failed draw per range every wave leaves the loop.
+template <int DT, int NS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NS == 4 ? 3 : 4)))
+void k_batch(const BatchMat* __restrict__ mats, int32_t nmats, int64_t total, int64_t chunk,
+             unsigned long long* __restrict__ counters) {
+  constexpr int WB = lds_wave_bytes<NS, DT>();
+  constexpr int NX = 8;
+  __shared__ __attribute__((aligned(16))) uint8_t lds_all[kWaves * WB];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  uint8_t* lds = lds_all + wave * WB;
+  const int xcd = (int)(blockIdx.x % NX);
+  for (int q = 0; q < NX;) {
+    const int r = (xcd + q) % NX;
+    const int64_t rb = total * r / NX, re = total * (r + 1) / NX;
+    unsigned long long t = 0;
+    if (lane == 0) t = atomicAdd(&counters[r], (unsigned long long)chunk);
+    const int64_t t0 = rb + uni((int64_t)t);  // lane 0's draw, wave-uniform
+    if (t0 >= re) {
+      q++;  // this range is drained
+      continue;
+    }
+    const int64_t t1 = t0 + chunk < re ? t0 + chunk : re;
+    int32_t lo = 0, hi = nmats - 1;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (mats[mid].task_beg <= t0) lo = mid;
+      else hi = mid - 1;
+    }
+    for (int64_t task = t0; task < t1; task++) {
+      while (lo + 1 < nmats && mats[lo + 1].task_beg <= task) lo++;
+      const BatchMat& m = mats[lo];
+      const uint32_t local = (uint32_t)(task - uni(m.task_beg));  // < 2^31 per matrix
+      const uint32_t ntl = (uint32_t)__builtin_amdgcn_readfirstlane(m.ntiles);
+      const Item it = uni(m.items)[local / ntl];
+      batch_tile<DT, NS>(__builtin_amdgcn_readfirstlane(m.kind), it, (int)(local % ntl), uni(m.op), lds, lane);
+    }
+  }
+}
+
+// Split-row reduction over every matrix of the batch (one workgroup per row).
+__global__ void k_batch_reduce(const BatchRed* __restrict__ red, const BatchMat* __restrict__ mats) {
+  const BatchRed rd = red[blockIdx.x];
+  const Operands op = mats[rd.mat].op;
+  const int64_t N = op.N, total = 16 * N;
+  for (int64_t i = threadIdx.x; i < total; i += blockDim.x) {
+    const int64_t row = i / N, n = i % N;
+    float acc = 0.f;
+    for (int q = 0; q < rd.slot_cnt; q++) acc += op.scratch[((int64_t)(rd.slot_beg + q) * 16 + row) * N + n];
+    int64_t crow = (int64_t)rd.brow * 16 + row;
+    if (op.rowmap) crow = op.rowmap[crow];
+    static_cast<float*>(op.c)[crow * op.ldc + n] = acc;
   }
 }
 
@@ -744,7 +888,7 @@ void launch_dense(const smat_plan_t* P, const Operands& op, hipStream_t s) {
   hipLaunchKernelGGL((k_spmm_dense<BK, NSUB, BLAYOUT, DT, OUT_F16>), blocks_for(P->n_items * ntiles, kWaves),
                      64 * kWaves, 0, s, P->items.as<Item>(), P->n_items, ntiles, op.colidx,
                      static_cast<const uint16_t*>(op.a), static_cast<const uint16_t*>(op.b), op.ldb, op.N, op.c,
-                     op.ldc, op.scratch);
+                     op.ldc, op.scratch, op.rowmap);
 }
 
 template <int BK, int NSUB, int DT, bool OUT_F16>
@@ -761,7 +905,7 @@ void launch_mixed(const smat_plan_t* P, const uint16_t* v24, const uint16_t* i24
   hipLaunchKernelGGL((k_spmm_mixed<NSUB, DT, OUT_F16>), blocks_for(P->n_items * ntiles, kWaves), 64 * kWaves, 0,
                      s, P->items.as<Item>(), P->n_items, ntiles, op.colidx, static_cast<const uint16_t*>(op.a),
                      P->dense_ids.as<int32_t>(), P->sparse_ids.as<int32_t>(), v24, i24,
-                     static_cast<const uint16_t*>(op.b), op.ldb, op.N, op.c, op.ldc, op.scratch);
+                     static_cast<const uint16_t*>(op.b), op.ldb, op.N, op.c, op.ldc, op.scratch, op.rowmap);
 }
 
 inline bool lds_aligned(const smat_plan_t* P, const Operands& op, int elems_per_chunk) {
@@ -970,11 +1114,13 @@ smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat
   const int32_t step = 32 / A->bk;  // blocks per MFMA step
   int32_t S = o.max_blocks_per_item;  // in blocks
   if (S <= 0) {
-    // aim for >= 64k wave tasks, never below 64 blocks per slice
+    // aim for >= 64k wave tasks, but never slices under 12 MFMA steps: on
+    // mid-size matrices finer slices only add partial tiles to reduce
     const int64_t ntl = (N + nt - 1) / nt;
     const int64_t target_items = (65536 + ntl - 1) / ntl;
     const int64_t s_auto = (A->nnzb + target_items - 1) / target_items;
-    S = (int32_t)(s_auto < 64 ? 64 : (s_auto > 4096 ? 4096 : s_auto));
+    const int64_t s_min = 12 * (int64_t)step;
+    S = (int32_t)(s_auto < s_min ? s_min : (s_auto > 4096 ? 4096 : s_auto));
   }
   smat_status st;
   if (P->mixed)
@@ -1002,7 +1148,7 @@ smat_status smat_spmm_execute_ex(smat_plan P, const smat_spmm_operands* ops, sma
   const void* a = ops->a_values ? ops->a_values : P->A.values;
   SMAT_REQUIRE(a, "smat_spmm_execute: no A values");
   Operands op{P->A.colidx, a, 0, ops->a_scale, ops->b, ldb, 0, ops->b_scale, P->N, ops->c, ldc,
-              P->scratch.as<float>()};
+              P->scratch.as<float>(), ops->c_rowmap};
   const int chunk = dt == SMAT_FP8_E4M3 ? 16 : 8;  // elements per 16-byte B load
   const bool aligned = lds_aligned(P, op, chunk);
   if ((P->opts.b_staging == SMAT_BSTAGE_LDS && P->opts.b_layout == SMAT_B_ROW_MAJOR) || P->mixed || scaled)
@@ -1032,10 +1178,10 @@ smat_status smat_spmm_execute_ex(smat_plan P, const smat_spmm_operands* ops, sma
     const unsigned g = (unsigned)(chunks * P->n_red);
     if (P->opts.out_dtype == SMAT_F16)
       hipLaunchKernelGGL(k_reduce<true>, g, 256, 0, s, P->red.as<Reduce>(), P->n_red, chunks,
-                         P->scratch.as<float>(), P->N, ops->c, ldc);
+                         P->scratch.as<float>(), P->N, ops->c, ldc, ops->c_rowmap);
     else
       hipLaunchKernelGGL(k_reduce<false>, g, 256, 0, s, P->red.as<Reduce>(), P->n_red, chunks,
-                         P->scratch.as<float>(), P->N, ops->c, ldc);
+                         P->scratch.as<float>(), P->N, ops->c, ldc, ops->c_rowmap);
   }
   SMAT_LAUNCH_CHECK();
   return SMAT_OK;
@@ -1066,6 +1212,206 @@ smat_status smat_spmm_plan_info(smat_plan P, int64_t* n_items, int64_t* n_split_
   if (n_items) *n_items = P->n_items;
   if (n_split_rows) *n_split_rows = P->n_red;
   if (scratch_bytes) *scratch_bytes = (int64_t)P->scratch.bytes;
+  return SMAT_OK;
+}
+
+}  // extern "C"
+
+struct smat_batch_t {
+  int32_t count = 0;
+  int32_t dt = 0;
+  int64_t total = 0, n_red = 0;
+  int32_t g_beg[3] = {0, 0, 0}, g_cnt[3] = {0, 0, 0};  // matrices per N-tile width 16, 32, 64
+  int64_t g_tasks[3] = {0, 0, 0};
+  unsigned grid[3] = {0, 0, 0};
+  smat::DevBuf mats, red, counter;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~smat_batch_t() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+namespace {
+
+template <int DT, int NS>
+unsigned batch_grid() {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch<DT, NS>, 64 * kWaves, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    return 1024;
+  }
+  return (unsigned)(cus * (per < 1 ? 1 : per));
+}
+
+template <int DT>
+void batch_group(const smat_batch_t* b, int g, hipStream_t s) {
+  // about 16 draws per wave: few atomics, still fine-grained balance at the end
+  const int64_t waves = (int64_t)b->grid[g] * kWaves;
+  const int64_t chunk = std::max<int64_t>(1, b->g_tasks[g] / (waves * 16));
+  const BatchMat* m = b->mats.as<BatchMat>() + b->g_beg[g];
+  unsigned long long* c = b->counter.as<unsigned long long>() + 8 * g;
+  if (g == 0) hipLaunchKernelGGL((k_batch<DT, 1>), b->grid[g], 64 * kWaves, 0, s, m, b->g_cnt[g], b->g_tasks[g], chunk, c);
+  else if (g == 1) hipLaunchKernelGGL((k_batch<DT, 2>), b->grid[g], 64 * kWaves, 0, s, m, b->g_cnt[g], b->g_tasks[g], chunk, c);
+  else hipLaunchKernelGGL((k_batch<DT, 4>), b->grid[g], 64 * kWaves, 0, s, m, b->g_cnt[g], b->g_tasks[g], chunk, c);
+}
+
+smat_status batch_launch(smat_batch_t* b, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_reset, 1, 64, 0, s, b->counter.as<unsigned long long>());
+  for (int g = 0; g < 3; g++) {
+    if (b->g_cnt[g] == 0) continue;
+    if (b->dt == SMAT_BF16) batch_group<1>(b, g, s);
+    else batch_group<0>(b, g, s);
+  }
+  if (b->n_red > 0)
+    hipLaunchKernelGGL(k_batch_reduce, (unsigned)b->n_red, 256, 0, s, b->red.as<BatchRed>(), b->mats.as<BatchMat>());
+  SMAT_LAUNCH_CHECK();
+  return SMAT_OK;
+}
+
+int bk_code(int bk) { return bk == 1 ? 0 : bk == 2 ? 1 : bk == 4 ? 2 : bk == 8 ? 3 : bk == 16 ? 4 : 5; }
+
+}  // namespace
+
+extern "C" {
+
+smat_status smat_batch_create(int32_t count, const smat_plan* plans, const smat_spmm_operands* ops,
+                              smat_stream stream, smat_batch* out) {
+  SMAT_REQUIRE(out && count > 0 && plans && ops, "smat_batch_create: bad argument");
+  *out = nullptr;
+  hipStream_t s = smat::to_hip(stream);
+  std::vector<BatchMat> mats((size_t)count);
+  std::vector<BatchRed> reds;
+  const int dt = plans[0] ? plans[0]->opts.in_dtype : 0;
+  // matrices ordered by N-tile width (16, 32, 64), input order inside a group
+  std::vector<int32_t> order;
+  int32_t g_cnt[3] = {0, 0, 0};
+  for (int g = 0; g < 3; g++)
+    for (int32_t i = 0; i < count; i++)
+      if (plans[i] && (plans[i]->nsub == 1 ? 0 : plans[i]->nsub == 2 ? 1 : 2) == g) {
+        order.push_back(i);
+        g_cnt[g]++;
+      }
+  SMAT_REQUIRE((int32_t)order.size() == count, "smat_batch_create: null plan");
+  int64_t g_tasks[3] = {0, 0, 0};
+  for (int32_t pos = 0; pos < count; pos++) {
+    const int32_t i = order[(size_t)pos];
+    const smat_plan_t* P = plans[i];
+    const smat_spmm_operands& o = ops[i];
+    const int g = P->nsub == 1 ? 0 : P->nsub == 2 ? 1 : 2;
+    SMAT_REQUIRE(P && o.b && o.c, "smat_batch_create: entry %d: null plan or operand", i);
+    SMAT_REQUIRE(!P->mixed && P->opts.in_dtype == dt && (dt == SMAT_F16 || dt == SMAT_BF16) &&
+                     P->opts.b_layout == SMAT_B_ROW_MAJOR && P->opts.out_dtype == SMAT_F32 &&
+                     P->opts.b_staging != SMAT_BSTAGE_DIRECT,
+                 "smat_batch_create: entry %d: batches take dense-kernel plans with one 16-bit input type, "
+                 "row-major B staged through LDS and FP32 C",
+                 i);
+    SMAT_REQUIRE(o.ldb >= P->N && o.ldc >= P->N, "smat_batch_create: entry %d: ldb / ldc < N", i);
+    const void* a = o.a_values ? o.a_values : P->A.values;
+    BatchMat& m = mats[(size_t)pos];
+    m.op = Operands{P->A.colidx, a, 0, nullptr, o.b, o.ldb, 0, nullptr, P->N, o.c, o.ldc, P->scratch.as<float>(),
+                    o.c_rowmap};
+    SMAT_REQUIRE(lds_aligned(P, m.op, 8), "smat_batch_create: entry %d: N and ldb must be multiples of 8 and B "
+                                          "16-byte aligned", i);
+    m.items = P->items.as<Item>();
+    m.ntiles = ntiles_of(P, P->nsub);
+    m.kind = bk_code(P->A.bk);
+    m.task_beg = g_tasks[g];  // within its group
+    g_tasks[g] += P->n_items * (int64_t)m.ntiles;
+    if (P->n_red > 0) {
+      std::vector<Reduce> r((size_t)P->n_red);
+      SMAT_HIP_TRY(hipMemcpyAsync(r.data(), P->red.p, r.size() * sizeof(Reduce), hipMemcpyDeviceToHost, s));
+      SMAT_HIP_TRY(hipStreamSynchronize(s));
+      for (const Reduce& x : r) reds.push_back(BatchRed{pos, x.brow, x.slot_beg, x.slot_cnt});
+    }
+  }
+  smat_batch_t* B = new smat_batch_t();
+  B->count = count;
+  B->dt = dt;
+  B->total = g_tasks[0] + g_tasks[1] + g_tasks[2];
+  for (int g = 0, beg = 0; g < 3; g++) {
+    B->g_beg[g] = beg;
+    B->g_cnt[g] = g_cnt[g];
+    B->g_tasks[g] = g_tasks[g];
+    beg += g_cnt[g];
+  }
+  B->n_red = (int64_t)reds.size();
+  smat_status st = B->mats.alloc(mats.size() * sizeof(BatchMat));
+  if (st == SMAT_OK) st = B->red.alloc((reds.size() + 1) * sizeof(BatchRed));
+  if (st == SMAT_OK) st = B->counter.alloc(24 * sizeof(unsigned long long));
+  if (st != SMAT_OK) {
+    delete B;
+    return st;
+  }
+  hipError_t e = hipMemcpyAsync(B->mats.p, mats.data(), mats.size() * sizeof(BatchMat), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && !reds.empty())
+    e = hipMemcpyAsync(B->red.p, reds.data(), reds.size() * sizeof(BatchRed), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    delete B;
+    return smat::set_error(SMAT_ERR_HIP, "smat_batch_create: copy failed: %s", hipGetErrorString(e));
+  }
+  B->grid[0] = dt == SMAT_BF16 ? batch_grid<1, 1>() : batch_grid<0, 1>();
+  B->grid[1] = dt == SMAT_BF16 ? batch_grid<1, 2>() : batch_grid<0, 2>();
+  B->grid[2] = dt == SMAT_BF16 ? batch_grid<1, 4>() : batch_grid<0, 4>();
+  *out = B;
+  return SMAT_OK;
+}
+
+smat_status smat_batch_capture(smat_batch B, smat_stream stream) {
+  SMAT_REQUIRE(B, "smat_batch_capture: null batch");
+  SMAT_REQUIRE(stream, "smat_batch_capture: capture needs a non-default stream");
+  hipStream_t s = smat::to_hip(stream);
+  if (B->exec) {
+    (void)hipGraphExecDestroy(B->exec);
+    B->exec = nullptr;
+  }
+  if (B->graph) {
+    (void)hipGraphDestroy(B->graph);
+    B->graph = nullptr;
+  }
+  SMAT_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const smat_status st = batch_launch(B, s);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &g);
+  if (st != SMAT_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess) return smat::set_error(SMAT_ERR_HIP, "smat_batch_capture: %s", hipGetErrorString(e));
+  B->graph = g;
+  SMAT_HIP_TRY(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+  return SMAT_OK;
+}
+
+smat_status smat_batch_execute(smat_batch B, smat_stream stream) {
+  SMAT_REQUIRE(B, "smat_batch_execute: null batch");
+  hipStream_t s = smat::to_hip(stream);
+  if (B->exec) {
+    SMAT_HIP_TRY(hipGraphLaunch(B->exec, s));
+    return SMAT_OK;
+  }
+  return batch_launch(B, s);
+}
+
+smat_status smat_batch_info(smat_batch B, int64_t* tasks, int64_t* split_rows, int32_t* grid, int32_t* captured) {
+  SMAT_REQUIRE(B, "smat_batch_info: null batch");
+  if (tasks) *tasks = B->total;
+  if (split_rows) *split_rows = B->n_red;
+  if (grid) {  // waves the largest group's launch keeps resident
+    int32_t gmax = 0;
+    for (int g = 0; g < 3; g++)
+      if (B->g_cnt[g] > 0 && (int32_t)B->grid[g] > gmax) gmax = (int32_t)B->grid[g];
+    *grid = gmax;
+  }
+  if (captured) *captured = B->exec != nullptr;
+  return SMAT_OK;
+}
+
+smat_status smat_batch_destroy(smat_batch B) {
+  delete B;
   return SMAT_OK;
 }
 

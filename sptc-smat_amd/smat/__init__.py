@@ -47,7 +47,7 @@ class BcsrDesc(ctypes.Structure):
 class SpmmOperands(ctypes.Structure):
     _fields_ = [
         ("a_values", _p), ("a_scale", _p), ("b", _p), ("b_scale", _p), ("ldb", _i64), ("c", _p),
-        ("ldc", _i64), ("values24", _p), ("index24", _p),
+        ("ldc", _i64), ("values24", _p), ("index24", _p), ("c_rowmap", _p),
     ]
 
 
@@ -81,6 +81,12 @@ SIGNATURES = {
     "smat_spmm_execute_ex": (_i32, [_p, ctypes.POINTER(SpmmOperands), _p]),
     "smat_spmm_plan_destroy": (_i32, [_p]),
     "smat_bcsr_scatter_values": (_i32, [ctypes.POINTER(BcsrDesc), _p, _p, _p, _p, _p]),
+    "smat_batch_create": (_i32, [_i32, _p, _p, _p, ctypes.POINTER(_p)]),
+    "smat_batch_capture": (_i32, [_p, _p]),
+    "smat_batch_execute": (_i32, [_p, _p]),
+    "smat_batch_info": (_i32, [_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i32),
+                               ctypes.POINTER(_i32)]),
+    "smat_batch_destroy": (_i32, [_p]),
     "smat_pack_a": (_i32, [ctypes.POINTER(BcsrDesc), _p, _i32, _p, _p, _p]),
     "smat_pack_b": (_i32, [_p, _i64, _i64, _i64, _i32, _p, _p, _p]),
     "smat_error_sums": (_i32, [_p, _i64, _p, _i64, _i64, _i64, _p, _p, _p]),
@@ -283,15 +289,20 @@ class SpmmPlan:
                                       ldc, stream_ptr(stream)))
         return C
 
-    def execute(self, B, C, *, a_values=None, a_scale=None, b_scale=None, ldb: int | None = None,
-                ldc: int | None = None, values24=None, index24=None, stream=None):
-        """smat_spmm_execute_ex: every operand explicit (FP8 / split plans)."""
+    def operands(self, B, C, *, a_values=None, a_scale=None, b_scale=None, ldb: int | None = None,
+                 ldc: int | None = None, values24=None, index24=None, c_rowmap=None) -> SpmmOperands:
         if ldb is None:
             ldb = self.N
         if ldc is None:
             ldc = C.shape[-1]
-        ops = SpmmOperands(ptr(a_values), ptr(a_scale), ptr(B), ptr(b_scale), ldb, ptr(C), ldc, ptr(values24),
-                           ptr(index24))
+        return SpmmOperands(ptr(a_values), ptr(a_scale), ptr(B), ptr(b_scale), ldb, ptr(C), ldc, ptr(values24),
+                            ptr(index24), ptr(c_rowmap))
+
+    def execute(self, B, C, *, a_values=None, a_scale=None, b_scale=None, ldb: int | None = None,
+                ldc: int | None = None, values24=None, index24=None, c_rowmap=None, stream=None):
+        """smat_spmm_execute_ex: every operand explicit (FP8 / split plans, reordered rows)."""
+        ops = self.operands(B, C, a_values=a_values, a_scale=a_scale, b_scale=b_scale, ldb=ldb, ldc=ldc,
+                            values24=values24, index24=index24, c_rowmap=c_rowmap)
         check(lib().smat_spmm_execute_ex(self.handle, ctypes.byref(ops), stream_ptr(stream)))
         return C
 
@@ -396,3 +407,41 @@ def rel_fro_error(C, R, stream=None) -> float:
                                 stream_ptr(stream)))
     e2, r2 = out.cpu().tolist()
     return (e2 / r2) ** 0.5 if r2 > 0 else (0.0 if e2 == 0 else float("inf"))
+
+
+class Batch:
+    """Many planned SpMMs in one persistent launch (smat_batch_*).  `entries`
+    is a list of (SpmmPlan, SpmmOperands); plans and buffers must outlive the
+    batch."""
+
+    def __init__(self, entries, stream=None):
+        self._keep = entries
+        n = len(entries)
+        plans = (_p * n)(*[e[0].handle for e in entries])
+        ops = (SpmmOperands * n)(*[e[1] for e in entries])
+        h = _p()
+        check(lib().smat_batch_create(n, ctypes.cast(plans, _p), ctypes.cast(ops, _p), stream_ptr(stream),
+                                      ctypes.byref(h)))
+        self.handle = h
+
+    def capture(self, stream):
+        check(lib().smat_batch_capture(self.handle, stream_ptr(stream)))
+
+    def __call__(self, stream=None):
+        check(lib().smat_batch_execute(self.handle, stream_ptr(stream)))
+
+    def info(self):
+        t, r, g, c = _i64(), _i64(), _i32(), _i32()
+        check(lib().smat_batch_info(self.handle, ctypes.byref(t), ctypes.byref(r), ctypes.byref(g), ctypes.byref(c)))
+        return {"tasks": t.value, "split_rows": r.value, "grid": g.value, "captured": bool(c.value)}
+
+    def close(self):
+        if self.handle:
+            lib().smat_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

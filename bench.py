@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="rmat", choices=["rmat", "band", "clustered", "powerlaw"])
+    ap.add_argument("--workload", default="rmat", choices=["rmat", "band", "clustered", "powerlaw", "batch64"])
     ap.add_argument("--scale", type=int, default=18, help="R-MAT scale")
     ap.add_argument("--n", type=int, default=256, help="columns of B")
     ap.add_argument("--bk", type=int, default=16, help="block width of a single (non-sweep) run")
@@ -238,6 +238,154 @@ class Ctx:
     pass
 
 
+def learned_pick(ctx, log):
+    """The learned cost model's configuration for the workload matrix
+    (features from GPU statistics, scores from the inference kernel).  None
+    when no trained weights are present."""
+    import os
+    from smat import tuner
+    if not os.path.exists(tuner.WEIGHTS_PATH):
+        log("no trained tuner weights; the sweep decides")
+        return None
+    t0 = time.perf_counter()
+    st = tuner.stats(ctx.M, ctx.K, ctx.d_rp, ctx.d_ci, ctx.d_v, ctx.nnz)
+    feats = tuner.features(st, ctx.N)
+    # the bench times configurations of the matrix as given (no reordering)
+    avail = tuner.available(ctx.M) * (tuner.ENC[:, -1] == 0)
+    _, b = tuner.Model().scores(feats, avail.astype("uint8"))
+    cfg = tuner.CONFIGS[b]
+    ms = (time.perf_counter() - t0) * 1e3
+    log(f"learned pick {cfg} ({ms:.1f} ms for statistics, features and inference)")
+    return {"config": cfg, "decide_ms": round(ms, 2)}
+
+
+def _gen_batch_matrix(seed):
+    from smat import tuner
+    return tuner.make_matrix(seed)
+
+
+def batch_plan(ctx, M, K, rp, ci, v, cfg, N):
+    """Device CSR (reordered when the configuration asks), BCSR, plan and
+    operands of one matrix of the batched workload."""
+    import numpy as np
+    import torch
+    import smat
+    from smat import tuner
+    tau, bk, nt, kernel = cfg
+    rowmap = None
+    if tau is not None:
+        rp, ci, v, perm = tuner.reorder_csr(rp, ci, v, tau)
+        rowmap = torch.from_numpy(perm.astype(np.int32)).cuda()
+    d_rp, d_ci = torch.from_numpy(rp).cuda(), torch.from_numpy(ci if ci.size else np.zeros(1, np.int32)).cuda()
+    d_v = torch.from_numpy((v if v.size else np.zeros(1, np.uint16)).view(np.int16)).cuda()
+    mixed = kernel == "mixed24"
+    A = smat.Bcsr(M, K, d_rp, d_ci, d_v, bk=bk, tag_rule=smat.SMAT_TAG_EXACT24 if mixed else smat.SMAT_TAG_REFERENCE)
+    v24 = i24 = None
+    if mixed:
+        v24, i24 = A.compress_24(smat.SMAT_24_SMFMAC)
+    # in a batch the matrices fill the GPU together: slices of 64 MFMA steps
+    plan = smat.SpmmPlan(A, N, n_tile=nt, max_blocks_per_item=64 * (32 // bk),
+                         kernel=smat.SMAT_KERNEL_MIXED24 if mixed else smat.SMAT_KERNEL_DENSE)
+    return {"A": A, "plan": plan, "v24": v24, "i24": i24, "rowmap": rowmap, "keep": (d_rp, d_ci, d_v)}
+
+
+def batched_workload(args, log):
+    """64 seeded matrices of uneven size (n log-uniform 2k-200k, nnz 1e4-5e6,
+    band / R-MAT / block-clustered / power-law), each times its own B (N
+    columns).  Plain: 16x16 dense blocks, no reordering, one launch per matrix
+    (the reference's way) and also as one batch.  Tuned: the learned model's
+    pick per matrix, all matrices in one persistent graph-captured launch.
+    The exhaustive sweep's pick is timed the same way as the judge."""
+    import multiprocessing as mp
+    import numpy as np
+    import scipy.sparse as sp
+    import torch
+    import smat
+    from smat import gen, tuner
+    N = args.n
+    t0 = time.time()
+    with mp.get_context("fork").Pool(12) as pool:
+        mats = pool.map(_gen_batch_matrix, range(64))
+    log(f"generated 64 matrices in {time.time() - t0:.1f}s, nnz total {sum(m[1][3].size for m in mats)}")
+    model = tuner.Model()
+    useful = sum(2.0 * m[1][3].size * N for m in mats)
+    Bs, Cs, R = [], [], []
+    picks, sweep_best, within = [], [], []
+    t_sweep = time.time()
+    for i, (kind, (M, K, rp, ci, v)) in enumerate(mats):
+        Bbits = gen.dense_b(K, N, seed=100 + i)
+        Bs.append(torch.from_numpy(Bbits.view(np.int16)).cuda())
+        Cs.append(torch.empty((M, N), dtype=torch.float32, device="cuda"))
+        A64 = sp.csr_matrix((v.view(np.float16).astype(np.float64), ci, rp), shape=(M, K))
+        R.append(torch.from_numpy(np.ascontiguousarray(A64 @ Bbits.view(np.float16).astype(np.float64))).cuda())
+        d_rp, d_ci = torch.from_numpy(rp).cuda(), torch.from_numpy(ci).cuda()
+        st = tuner.stats(M, K, d_rp, d_ci, torch.from_numpy(v.view(np.int16)).cuda(), int(ci.size))
+        feats = tuner.features(st, N)
+        _, b = model.scores(feats, tuner.available(M))
+        picks.append(tuner.CONFIGS[b])
+        times, _ = tuner.sweep(M, K, rp, ci, v, N, B=Bs[-1])
+        sb = int(np.argmin(times))
+        sweep_best.append(tuner.CONFIGS[sb])
+        within.append(float(times[b] / times[sb]))
+    log(f"features, model picks and exhaustive sweeps in {time.time() - t_sweep:.1f}s")
+
+    def build(cfgs):
+        return [batch_plan(None, M, K, rp, ci, v, cfg, N) for (kind, (M, K, rp, ci, v)), cfg in zip(mats, cfgs)]
+
+    def ops(built):
+        return [(b["plan"], b["plan"].operands(Bs[i], Cs[i], values24=b["v24"], index24=b["i24"],
+                                                c_rowmap=b["rowmap"])) for i, b in enumerate(built)]
+
+    def errors():
+        return max(smat.rel_fro_error(Cs[i], R[i]) for i in range(len(mats)))
+
+    def time_fn(fn, stream):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        w0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(args.steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps, (time.perf_counter() - w0) * 1e3 / args.steps
+
+    res = {}
+    s = torch.cuda.Stream()
+    for name, cfgs in (("plain", [tuner.CONFIGS[tuner.PLAIN]] * 64), ("tuned", picks), ("sweep_best", sweep_best)):
+        built = build(cfgs)
+        if name == "plain":  # one launch per matrix, plans with the library's default slices
+            seq_plans = [smat.SpmmPlan(b["A"], N) for b in built]
+            with torch.cuda.stream(s):
+                ms, wall = time_fn(lambda: [p(Bs[i], Cs[i]) for i, p in enumerate(seq_plans)], s)
+            res["plain_sequential"] = {"event_ms": round(ms, 4), "wall_ms": round(wall, 4),
+                                       "useful_gflops": round(useful / (ms * 1e-3) / 1e9, 1),
+                                       "max_rel_fro_err": errors()}
+            for p in seq_plans:
+                p.close()
+        batch = smat.Batch(ops(built))
+        with torch.cuda.stream(s):
+            batch.capture(s)
+            ms, wall = time_fn(lambda: batch(s), s)
+        res[f"{name}_batched"] = {"event_ms": round(ms, 4), "wall_ms": round(wall, 4),
+                                  "useful_gflops": round(useful / (ms * 1e-3) / 1e9, 1),
+                                  "max_rel_fro_err": errors(), "tasks": batch.info()["tasks"]}
+        log(f"{name}: {res}")
+        batch.close()
+        for b in built:
+            b["plan"].close()
+            b["A"].close()
+    within = np.asarray(within)
+    return {"useful_flop": useful, "results": res,
+            "model": {"within_10pct_of_sweep_best": float(np.mean(within <= 1.10)),
+                      "median_ratio_to_sweep_best": float(np.median(within)),
+                      "picks": [str(c) for c in picks]},
+            "kinds": [m[0] for m in mats], "rows": [int(m[1][0]) for m in mats],
+            "nnz": [int(m[1][3].size) for m in mats]}
+
+
 def precision_ladder(ctx, v32, Bf, bk, steps, warmup, log):
     """The tuned block width in every precision mode, from the FP32 inputs:
     pack A (preprocessing) and B (reported separately), time the SpMM, and the
@@ -302,13 +450,29 @@ def main():
     from smat import gen
 
     import smat
+    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)  # noqa: E731
+    if args.workload == "batch64":
+        r = batched_workload(args, log)
+        res = r["results"]
+        tuned, plain = res["tuned_batched"], res["plain_sequential"]
+        print(json.dumps({
+            "metric": "SpMM useful GFLOP/s (2*nnz*N / t), learned-tuned batch vs plain FP16 BCSR",
+            "value": tuned["useful_gflops"], "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": tuned["wall_ms"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f16 in, f32 accumulate/out", "data": "synthetic (seeded generators)",
+            "config": {"workload": f"64 seeded matrices (n 2k-200k log-uniform, nnz 1e4-5e6), N={args.n}, "
+                                   "one persistent hipGraph-captured launch", "N": args.n,
+                       "nnz_total": sum(r["nnz"])},
+            "speedup_vs_plain": round(tuned["useful_gflops"] / plain["useful_gflops"], 3),
+            "batched": res, "model": r["model"], "matrices": {"kinds": r["kinds"], "rows": r["rows"],
+                                                              "nnz": r["nnz"]}}))
+        return
     t0 = time.time()
     (M, K, rp, ci, v, v32), desc = make_workload(args)
     if args.dtype == "bf16":
         v = gen.f32_to_bf16_bits(v.view(np.float16).astype(np.float32))
     ctx = Ctx()
     ctx.args, ctx.M, ctx.K, ctx.N, ctx.nnz = args, M, K, args.n, int(ci.size)
-    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)  # noqa: E731
     log(f"{desc}: {M} rows, nnz {ctx.nnz}, generated in {time.time() - t0:.1f}s")
     dev = torch.device("cuda:0")
     ctx.d_rp = torch.from_numpy(rp).to(dev)
@@ -348,6 +512,7 @@ def main():
             sys.exit(3)
         return st
 
+    learned = sweep_best = None
     if args.single:
         tuned = measure(args.bk, args.n_tile, args.kernel, args.steps, args.warmup)
         plain = None
@@ -362,9 +527,16 @@ def main():
         if args.workload in ("clustered", "band"):
             sweep.append(measure(16, 0, "mixed24", max(3, args.steps // 2), 1))
         best = min(sweep, key=lambda s: s["event_ms"])
-        # re-time the winner with the full step count
-        tuned = measure(best["bk"], 0 if best["n_tile"] == "auto" else best["n_tile"], best["kernel"],
-                        args.steps, args.warmup)
+        # re-time the sweep's winner with the full step count: the judge
+        sweep_best = measure(best["bk"], 0 if best["n_tile"] == "auto" else best["n_tile"], best["kernel"],
+                             args.steps, args.warmup)
+        learned = learned_pick(ctx, log)
+        if learned is not None:
+            tau, bk, nt, kernel = learned["config"]
+            tuned = measure(bk, nt, kernel, args.steps, args.warmup)
+            learned["ratio_to_sweep_best"] = round(tuned["event_ms"] / sweep_best["event_ms"], 3)
+        else:
+            tuned = sweep_best
 
     ladder = None
     if not args.no_ladder:
@@ -384,7 +556,7 @@ def main():
         except Exception as ex:  # the baseline is a report, never the measurement
             cpu = {"error": str(ex)}
     out = {
-        "metric": "SpMM useful GFLOP/s (2*nnz*N / t), tuned vs plain FP16 BCSR",
+        "metric": "SpMM useful GFLOP/s (2*nnz*N / t), learned-tuned vs plain FP16 BCSR",
         "value": tuned["useful_gflops"],
         "unit": "GFLOP/s",
         "n_gpus": 1,
@@ -398,7 +570,11 @@ def main():
         "data": "synthetic (seeded generator; B U[-1,1] seed 7)",
         "config": {"workload": f"{desc}, B {K}x{ctx.N} row-major, FP32 C", "rows": M, "nnz": ctx.nnz,
                    "N": ctx.N, "tuned": {k: tuned[k] for k in ("bk", "n_tile", "kernel", "nnzb")},
-                   "tuning": "exhaustive sweep over block width x N-tile" if not args.single else "fixed"},
+                   "tuning": ("learned MLP pick (exhaustive sweep reported as judge)" if learned else
+                              "exhaustive sweep over block width x N-tile") if not args.single else "fixed"},
+        "learned": learned,
+        "sweep_best": {k: sweep_best[k] for k in ("bk", "n_tile", "kernel", "event_ms", "useful_gflops")}
+        if sweep_best else None,
         "plain": plain,
         "speedup_vs_plain": round(tuned["useful_gflops"] / plain["useful_gflops"], 3) if plain else None,
         "rel_fro_err": tuned["rel_fro_err"],

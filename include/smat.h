@@ -309,8 +309,8 @@ smat_status smat_error_sums(const float* C, int64_t ldc, const double* R, int64_
  * A batch runs `count` planned SpMMs (each with its own block width, N-tile
  * and operands) as one persistent kernel: a grid sized to fill the GPU
  * draws (matrix, work item, N-tile) tasks from one device counter, then one
- * kernel reduces the split rows of every matrix.  Plans must be dense-kernel
- * plans with the same 16-bit input type (F16 or BF16), row-major B staged
+ * kernel reduces the split rows of every matrix.  Plans (dense or mixed 2:4)
+ * must share one 16-bit input type (F16 or BF16) and use row-major B staged
  * through LDS and FP32 C; the plans and every operand buffer must outlive the
  * batch (their addresses are baked in).  No reference counterpart: the
  * reference launches one kernel per matrix (tester.h:944-1011).
@@ -330,6 +330,68 @@ smat_status smat_batch_destroy(smat_batch batch);
 /* Number of work items and split rows of a plan (for reports and tests). */
 smat_status smat_spmm_plan_info(smat_plan plan, int64_t* n_items, int64_t* n_split_rows,
                                 int64_t* scratch_bytes);
+
+/* ------------------------------------------------------------------------ */
+/* Learned tuner: matrix statistics, features and the MLP cost model         */
+/* ------------------------------------------------------------------------ */
+/*
+ * No reference counterpart: the reference tunes by hand or by an exhaustive
+ * threshold search (reorder_matrix.cpp:17-40).  The model scores candidate
+ * configurations (reorder threshold, block width, dense / 2:4 kernel, N-tile)
+ * of a matrix from its features; the exhaustive sweep stays the fallback.
+ */
+typedef struct {
+  int64_t rows, cols, nnz;
+  int64_t max_row;          /* longest row */
+  int64_t row_hist[12];     /* rows of length 0, 1, 2-3, 4-7, ..., 512-1023, >= 1024 */
+  int64_t sum_abs_offset;   /* sum over entries of |row - col| */
+  int64_t near_diag;        /* entries with |row - col| <= 64 */
+  int64_t nnzb[6];          /* stored blocks of the 16 x bk BCSR, bk = 1, 2, 4, 8, 16, 32 */
+  int64_t nnzb16;           /* stored 16 x 16 blocks */
+  int64_t blocks24;         /* of those, blocks that are 2:4 in every aligned 1x4 group */
+} smat_matrix_stats_t;
+
+/* Exact integer statistics of a device CSR (GPU; one host sync). */
+smat_status smat_matrix_stats(int64_t M, int64_t K, int64_t nnz, const int32_t* rowptr,
+                              const int32_t* colidx, const uint16_t* values,
+                              smat_matrix_stats_t* out_host, smat_stream stream);
+
+#define SMAT_TUNER_NF 26
+/*
+ * Feature vector (host): log2 rows, log2 nnz, log2 mean row length,
+ * log2 longest / mean row, the 12 row-length bucket fractions, mean |row - col|
+ * / rows, fraction of entries within 64 of the diagonal, log2 fill of the
+ * 16 x bk blocks for bk = 1..32, fraction of 2:4 blocks, log2 N.
+ */
+smat_status smat_tuner_features(const smat_matrix_stats_t* stats, int64_t N, float* out_host);
+
+/* Dense layer: Y[b][o] = act(sum_i X[b][i] W[o][i] + bias[o]), act = ReLU if relu. */
+smat_status smat_linear_forward(const float* X, int64_t B, int32_t n_in, const float* W,
+                                const float* bias, int32_t n_out, int32_t relu, float* Y,
+                                smat_stream stream);
+/* Its gradients from dY (Y: the layer's output, for the ReLU mask): dW, db and,
+ * when dX is not NULL, dX.  Deterministic (fixed summation order). */
+smat_status smat_linear_backward(const float* X, const float* Y, const float* dY, int64_t B,
+                                 int32_t n_in, const float* W, int32_t n_out, int32_t relu,
+                                 float* dX, float* dW, float* db, smat_stream stream);
+/* dpred = 2 (pred - y) / B and loss[0] = mean (pred - y)^2 (device scalars). */
+smat_status smat_mse_grad(const float* pred, const float* y, int64_t B, float* dpred,
+                          float* loss, smat_stream stream);
+/* One bias-corrected Adam step t >= 1 on n parameters. */
+smat_status smat_adam_step(float* p, const float* g, float* m, float* v, int64_t n, float lr,
+                           float beta1, float beta2, float eps, int32_t t, smat_stream stream);
+/*
+ * Inference: scores the n_cfg candidates (cfg_enc: [n_cfg][n_enc]) of one
+ * matrix (feats: [n_feat]) with the network in -> hidden -> hidden -> 1
+ * (ReLU; inputs standardised with mu / sigma) and writes scores[n_cfg] and
+ * best[0], the lowest-scoring candidate with avail != 0 (-1 if none).
+ * Device arrays; n_feat + n_enc <= 64, hidden <= 64, n_cfg <= 256.
+ */
+smat_status smat_tuner_predict(const float* feats, int32_t n_feat, const float* cfg_enc, int32_t n_enc,
+                               const uint8_t* avail, int32_t n_cfg, const float* mu, const float* sigma,
+                               const float* W1, const float* b1, const float* W2, const float* b2,
+                               const float* W3, const float* b3, int32_t hidden, float* scores,
+                               int32_t* best, smat_stream stream);
 
 /* ------------------------------------------------------------------------ */
 /* Reference drop-in launchers (src/cuda_hgemm/src/main.cu:10-77)           */

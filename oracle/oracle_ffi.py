@@ -48,6 +48,12 @@ _SIG = {
     "or_pack_b": (None, [_i64, _i64, _i64, _p, ctypes.c_int, _p, _p]),
     "or_spmm_bcsr_scaled_f64": (None, [_i64, ctypes.c_int, _p, _p, ctypes.c_int, _p, _p, _i64, _i64, _p, _p,
                                        _p, _i64]),
+    "or_matrix_stats": (None, [_i64, _i64, _p, _p, _p, _p]),
+    "or_mlp_forward": (None, [_i64, ctypes.c_int, ctypes.c_int, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "or_mlp_backward": (ctypes.c_double, [_i64, ctypes.c_int, ctypes.c_int, _p, _p, _p, _p, _p, _p, _p, _p, _p,
+                                          _p, _p, _p, _p, _p]),
+    "or_adam_step": (None, [_p, _p, _p, _p, _i64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                            ctypes.c_double, ctypes.c_int]),
 }
 
 _REF_SIG = {
@@ -311,3 +317,49 @@ def spmm_scaled_f64(bk, rowptr, colidx, dt, a, a_scale, K, N, b, b_scale):
                                   P(np.ascontiguousarray(a)), None if a_scale is None else P(a_scale), K, N,
                                   P(np.ascontiguousarray(b)), None if b_scale is None else P(b_scale), P(C), N)
     return C
+
+
+# ---- learned tuner ------------------------------------------------------------
+STAT_FIELDS = ["rows", "cols", "nnz", "max_row"] + [f"row_hist{b}" for b in range(12)] + [
+    "sum_abs_offset", "near_diag"] + [f"nnzb{w}" for w in range(6)] + ["nnzb16", "blocks24"]
+
+
+def matrix_stats(M, K, rowptr, colidx, values_u16):
+    out = np.zeros(26, np.int64)
+    rp = np.ascontiguousarray(rowptr, np.int32)
+    ci = np.ascontiguousarray(colidx, np.int32) if colidx.size else np.zeros(1, np.int32)
+    v = np.ascontiguousarray(values_u16, np.uint16) if values_u16.size else np.zeros(1, np.uint16)
+    lib().or_matrix_stats(M, K, P(rp), P(ci), P(v), P(out))
+    return dict(zip(STAT_FIELDS, out.tolist()))
+
+
+def _as_f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def mlp_forward(X, params):
+    W1, b1, W2, b2, W3, b3 = [_as_f32(p) for p in params]
+    X = _as_f32(X)
+    B, n_in = X.shape
+    h = W1.shape[0]
+    pred = np.zeros(B)
+    H1, H2 = np.zeros((B, h)), np.zeros((B, h))
+    lib().or_mlp_forward(B, n_in, h, P(X), P(W1), P(b1), P(W2), P(b2), P(W3), P(b3), P(pred), P(H1), P(H2))
+    return pred, H1, H2
+
+
+def mlp_backward(X, y, params):
+    W1, b1, W2, b2, W3, b3 = [_as_f32(p) for p in params]
+    X, y = _as_f32(X), _as_f32(y)
+    B, n_in = X.shape
+    h = W1.shape[0]
+    g = [np.zeros(p.shape) for p in (W1, b1, W2, b2, W3, b3)]
+    loss = lib().or_mlp_backward(B, n_in, h, P(X), P(y), P(W1), P(b1), P(W2), P(b2), P(W3), P(b3),
+                                 *[P(x) for x in g])
+    return loss, g
+
+
+def adam_step(p, g, m, v, lr, b1, b2, eps, t):
+    for a in (p, g, m, v):
+        assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    lib().or_adam_step(P(p), P(g), P(m), P(v), p.size, lr, b1, b2, eps, t)

@@ -40,10 +40,16 @@
 //                              FP16 operands; dense brute force
 // (The precision modes have no reference counterpart: the reference computes
 // in FP16 only, mmio_highlevel.h:608 and ptx.h:20-41.)
+//   or_matrix_stats            brute force over a dense copy (tests), hand cases
+//   or_mlp_forward/backward    central finite differences of the FP64 loss
+//   or_adam_step               closed form of the first two steps
+// (The learned tuner has no reference counterpart; the reference's tuning is
+// an exhaustive threshold search, reorder_matrix.cpp:17-40.)
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <numeric>
 #include <set>
 #include <utility>
@@ -769,6 +775,123 @@ void or_spmm_bcsr_scaled_f64(int64_t nbrows, int bk, const int32_t* rowptr, cons
         C[(br * 16 + r) * ldc + n] = acc[n] * sa * sb;
       }
     }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Learned tuner (no reference counterpart; restated from the definitions in
+// include/smat.h).
+// ---------------------------------------------------------------------------
+extern "C" {
+
+// out: [0] rows [1] cols [2] nnz [3] longest row [4..15] row-length buckets
+// [16] sum |r - c| [17] entries with |r - c| <= 64 [18..23] 16 x bk blocks for
+// bk = 1..32 [24] 16 x 16 blocks [25] of those, 2:4 in every aligned 1x4 group.
+void or_matrix_stats(int64_t M, int64_t K, const int32_t* rp, const int32_t* ci, const uint16_t* val,
+                     int64_t* out) {
+  std::memset(out, 0, 26 * sizeof(int64_t));
+  out[0] = M;
+  out[1] = K;
+  out[2] = rp[M];
+  for (int64_t br = 0; br * 16 < M; br++) {
+    std::set<int64_t> blocks[6];
+    std::set<int64_t> violated;
+    for (int64_t r = br * 16; r < br * 16 + 16 && r < M; r++) {
+      const int64_t len = rp[r + 1] - rp[r];
+      int b = 0;
+      if (len > 0) {
+        b = 1;
+        while (b < 11 && (len >> b) != 0) b++;
+      }
+      out[4 + b]++;
+      out[3] = std::max<int64_t>(out[3], len);
+      std::map<int64_t, int> group_count;  // nonzero values per aligned 1x4 group of this row
+      for (int64_t q = rp[r]; q < rp[r + 1]; q++) {
+        const int64_t c = ci[q];
+        out[16] += r > c ? r - c : c - r;
+        out[17] += (r > c ? r - c : c - r) <= 64;
+        for (int w = 0; w < 6; w++) blocks[w].insert(c >> w);
+        if ((val[q] & 0x7fff) != 0 && ++group_count[c / 4] > 2) violated.insert(c / 16);
+      }
+    }
+    for (int w = 0; w < 6; w++) out[18 + w] += (int64_t)blocks[w].size();
+    out[24] += (int64_t)blocks[4].size();
+    out[25] += (int64_t)blocks[4].size() - (int64_t)violated.size();
+  }
+}
+
+// Network in -> h -> h -> 1, ReLU on the hidden layers, FP64.  Parameters in
+// the device layout (W[o][i] row-major).  H1, H2: [B][h] activations.
+void or_mlp_forward(int64_t B, int n_in, int h, const float* X, const float* W1, const float* b1,
+                    const float* W2, const float* b2, const float* W3, const float* b3, double* pred,
+                    double* H1, double* H2) {
+  for (int64_t b = 0; b < B; b++) {
+    for (int o = 0; o < h; o++) {
+      double s = b1[o];
+      for (int i = 0; i < n_in; i++) s += (double)X[b * n_in + i] * (double)W1[o * n_in + i];
+      H1[b * h + o] = s > 0 ? s : 0;
+    }
+    for (int o = 0; o < h; o++) {
+      double s = b2[o];
+      for (int i = 0; i < h; i++) s += H1[b * h + i] * (double)W2[o * h + i];
+      H2[b * h + o] = s > 0 ? s : 0;
+    }
+    double s = b3[0];
+    for (int i = 0; i < h; i++) s += H2[b * h + i] * (double)W3[i];
+    pred[b] = s;
+  }
+}
+
+// Loss = mean (pred - y)^2 and its gradients with respect to every parameter.
+double or_mlp_backward(int64_t B, int n_in, int h, const float* X, const float* y, const float* W1,
+                       const float* b1, const float* W2, const float* b2, const float* W3, const float* b3,
+                       double* gW1, double* gb1, double* gW2, double* gb2, double* gW3, double* gb3) {
+  std::vector<double> pred((size_t)B), H1((size_t)(B * h)), H2((size_t)(B * h));
+  or_mlp_forward(B, n_in, h, X, W1, b1, W2, b2, W3, b3, pred.data(), H1.data(), H2.data());
+  std::fill(gW1, gW1 + h * n_in, 0.0);
+  std::fill(gb1, gb1 + h, 0.0);
+  std::fill(gW2, gW2 + h * h, 0.0);
+  std::fill(gb2, gb2 + h, 0.0);
+  std::fill(gW3, gW3 + h, 0.0);
+  gb3[0] = 0.0;
+  double loss = 0;
+  std::vector<double> d2((size_t)h), d1((size_t)h);
+  for (int64_t b = 0; b < B; b++) {
+    const double e = pred[(size_t)b] - (double)y[b];
+    loss += e * e;
+    const double dp = 2.0 * e / (double)B;
+    gb3[0] += dp;
+    for (int i = 0; i < h; i++) {
+      gW3[i] += dp * H2[(size_t)(b * h + i)];
+      d2[(size_t)i] = H2[(size_t)(b * h + i)] > 0 ? dp * (double)W3[i] : 0.0;
+    }
+    for (int o = 0; o < h; o++) {
+      gb2[o] += d2[(size_t)o];
+      for (int i = 0; i < h; i++) gW2[o * h + i] += d2[(size_t)o] * H1[(size_t)(b * h + i)];
+    }
+    for (int i = 0; i < h; i++) {
+      double s = 0;
+      for (int o = 0; o < h; o++) s += d2[(size_t)o] * (double)W2[o * h + i];
+      d1[(size_t)i] = H1[(size_t)(b * h + i)] > 0 ? s : 0.0;
+    }
+    for (int o = 0; o < h; o++) {
+      gb1[o] += d1[(size_t)o];
+      for (int i = 0; i < n_in; i++) gW1[o * n_in + i] += d1[(size_t)o] * (double)X[b * n_in + i];
+    }
+  }
+  return loss / (double)B;
+}
+
+// Bias-corrected Adam step t >= 1 (Kingma & Ba 2015, algorithm 1), FP64.
+void or_adam_step(double* p, const double* g, double* m, double* v, int64_t n, double lr, double b1, double b2,
+                  double eps, int t) {
+  const double c1 = 1 - std::pow(b1, t), c2 = 1 - std::pow(b2, t);
+  for (int64_t i = 0; i < n; i++) {
+    m[i] = b1 * m[i] + (1 - b1) * g[i];
+    v[i] = b2 * v[i] + (1 - b2) * g[i] * g[i];
+    p[i] -= lr * (m[i] / c1) / (std::sqrt(v[i] / c2) + eps);
+  }
 }
 
 }  // extern "C"

@@ -501,31 +501,37 @@ __global__ __launch_bounds__(256) void k_spmm_lds(const Item* __restrict__ items
 // sparse_ids[sbeg..send)) through SMFMAC in quads, from the compressed arrays
 // of smat_compress_24 (SMAT_24_SMFMAC layout, indexed by BCSR block).  Both
 // accumulate into the same FP32 tile.  One register stage of prefetch.
+// 2:4 operands of a mixed plan.
+struct Mixed24 {
+  const int32_t* dense_ids;
+  const int32_t* sparse_ids;
+  const uint16_t* values24;
+  const uint16_t* index24;
+};
+
+template <int NSUB>
+constexpr int mixed_wave_bytes() {
+  return 64 * (16 * NSUB + 8) * 2;
+}
+
 template <int NSUB, int DT, bool OUT_F16>
-__global__ __launch_bounds__(256) void k_spmm_mixed(
-    const Item* __restrict__ items, int64_t n_items, int32_t ntiles, const int32_t* __restrict__ colidx,
-    const uint16_t* __restrict__ values, const int32_t* __restrict__ dense_ids,
-    const int32_t* __restrict__ sparse_ids, const uint16_t* __restrict__ values24,
-    const uint16_t* __restrict__ index24, const uint16_t* __restrict__ B, int64_t ldb, int64_t N,
-    void* __restrict__ C, int64_t ldc, float* __restrict__ scratch, const int32_t* __restrict__ rowmap) {
+__device__ __forceinline__ void mixed_tile(const Item& it, int64_t n0, const Operands& op, const Mixed24& mx,
+                                           uint8_t* __restrict__ lds_bytes, int lane) {
   constexpr int NT = 16 * NSUB;
   constexpr int CPR = NT / 8;
   constexpr int RPL = 64 / CPR;
   constexpr int NLD_D = 32 / RPL;  // loads per dense pair (32 rows)
   constexpr int NLD_S = 64 / RPL;  // loads per 2:4 quad (64 rows)
   constexpr int PITCH = NT + 8;
-  __shared__ __attribute__((aligned(16))) uint16_t lds_all[kWaves * 64 * PITCH];
-
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t task = (int64_t)wg * kWaves + wave;
-  const int64_t item_id = task / ntiles;
-  if (item_id >= n_items) return;
-  const int nt = (int)(task % ntiles);
-  const Item it = items[item_id];
-  const int64_t n0 = (int64_t)nt * NT;
-  uint16_t* lds = lds_all + wave * 64 * PITCH;
+  uint16_t* lds = reinterpret_cast<uint16_t*>(lds_bytes);
+  const int32_t* __restrict__ colidx = op.colidx;
+  const uint16_t* __restrict__ values = static_cast<const uint16_t*>(op.a);
+  const uint16_t* __restrict__ B = static_cast<const uint16_t*>(op.b);
+  const int64_t ldb = op.ldb, N = op.N;
+  const int32_t* __restrict__ dense_ids = mx.dense_ids;
+  const int32_t* __restrict__ sparse_ids = mx.sparse_ids;
+  const uint16_t* __restrict__ values24 = mx.values24;
+  const uint16_t* __restrict__ index24 = mx.index24;
 
   const int r16 = lane & 15, grp = lane >> 4;
   const int lrow = lane / CPR, lch = lane % CPR;
@@ -614,7 +620,23 @@ __global__ __launch_bounds__(256) void k_spmm_mixed(
       }
     }
   }
-  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, C, ldc, scratch, rowmap);
+  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, op.c, op.ldc, op.scratch, op.rowmap);
+}
+
+template <int NSUB, int DT, bool OUT_F16>
+__global__ __launch_bounds__(256) void k_spmm_mixed(const Item* __restrict__ items, int64_t n_items, int32_t ntiles,
+                                                    Operands op, Mixed24 mx) {
+  constexpr int WB = mixed_wave_bytes<NSUB>();
+  __shared__ __attribute__((aligned(16))) uint8_t lds_all[kWaves * WB];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t task = (int64_t)wg * kWaves + wave;
+  const int64_t item_id = task / ntiles;
+  if (item_id >= n_items) return;
+  const int nt = (int)(task % ntiles);
+  const Item it = items[item_id];
+  mixed_tile<NSUB, DT, OUT_F16>(it, (int64_t)nt * 16 * NSUB, op, mx, lds_all + wave * WB, lane);
 }
 
 struct Reduce {
@@ -659,10 +681,11 @@ __global__ void k_reduce(const Reduce* __restrict__ red, int64_t n_red, int32_t 
 // register budget of the largest and halve the occupancy of the rest.
 struct BatchMat {
   Operands op;
+  Mixed24 mx;  // mixed (2:4) plans only
   const Item* items;
   int64_t task_beg;  // global index of this matrix's first task
   int32_t ntiles;
-  int32_t kind;      // block-width code 0..5 for 1, 2, 4, 8, 16, 32
+  int32_t kind;      // block-width code 0..5 for 1, 2, 4, 8, 16, 32; 6: mixed dense / 2:4 (16 x 16)
 };
 
 struct BatchRed {
@@ -707,10 +730,11 @@ __device__ inline Operands uni(const Operands& o) {
 }
 
 template <int DT, int NS>
-__device__ __forceinline__ void batch_tile(int bkc, const Item& it, int nt, const Operands& op, uint8_t* lds,
-                                           int lane) {
+__device__ __forceinline__ void batch_tile(int bkc, const Item& it, int nt, const Operands& op, const Mixed24& mx,
+                                           uint8_t* lds, int lane) {
   const int64_t n0 = (int64_t)nt * 16 * NS;
   switch (bkc) {
+    case 6: mixed_tile<NS, DT, false>(it, n0, op, mx, lds, lane); break;
     case 0: lds_tile<1, NS, DT, false>(it, n0, op, lds, lane); break;
     case 1: lds_tile<2, NS, DT, false>(it, n0, op, lds, lane); break;
     case 2: lds_tile<4, NS, DT, false>(it, n0, op, lds, lane); break;
@@ -732,7 +756,8 @@ template <int DT, int NS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NS == 4 ? 3 : 4)))
 void k_batch(const BatchMat* __restrict__ mats, int32_t nmats, int64_t total, int64_t chunk,
              unsigned long long* __restrict__ counters) {
-  constexpr int WB = lds_wave_bytes<NS, DT>();
+  constexpr int WB = lds_wave_bytes<NS, DT>() > mixed_wave_bytes<NS>() ? lds_wave_bytes<NS, DT>()
+                                                                          : mixed_wave_bytes<NS>();
   constexpr int NX = 8;
   __shared__ __attribute__((aligned(16))) uint8_t lds_all[kWaves * WB];
   const int lane = threadIdx.x & 63;
@@ -762,7 +787,11 @@ void k_batch(const BatchMat* __restrict__ mats, int32_t nmats, int64_t total, in
       const uint32_t local = (uint32_t)(task - uni(m.task_beg));  // < 2^31 per matrix
       const uint32_t ntl = (uint32_t)__builtin_amdgcn_readfirstlane(m.ntiles);
       const Item it = uni(m.items)[local / ntl];
-      batch_tile<DT, NS>(__builtin_amdgcn_readfirstlane(m.kind), it, (int)(local % ntl), uni(m.op), lds, lane);
+      const int kind = __builtin_amdgcn_readfirstlane(m.kind);
+      const Mixed24 mx = kind == 6 ? Mixed24{uni(m.mx.dense_ids), uni(m.mx.sparse_ids), uni(m.mx.values24),
+                                             uni(m.mx.index24)}
+                                   : Mixed24{nullptr, nullptr, nullptr, nullptr};
+      batch_tile<DT, NS>(kind, it, (int)(local % ntl), uni(m.op), mx, lds, lane);
     }
   }
 }
@@ -902,10 +931,9 @@ template <int NSUB, int DT, bool OUT_F16>
 void launch_mixed(const smat_plan_t* P, const uint16_t* v24, const uint16_t* i24, const Operands& op,
                   hipStream_t s) {
   const int32_t ntiles = ntiles_of(P, NSUB);
+  const Mixed24 mx{P->dense_ids.as<int32_t>(), P->sparse_ids.as<int32_t>(), v24, i24};
   hipLaunchKernelGGL((k_spmm_mixed<NSUB, DT, OUT_F16>), blocks_for(P->n_items * ntiles, kWaves), 64 * kWaves, 0,
-                     s, P->items.as<Item>(), P->n_items, ntiles, op.colidx, static_cast<const uint16_t*>(op.a),
-                     P->dense_ids.as<int32_t>(), P->sparse_ids.as<int32_t>(), v24, i24,
-                     static_cast<const uint16_t*>(op.b), op.ldb, op.N, op.c, op.ldc, op.scratch, op.rowmap);
+                     s, P->items.as<Item>(), P->n_items, ntiles, op, mx);
 }
 
 inline bool lds_aligned(const smat_plan_t* P, const Operands& op, int elems_per_chunk) {
@@ -1302,12 +1330,15 @@ smat_status smat_batch_create(int32_t count, const smat_plan* plans, const smat_
     const smat_spmm_operands& o = ops[i];
     const int g = P->nsub == 1 ? 0 : P->nsub == 2 ? 1 : 2;
     SMAT_REQUIRE(P && o.b && o.c, "smat_batch_create: entry %d: null plan or operand", i);
-    SMAT_REQUIRE(!P->mixed && P->opts.in_dtype == dt && (dt == SMAT_F16 || dt == SMAT_BF16) &&
+    SMAT_REQUIRE(P->opts.in_dtype == dt && (dt == SMAT_F16 || dt == SMAT_BF16) &&
                      P->opts.b_layout == SMAT_B_ROW_MAJOR && P->opts.out_dtype == SMAT_F32 &&
                      P->opts.b_staging != SMAT_BSTAGE_DIRECT,
-                 "smat_batch_create: entry %d: batches take dense-kernel plans with one 16-bit input type, "
-                 "row-major B staged through LDS and FP32 C",
+                 "smat_batch_create: entry %d: batches take plans with one 16-bit input type, row-major B "
+                 "staged through LDS and FP32 C",
                  i);
+    if (P->mixed)
+      SMAT_REQUIRE(o.values24 && o.index24 && (reinterpret_cast<uintptr_t>(o.values24) & 15) == 0,
+                   "smat_batch_create: entry %d: a 2:4 plan needs 16-byte aligned values24 and index24", i);
     SMAT_REQUIRE(o.ldb >= P->N && o.ldc >= P->N, "smat_batch_create: entry %d: ldb / ldc < N", i);
     const void* a = o.a_values ? o.a_values : P->A.values;
     BatchMat& m = mats[(size_t)pos];
@@ -1317,7 +1348,10 @@ smat_status smat_batch_create(int32_t count, const smat_plan* plans, const smat_
                                           "16-byte aligned", i);
     m.items = P->items.as<Item>();
     m.ntiles = ntiles_of(P, P->nsub);
-    m.kind = bk_code(P->A.bk);
+    m.kind = P->mixed ? 6 : bk_code(P->A.bk);
+    m.mx = P->mixed ? Mixed24{P->dense_ids.as<int32_t>(), P->sparse_ids.as<int32_t>(), o.values24,
+                              static_cast<const uint16_t*>(o.index24)}
+                    : Mixed24{nullptr, nullptr, nullptr, nullptr};
     m.task_beg = g_tasks[g];  // within its group
     g_tasks[g] += P->n_items * (int64_t)m.ntiles;
     if (P->n_red > 0) {

@@ -1,0 +1,368 @@
+// Learned tuner: matrix statistics and features (GPU), and the MLP cost model
+// (forward, backward, Adam and a fused inference kernel, FP32).
+//
+// The reference picks its configuration by hand or by an exhaustive search
+// over the Jaccard threshold (reorder_matrix.cpp:17-40, collect_taus.py:37-44);
+// the learned model has no reference counterpart.  The exhaustive sweep stays
+// the fallback and the judge of the model (host side, smat/tuner.py).
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "smat_internal.h"
+
+namespace {
+
+inline unsigned grid_of(int64_t n, int t) {
+  const int64_t g = (n + t - 1) / t;
+  return (unsigned)(g < 1 ? 1 : (g > (1 << 30) ? (1 << 30) : g));
+}
+
+// ---- statistics ---------------------------------------------------------------
+// Row-length histogram bucket: 0 -> 0, 1 -> 1, [2^(b-1), 2^b) -> b, capped at 11.
+__device__ inline int len_bucket(int64_t len) {
+  if (len <= 0) return 0;
+  int b = 64 - __clzll((unsigned long long)len);  // floor(log2 len) + 1
+  return b > 11 ? 11 : b;
+}
+
+// Per row: histogram, longest row; per entry: |row - col| sum, entries near the
+// diagonal, and the sort key (block row, column).  Block-local LDS bins, then
+// one atomic per bin and block.
+__global__ __launch_bounds__(256) void k_row_stats(int64_t M, const int32_t* __restrict__ rp,
+                                                   const int32_t* __restrict__ ci,
+                                                   unsigned long long* __restrict__ acc,
+                                                   uint64_t* __restrict__ keys) {
+  // acc: [0..11] histogram, [12] max row, [13] sum |r-c|, [14] near diagonal
+  __shared__ unsigned long long hist[12], sums[2];
+  __shared__ unsigned long long mx;
+  if (threadIdx.x < 12) hist[threadIdx.x] = 0;
+  if (threadIdx.x < 2) sums[threadIdx.x] = 0;
+  if (threadIdx.x == 0) mx = 0;
+  __syncthreads();
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < M) {
+    const int32_t b = rp[r], e = rp[r + 1];
+    const int64_t len = e - b;
+    atomicAdd(&hist[len_bucket(len)], 1ull);
+    atomicMax(&mx, (unsigned long long)len);
+    unsigned long long s = 0, nd = 0;
+    for (int32_t q = b; q < e; q++) {
+      const int64_t c = ci[q];
+      const int64_t d = r > c ? r - c : c - r;
+      s += (unsigned long long)d;
+      nd += d <= 64;
+      keys[q] = ((uint64_t)(r / 16) << 32) | (uint32_t)c;
+    }
+    atomicAdd(&sums[0], s);
+    atomicAdd(&sums[1], nd);
+  }
+  __syncthreads();
+  if (threadIdx.x < 12 && hist[threadIdx.x]) atomicAdd(&acc[threadIdx.x], hist[threadIdx.x]);
+  if (threadIdx.x == 0) {
+    atomicMax(&acc[12], mx);
+    atomicAdd(&acc[13], sums[0]);
+    atomicAdd(&acc[14], sums[1]);
+  }
+}
+
+// On keys sorted by (block row, column): a key starts a new 16 x bk block iff
+// its block row or its column / bk differs from the previous key's.
+__global__ __launch_bounds__(256) void k_block_heads(int64_t n, const uint64_t* __restrict__ keys,
+                                                     unsigned long long* __restrict__ nnzb) {
+  __shared__ unsigned long long cnt[6];
+  if (threadIdx.x < 6) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const uint64_t k = keys[i];
+    const uint64_t p = i > 0 ? keys[i - 1] : ~0ull;
+    const bool new_row = i == 0 || (k >> 32) != (p >> 32);
+    const uint32_t c = (uint32_t)k, pc = (uint32_t)p;
+#pragma unroll
+    for (int w = 0; w < 6; w++)
+      if (new_row || (c >> w) != (pc >> w)) atomicAdd(&cnt[w], 1ull);
+  }
+  __syncthreads();
+  if (threadIdx.x < 6 && cnt[threadIdx.x]) atomicAdd(&nnzb[threadIdx.x], cnt[threadIdx.x]);
+}
+
+__global__ void k_count_tag1(int64_t n, const int32_t* __restrict__ tags, unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long c;
+  if (threadIdx.x == 0) c = 0;
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && tags[i] == 1) atomicAdd(&c, 1ull);
+  __syncthreads();
+  if (threadIdx.x == 0 && c) atomicAdd(out, c);
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+// ---- MLP kernels --------------------------------------------------------------
+// Y[b][o] = act(sum_i X[b][i] W[o][i] + bias[o]); act = ReLU or identity.
+__global__ void k_linear_fwd(const float* __restrict__ X, int64_t B, int32_t n_in, const float* __restrict__ W,
+                             const float* __restrict__ bias, int32_t n_out, int32_t relu, float* __restrict__ Y) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= B * n_out) return;
+  const int64_t b = t / n_out;
+  const int32_t o = (int32_t)(t % n_out);
+  float s = bias[o];
+  for (int32_t i = 0; i < n_in; i++) s = fmaf(X[b * n_in + i], W[(int64_t)o * n_in + i], s);
+  Y[t] = relu && s < 0.f ? 0.f : s;
+}
+
+// dZ = dY, masked by the ReLU (Y > 0) when the layer has one.
+__device__ inline float dz(const float* __restrict__ Y, const float* __restrict__ dY, int64_t idx, int32_t relu) {
+  return relu && !(Y[idx] > 0.f) ? 0.f : dY[idx];
+}
+
+// dX[b][i] = sum_o dZ[b][o] W[o][i]
+__global__ void k_linear_bwd_x(const float* __restrict__ Y, const float* __restrict__ dY, int64_t B, int32_t n_in,
+                               const float* __restrict__ W, int32_t n_out, int32_t relu, float* __restrict__ dX) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= B * n_in) return;
+  const int64_t b = t / n_in;
+  const int32_t i = (int32_t)(t % n_in);
+  float s = 0.f;
+  for (int32_t o = 0; o < n_out; o++) s = fmaf(dz(Y, dY, b * n_out + o, relu), W[(int64_t)o * n_in + i], s);
+  dX[t] = s;
+}
+
+// dW[o][i] = sum_b dZ[b][o] X[b][i]; db[o] = sum_b dZ[b][o] (thread per weight,
+// rows of the batch in order: deterministic).
+__global__ void k_linear_bwd_w(const float* __restrict__ X, const float* __restrict__ Y, const float* __restrict__ dY,
+                               int64_t B, int32_t n_in, int32_t n_out, int32_t relu, float* __restrict__ dW,
+                               float* __restrict__ db) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nw = (int64_t)n_out * (n_in + 1);
+  if (t >= nw) return;
+  const int32_t o = (int32_t)(t / (n_in + 1)), i = (int32_t)(t % (n_in + 1));  // i == n_in: the bias
+  float s = 0.f;
+  for (int64_t b = 0; b < B; b++) {
+    const float g = dz(Y, dY, b * n_out + o, relu);
+    s = i < n_in ? fmaf(g, X[b * n_in + i], s) : s + g;
+  }
+  if (i < n_in) dW[(int64_t)o * n_in + i] = s;
+  else db[o] = s;
+}
+
+// Mean squared error of pred against y over B rows: dpred = 2 (pred - y) / B,
+// loss[0] = mean (pred - y)^2 (one workgroup, fixed summation order).
+__global__ __launch_bounds__(256) void k_mse(const float* __restrict__ pred, const float* __restrict__ y, int64_t B,
+                                             float* __restrict__ dpred, float* __restrict__ loss) {
+  __shared__ float part[256];
+  float s = 0.f;
+  for (int64_t b = threadIdx.x; b < B; b += 256) {
+    const float d = pred[b] - y[b];
+    dpred[b] = 2.f * d / (float)B;
+    s = fmaf(d, d, s);
+  }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss[0] = part[0] / (float)B;
+}
+
+// Adam (Kingma & Ba 2015, bias-corrected), step t >= 1.
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps, float c1,
+                       float c2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// Inference: one workgroup scores every candidate configuration of one matrix
+// with the 3-layer network (inputs standardised in the kernel) and writes the
+// scores and the index of the lowest available one.  Layer widths <= 64.
+constexpr int kMaxIn = 64, kMaxHidden = 64, kMaxCfg = 256;
+
+__global__ __launch_bounds__(256) void k_mlp_predict(
+    const float* __restrict__ feats, int32_t n_feat, const float* __restrict__ cfg_enc, int32_t n_enc,
+    const uint8_t* __restrict__ avail, int32_t n_cfg, const float* __restrict__ mu, const float* __restrict__ sigma,
+    const float* __restrict__ W1, const float* __restrict__ b1, const float* __restrict__ W2,
+    const float* __restrict__ b2, const float* __restrict__ W3, const float* __restrict__ b3, int32_t hidden,
+    float* __restrict__ scores, int32_t* __restrict__ best) {
+  __shared__ float x[kMaxIn], h1[kMaxHidden], h2[kMaxHidden], sc[kMaxCfg];
+  const int n_in = n_feat + n_enc;
+  for (int c = 0; c < n_cfg; c++) {
+    for (int i = threadIdx.x; i < n_in; i += blockDim.x) {
+      const float raw = i < n_feat ? feats[i] : cfg_enc[c * n_enc + (i - n_feat)];
+      x[i] = (raw - mu[i]) / sigma[i];
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < hidden; o += blockDim.x) {
+      float s = b1[o];
+      for (int i = 0; i < n_in; i++) s = fmaf(x[i], W1[o * n_in + i], s);
+      h1[o] = s > 0.f ? s : 0.f;
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < hidden; o += blockDim.x) {
+      float s = b2[o];
+      for (int i = 0; i < hidden; i++) s = fmaf(h1[i], W2[o * hidden + i], s);
+      h2[o] = s > 0.f ? s : 0.f;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float s = b3[0];
+      for (int i = 0; i < hidden; i++) s = fmaf(h2[i], W3[i], s);
+      sc[c] = s;
+      scores[c] = s;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int bi = -1;
+    for (int c = 0; c < n_cfg; c++)
+      if (avail[c] && (bi < 0 || sc[c] < sc[bi])) bi = c;
+    best[0] = bi;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+smat_status smat_matrix_stats(int64_t M, int64_t K, int64_t nnz, const int32_t* rowptr, const int32_t* colidx,
+                              const uint16_t* values, smat_matrix_stats_t* out, smat_stream stream) {
+  SMAT_REQUIRE(out && rowptr && M > 0 && K > 0 && nnz >= 0, "smat_matrix_stats: bad argument");
+  SMAT_REQUIRE(M % 16 == 0 && K % 32 == 0, "smat_matrix_stats: M must be a multiple of 16 and K of 32");
+  SMAT_REQUIRE(nnz == 0 || (colidx && values), "smat_matrix_stats: null CSR arrays");
+  hipStream_t s = smat::to_hip(stream);
+  smat::DevBuf acc, keys, keys2, tmp;
+  SMAT_TRY(acc.alloc(32 * sizeof(unsigned long long)));  // 15 row stats, 6 block counts, 1 2:4 count
+  SMAT_TRY(keys.alloc((nnz + 1) * 8));
+  SMAT_TRY(keys2.alloc((nnz + 1) * 8));
+  unsigned long long* a = acc.as<unsigned long long>();
+  hipLaunchKernelGGL(k_zero_u64, 1, 32, 0, s, a, 32);
+  hipLaunchKernelGGL(k_row_stats, grid_of(M, 256), 256, 0, s, M, rowptr, colidx, a, keys.as<uint64_t>());
+  SMAT_LAUNCH_CHECK();
+  if (nnz > 0) {
+    size_t tb = 0;
+    SMAT_HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, keys.as<uint64_t>(), keys2.as<uint64_t>(), (int)nnz, 0,
+                                                   64, s));
+    SMAT_TRY(tmp.alloc(tb));
+    SMAT_HIP_TRY(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, keys.as<uint64_t>(), keys2.as<uint64_t>(), (int)nnz, 0,
+                                                   64, s));
+    hipLaunchKernelGGL(k_block_heads, grid_of(nnz, 256), 256, 0, s, nnz, keys2.as<uint64_t>(), a + 15);
+    SMAT_LAUNCH_CHECK();
+  }
+  // 2:4 compliance of the 16 x 16 blocks under the exact rule
+  smat_bcsr A16 = nullptr;
+  SMAT_TRY(smat_bcsr_from_csr(M, K, nnz, rowptr, colidx, values, 16, SMAT_TAG_EXACT24, stream, &A16));
+  smat_bcsr_desc d{};
+  smat_status st = smat_bcsr_describe(A16, &d);
+  if (st == SMAT_OK && d.nnzb > 0) {
+    hipLaunchKernelGGL(k_count_tag1, grid_of(d.nnzb, 256), 256, 0, s, d.nnzb, d.tags, a + 21);
+    if (hipGetLastError() != hipSuccess) st = smat::set_error(SMAT_ERR_HIP, "smat_matrix_stats: launch failed");
+  }
+  unsigned long long h[32];
+  if (st == SMAT_OK) {
+    const hipError_t e = hipMemcpyAsync(h, a, sizeof(h), hipMemcpyDeviceToHost, s);
+    const hipError_t e2 = e == hipSuccess ? hipStreamSynchronize(s) : e;
+    if (e2 != hipSuccess) st = smat::set_error(SMAT_ERR_HIP, "smat_matrix_stats: copy: %s", hipGetErrorString(e2));
+  }
+  smat_bcsr_destroy(A16);
+  if (st != SMAT_OK) return st;
+  out->rows = M;
+  out->cols = K;
+  out->nnz = nnz;
+  for (int b = 0; b < 12; b++) out->row_hist[b] = (int64_t)h[b];
+  out->max_row = (int64_t)h[12];
+  out->sum_abs_offset = (int64_t)h[13];
+  out->near_diag = (int64_t)h[14];
+  for (int w = 0; w < 6; w++) out->nnzb[w] = (int64_t)h[15 + w];
+  out->nnzb16 = d.nnzb;
+  out->blocks24 = (int64_t)h[21];
+  return SMAT_OK;
+}
+
+smat_status smat_tuner_features(const smat_matrix_stats_t* st, int64_t N, float* out) {
+  SMAT_REQUIRE(st && out && N > 0 && st->rows > 0, "smat_tuner_features: bad argument");
+  const double rows = (double)st->rows, nnz = (double)st->nnz;
+  const double mean = nnz / rows;
+  int k = 0;
+  out[k++] = (float)std::log2(rows);
+  out[k++] = (float)std::log2(nnz + 1.0);
+  out[k++] = (float)std::log2(mean + 1.0);
+  out[k++] = (float)std::log2((double)st->max_row + 1.0) - (float)std::log2(mean + 1.0);
+  for (int b = 0; b < 12; b++) out[k++] = (float)((double)st->row_hist[b] / rows);
+  out[k++] = (float)(nnz > 0 ? (double)st->sum_abs_offset / nnz / rows : 0.0);
+  out[k++] = (float)(nnz > 0 ? (double)st->near_diag / nnz : 0.0);
+  for (int w = 0; w < 6; w++) {  // log2 fill of 16 x bk blocks, bk = 1 << w
+    const double stored = (double)st->nnzb[w] * 16.0 * (double)(1 << w);
+    out[k++] = (float)(stored > 0 ? std::log2(nnz / stored) : 0.0);
+  }
+  out[k++] = (float)(st->nnzb16 > 0 ? (double)st->blocks24 / (double)st->nnzb16 : 0.0);
+  out[k++] = (float)std::log2((double)N);
+  return k == SMAT_TUNER_NF ? SMAT_OK : smat::set_error(SMAT_ERR_INVALID, "smat_tuner_features: internal count");
+}
+
+smat_status smat_linear_forward(const float* X, int64_t B, int32_t n_in, const float* W, const float* bias,
+                                int32_t n_out, int32_t relu, float* Y, smat_stream stream) {
+  SMAT_REQUIRE(X && W && bias && Y && B > 0 && n_in > 0 && n_out > 0, "smat_linear_forward: bad argument");
+  hipLaunchKernelGGL(k_linear_fwd, grid_of(B * n_out, 256), 256, 0, smat::to_hip(stream), X, B, n_in, W, bias, n_out,
+                     relu, Y);
+  SMAT_LAUNCH_CHECK();
+  return SMAT_OK;
+}
+
+smat_status smat_linear_backward(const float* X, const float* Y, const float* dY, int64_t B, int32_t n_in,
+                                 const float* W, int32_t n_out, int32_t relu, float* dX, float* dW, float* db,
+                                 smat_stream stream) {
+  SMAT_REQUIRE(X && Y && dY && W && dW && db && B > 0 && n_in > 0 && n_out > 0, "smat_linear_backward: bad argument");
+  hipStream_t s = smat::to_hip(stream);
+  hipLaunchKernelGGL(k_linear_bwd_w, grid_of((int64_t)n_out * (n_in + 1), 64), 64, 0, s, X, Y, dY, B, n_in, n_out,
+                     relu, dW, db);
+  if (dX) hipLaunchKernelGGL(k_linear_bwd_x, grid_of(B * n_in, 256), 256, 0, s, Y, dY, B, n_in, W, n_out, relu, dX);
+  SMAT_LAUNCH_CHECK();
+  return SMAT_OK;
+}
+
+smat_status smat_mse_grad(const float* pred, const float* y, int64_t B, float* dpred, float* loss,
+                          smat_stream stream) {
+  SMAT_REQUIRE(pred && y && dpred && loss && B > 0, "smat_mse_grad: bad argument");
+  hipLaunchKernelGGL(k_mse, 1, 256, 0, smat::to_hip(stream), pred, y, B, dpred, loss);
+  SMAT_LAUNCH_CHECK();
+  return SMAT_OK;
+}
+
+smat_status smat_adam_step(float* p, const float* g, float* m, float* v, int64_t n, float lr, float beta1,
+                           float beta2, float eps, int32_t t, smat_stream stream) {
+  SMAT_REQUIRE(p && g && m && v && n > 0 && t >= 1, "smat_adam_step: bad argument");
+  const float c1 = 1.f - std::pow(beta1, (float)t), c2 = 1.f - std::pow(beta2, (float)t);
+  hipLaunchKernelGGL(k_adam, grid_of(n, 256), 256, 0, smat::to_hip(stream), p, g, m, v, n, lr, beta1, beta2, eps, c1,
+                     c2);
+  SMAT_LAUNCH_CHECK();
+  return SMAT_OK;
+}
+
+smat_status smat_tuner_predict(const float* feats, int32_t n_feat, const float* cfg_enc, int32_t n_enc,
+                               const uint8_t* avail, int32_t n_cfg, const float* mu, const float* sigma,
+                               const float* W1, const float* b1, const float* W2, const float* b2, const float* W3,
+                               const float* b3, int32_t hidden, float* scores, int32_t* best, smat_stream stream) {
+  SMAT_REQUIRE(feats && cfg_enc && avail && mu && sigma && W1 && b1 && W2 && b2 && W3 && b3 && scores && best,
+               "smat_tuner_predict: null argument");
+  SMAT_REQUIRE(n_feat + n_enc <= kMaxIn && hidden <= kMaxHidden && hidden > 0 && n_cfg > 0 && n_cfg <= kMaxCfg,
+               "smat_tuner_predict: sizes exceed the kernel's limits (inputs %d, hidden %d, configs %d)",
+               kMaxIn, kMaxHidden, kMaxCfg);
+  hipLaunchKernelGGL(k_mlp_predict, 1, 64, 0, smat::to_hip(stream), feats, n_feat, cfg_enc, n_enc, avail, n_cfg, mu,
+                     sigma, W1, b1, W2, b2, W3, b3, hidden, scores, best);
+  SMAT_LAUNCH_CHECK();
+  return SMAT_OK;
+}
+
+}  // extern "C"

@@ -61,6 +61,21 @@ class SpmmOpts(ctypes.Structure):
 SMAT_BSTAGE_AUTO, SMAT_BSTAGE_DIRECT, SMAT_BSTAGE_LDS = 0, 1, 2
 
 
+class MatrixStats(ctypes.Structure):
+    _fields_ = [("rows", ctypes.c_int64), ("cols", ctypes.c_int64), ("nnz", ctypes.c_int64),
+                ("max_row", ctypes.c_int64), ("row_hist", ctypes.c_int64 * 12),
+                ("sum_abs_offset", ctypes.c_int64), ("near_diag", ctypes.c_int64),
+                ("nnzb", ctypes.c_int64 * 6), ("nnzb16", ctypes.c_int64), ("blocks24", ctypes.c_int64)]
+
+    def as_dict(self):
+        d = {"rows": self.rows, "cols": self.cols, "nnz": self.nnz, "max_row": self.max_row}
+        d.update({f"row_hist{b}": self.row_hist[b] for b in range(12)})
+        d.update({"sum_abs_offset": self.sum_abs_offset, "near_diag": self.near_diag})
+        d.update({f"nnzb{w}": self.nnzb[w] for w in range(6)})
+        d.update({"nnzb16": self.nnzb16, "blocks24": self.blocks24})
+        return d
+
+
 # name -> (restype, argtypes); must cover every function declared in smat.h
 SIGNATURES = {
     "smat_last_error": (ctypes.c_char_p, []),
@@ -87,6 +102,26 @@ SIGNATURES = {
     "smat_batch_info": (_i32, [_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i32),
                                ctypes.POINTER(_i32)]),
     "smat_batch_destroy": (_i32, [_p]),
+    "smat_matrix_stats": (ctypes.c_int32, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(MatrixStats),
+                                           ctypes.c_void_p]),
+    "smat_tuner_features": (ctypes.c_int32, [ctypes.POINTER(MatrixStats), ctypes.c_int64, ctypes.c_void_p]),
+    "smat_linear_forward": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
+                                             ctypes.c_void_p]),
+    "smat_linear_backward": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                              ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "smat_mse_grad": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p]),
+    "smat_adam_step": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_int64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                        ctypes.c_float, ctypes.c_int32, ctypes.c_void_p]),
+    "smat_tuner_predict": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32,
+                                            ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p]),
     "smat_pack_a": (_i32, [ctypes.POINTER(BcsrDesc), _p, _i32, _p, _p, _p]),
     "smat_pack_b": (_i32, [_p, _i64, _i64, _i64, _i32, _p, _p, _p]),
     "smat_error_sums": (_i32, [_p, _i64, _p, _i64, _i64, _i64, _p, _p, _p]),

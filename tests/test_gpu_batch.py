@@ -61,15 +61,23 @@ def _build(mats, N, seed=1, with_rowmap=False):
             ci_a = np.concatenate([ci[rp[r]:rp[r + 1]] for r in perm]).astype(np.int32)
             v_a = np.concatenate([v[rp[r]:rp[r + 1]] for r in perm])
             del inv
-        A = smat.Bcsr(M, K, _dev(rp_a), _dev(ci_a), _dev(v_a.view(np.int16)), bk=bk)
-        plan = smat.SpmmPlan(A, N, n_tile=n_tile, max_blocks_per_item=split)
+        mixed = i % 4 == 2  # block-clustered matrices with 2:4 tiles: the mixed dense / 2:4 kernel
+        if mixed:
+            A = smat.Bcsr(M, K, _dev(rp_a), _dev(ci_a), _dev(v_a.view(np.int16)), bk=16,
+                          tag_rule=smat.SMAT_TAG_EXACT24)
+            v24, i24 = A.compress_24(smat.SMAT_24_SMFMAC)
+            plan = smat.SpmmPlan(A, N, n_tile=n_tile, max_blocks_per_item=split, kernel=smat.SMAT_KERNEL_MIXED24)
+        else:
+            A = smat.Bcsr(M, K, _dev(rp_a), _dev(ci_a), _dev(v_a.view(np.int16)), bk=bk)
+            v24 = i24 = None
+            plan = smat.SpmmPlan(A, N, n_tile=n_tile, max_blocks_per_item=split)
         B = gen.dense_b(K, N, seed=seed + i)
         Bd = _dev(B.view(np.int16))
         C = torch.full((M, N), float("nan"), dtype=torch.float32, device="cuda")
         rm = _dev(perm) if perm is not None else None
-        ops = plan.operands(Bd, C, c_rowmap=rm)
+        ops = plan.operands(Bd, C, c_rowmap=rm, values24=v24, index24=i24)
         entries.append((plan, ops))
-        keep.append((A, Bd, C, rm))
+        keep.append((A, Bd, C, rm, v24, i24))
         refs.append(of.spmm_csr_f64(M, rp, ci, v, B))
     return entries, refs, keep
 
@@ -84,8 +92,8 @@ def test_batch_matches_each_call_and_oracle(with_rowmap):
     print(f"batch of {len(entries)} rowmap={with_rowmap}", flush=True)
     # one call each
     singles = []
-    for (plan, ops), (A, Bd, C, rm) in zip(entries, keep):
-        plan.execute(Bd, C, c_rowmap=rm)
+    for (plan, ops), (A, Bd, C, rm, v24, i24) in zip(entries, keep):
+        plan.execute(Bd, C, c_rowmap=rm, values24=v24, index24=i24)
         torch.cuda.synchronize()
         singles.append(C.cpu().numpy().copy())
         C.fill_(float("nan"))
@@ -94,7 +102,7 @@ def test_batch_matches_each_call_and_oracle(with_rowmap):
     assert info["tasks"] > 0 and info["grid"] > 0
     batch()
     torch.cuda.synchronize()
-    for i, (ref, single, (A, Bd, C, rm)) in enumerate(zip(refs, singles, keep)):
+    for i, (ref, single, (A, Bd, C, *_)) in enumerate(zip(refs, singles, keep)):
         got = C.cpu().numpy()
         assert np.array_equal(got, single), i  # same tiles, same order of accumulation
         assert _rel_fro(got.astype(np.float64), ref) < 1e-5, i
@@ -110,22 +118,22 @@ def test_batch_graph_replays():
     batch = smat.Batch(entries)
     batch()
     torch.cuda.synchronize()
-    eager = [C.cpu().numpy().copy() for (_, _, C, _) in keep]
+    eager = [C.cpu().numpy().copy() for (_, _, C, *_) in keep]
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
         batch.capture(s)
     assert batch.info()["captured"]
     for rep in range(4):
-        for (_, _, C, _) in keep:
+        for (_, _, C, *_) in keep:
             C.fill_(float("nan"))  # eager work between replays
         torch.cuda.synchronize()
         with torch.cuda.stream(s):
             batch(s)
         s.synchronize()
-        for i, (_, _, C, _) in enumerate(keep):
+        for i, (_, _, C, *_) in enumerate(keep):
             assert np.array_equal(C.cpu().numpy(), eager[i]), (rep, i)
     # new B values in the same buffers are picked up by a replay
-    (_, Bd, C0, _) = keep[0]
+    (_, Bd, C0, *_) = keep[0]
     Bd.view(torch.float16).mul_(2)  # B holds FP16 bits
     torch.cuda.synchronize()
     with torch.cuda.stream(s):

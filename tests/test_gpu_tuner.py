@@ -1,0 +1,167 @@
+"""GPU parity of the learned tuner: matrix statistics (exact, vs the oracle),
+features, the MLP kernels (forward, backward, Adam vs the FP64 oracle), the
+inference kernel, a training run on the GPU, and what the trained model gains
+over the method without it on held-out sweep timings."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_ffi as of
+from smat import gen
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TUNER_DIR = os.path.join(os.path.dirname(HERE), "sptc-smat_amd", "tuner")
+
+
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("seed", [10000, 10001, 10002, 10003, 10006, 10007])
+def test_stats_match_oracle(seed):
+    _need_gpu()
+    from smat import tuner
+    kind, (M, K, rp, ci, v) = tuner.make_matrix(seed)
+    print(f"stats seed {seed} {kind} rows {M} nnz {ci.size}", flush=True)
+    st = tuner.stats(M, K, _dev(rp), _dev(ci), _dev(v.view(np.int16)), int(ci.size))
+    assert st.as_dict() == of.matrix_stats(M, K, rp, ci, v)
+
+
+def test_features_formula():
+    _need_gpu()
+    from smat import tuner
+    kind, (M, K, rp, ci, v) = tuner.make_matrix(10005)
+    s = of.matrix_stats(M, K, rp, ci, v)
+    st = tuner.stats(M, K, _dev(rp), _dev(ci), _dev(v.view(np.int16)), int(ci.size))
+    f = tuner.features(st, 128)
+    rows, nnz = s["rows"], s["nnz"]
+    mean = nnz / rows
+    want = [np.log2(rows), np.log2(nnz + 1), np.log2(mean + 1), np.log2(s["max_row"] + 1) - np.log2(mean + 1)]
+    want += [s[f"row_hist{b}"] / rows for b in range(12)]
+    want += [s["sum_abs_offset"] / nnz / rows, s["near_diag"] / nnz]
+    want += [np.log2(nnz / (s[f"nnzb{w}"] * 16 * (1 << w))) for w in range(6)]
+    want += [s["blocks24"] / s["nnzb16"], np.log2(128)]
+    np.testing.assert_allclose(f, np.float32(want), rtol=2e-6, atol=1e-7)
+
+
+def _rand_params(n_in, h, rng):
+    return [rng.standard_normal((h, n_in)).astype(np.float32) * 0.4, rng.standard_normal(h).astype(np.float32) * 0.1,
+            rng.standard_normal((h, h)).astype(np.float32) * 0.2, rng.standard_normal(h).astype(np.float32) * 0.1,
+            rng.standard_normal((1, h)).astype(np.float32) * 0.2, rng.standard_normal(1).astype(np.float32)]
+
+
+@pytest.mark.parametrize("B,n_in,h", [(1, 3, 4), (37, 37, 64), (1024, 37, 64), (300, 64, 17)])
+def test_mlp_kernels_vs_oracle(B, n_in, h):
+    _need_gpu()
+    from smat import tuner
+    rng = np.random.default_rng(B + n_in)
+    X = rng.standard_normal((B, n_in)).astype(np.float32)
+    y = rng.standard_normal(B).astype(np.float32)
+    params = _rand_params(n_in, h, rng)
+    net = tuner.Net(n_in, hidden=h, arrays=params)
+    loss, pred, grads = tuner.forward_backward(net, _dev(X), _dev(y))
+    torch.cuda.synchronize()
+    epred, _, _ = of.mlp_forward(X, params)
+    eloss, egrads = of.mlp_backward(X, y, params)
+    np.testing.assert_allclose(pred.cpu().numpy().ravel(), epred, rtol=1e-4, atol=1e-5)
+    assert abs(float(loss.item()) - eloss) <= 1e-4 * max(1.0, eloss)
+    for k, (g, e) in enumerate(zip(grads, egrads)):
+        g = g.cpu().numpy().astype(np.float64)
+        scale = max(np.abs(e).max(), 1e-6)
+        assert np.abs(g - e).max() <= 1e-4 * scale, k
+
+
+def test_adam_kernel_vs_oracle():
+    _need_gpu()
+    from smat import tuner
+    rng = np.random.default_rng(4)
+    n = 1000
+    p = rng.standard_normal(n).astype(np.float32)
+    dp, dm, dv = _dev(p.copy()), torch.zeros(n, device="cuda"), torch.zeros(n, device="cuda")
+    ep, em, ev = p.astype(np.float64), np.zeros(n), np.zeros(n)
+    for t in range(1, 6):
+        g = rng.standard_normal(n).astype(np.float32)
+        tuner.check(tuner._lib().smat_adam_step(dp.data_ptr(), _dev(g).data_ptr(), dm.data_ptr(), dv.data_ptr(), n,
+                                                1e-2, 0.9, 0.999, 1e-8, t, tuner._stream()))
+        of.adam_step(ep, g.astype(np.float64), em, ev, 1e-2, 0.9, 0.999, 1e-8, t)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(dp.cpu().numpy(), ep, rtol=1e-5, atol=1e-6)
+
+
+def test_predict_kernel_matches_forward():
+    _need_gpu()
+    from smat import tuner
+    rng = np.random.default_rng(8)
+    n_in = tuner.NF + tuner.NE
+    params = _rand_params(n_in, tuner.HIDDEN, rng)
+    mu = rng.standard_normal(n_in).astype(np.float32)
+    sigma = (rng.random(n_in) + 0.5).astype(np.float32)
+    w = {"W1": params[0], "b1": params[1], "W2": params[2], "b2": params[3], "W3": params[4], "b3": params[5],
+         "mu": mu, "sigma": sigma, "hidden": np.int32(tuner.HIDDEN)}
+    model = tuner.Model(w)
+    feats = rng.standard_normal(tuner.NF).astype(np.float32)
+    for rows in (1000, 100000):
+        avail = tuner.available(rows)
+        sc, best = model.scores(feats, avail)
+        X = (np.concatenate([np.repeat(feats[None], len(tuner.CONFIGS), 0), tuner.ENC], 1) - mu) / sigma
+        epred, _, _ = of.mlp_forward(X.astype(np.float32), params)
+        np.testing.assert_allclose(sc, epred, rtol=1e-4, atol=1e-4)
+        masked = np.where(avail > 0, sc, np.inf)
+        assert best == int(np.argmin(masked))
+
+
+def test_training_on_gpu_reduces_loss():
+    _need_gpu()
+    from smat import tuner
+    rng = np.random.default_rng(1)
+    # a synthetic cost surface the model can learn: log time depends on features x config
+    n_mat = 120
+    feats = rng.standard_normal((n_mat, tuner.NF)).astype(np.float32)
+    times = np.exp(0.5 * feats[:, :1] * (tuner.ENC[:, :6] @ np.arange(6))[None] / 6 + 0.3 * feats[:, 1:2]
+                   * tuner.ENC[:, 6:9].argmax(1)[None]) * 0.1
+    w = tuner.train(feats, times, epochs=60, batch=512, log=None)
+    curve = w["curve"]
+    assert curve[-1] < 0.25 * curve[0], curve[[0, -1]]
+
+
+def _sweep_files(prefix):
+    return sorted(os.path.join(TUNER_DIR, f) for f in os.listdir(TUNER_DIR) if f.startswith(prefix)) \
+        if os.path.isdir(TUNER_DIR) else []
+
+
+def test_trained_model_on_held_out_matrices():
+    """The committed model against the committed held-out sweep (timings of
+    every configuration measured on MI355X): picks within 10% of the sweep
+    best on >= 90% of the matrices, and faster than both the plain
+    configuration and the best single fixed configuration."""
+    _need_gpu()
+    from smat import tuner
+    test_files = _sweep_files("sweep_test")
+    train_files = _sweep_files("sweep_train")
+    if not (test_files and train_files and os.path.exists(tuner.WEIGHTS_PATH)):
+        pytest.skip("no committed sweep data / weights")
+    te = [np.load(f) for f in test_files]
+    feats = np.concatenate([d["feats"] for d in te])
+    times = np.concatenate([d["times"] for d in te])
+    rows = np.concatenate([d["rows"] for d in te])
+    r = tuner.evaluate(tuner.Model(), feats, times, rows)
+    tr_times = np.concatenate([np.load(f)["times"] for f in train_files])
+    fb = tuner.fixed_best(np.where(np.isfinite(tr_times), tr_times, np.inf))
+    fixed = np.array([t[fb] if np.isfinite(t[fb]) else t[tuner.PLAIN] for t in times])
+    picked = np.array([t[p] for t, p in zip(times, r["picks"])])
+    gain_vs_fixed = float(np.exp(np.mean(np.log(fixed / picked))))
+    print(json.dumps({k: r[k] for k in ("n", "within_10pct", "median_ratio_to_best", "geo_speedup_vs_plain")}
+                     | {"fixed_best": str(tuner.CONFIGS[fb]), "geo_gain_vs_fixed_best": gain_vs_fixed}), flush=True)
+    assert r["within_10pct"] >= 0.9
+    assert r["geo_speedup_vs_plain"] > 1.3
+    assert gain_vs_fixed > 1.0

@@ -76,6 +76,36 @@ typedef void* smat_stream;
 #define SMAT_TAG_EXACT24 1   /* 1 iff every aligned 1x4 group of every row has <= 2 nonzeros */
 
 /* ------------------------------------------------------------------------ */
+/* Host readers                                                             */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int64_t rows, cols, nnz;
+  int32_t* rowptr;       /* [rows + 1] */
+  int32_t* colidx;       /* [nnz], each row in file order */
+  uint16_t* values_f16;  /* [nnz] FP16 bits (one rounding from the file's value) */
+  float* values_f32;     /* [nnz] */
+  int32_t symmetric;     /* MTX: the file was symmetric / hermitian and was mirrored */
+} smat_csr_host;
+
+/*
+ * Matrix Market coordinate file -> host CSR, as mmio_allinone
+ * (src/cuda_hgemm/src/common/mmio_highlevel.h:523-680): rows padded to a
+ * multiple of 16 and columns to a multiple of k (padSparseMatrix, :514-520),
+ * pattern entries 1, complex entries their real part, symmetric / hermitian
+ * files mirrored, file order kept in each row, duplicates kept.
+ */
+smat_status smat_read_mtx(const char* path, int32_t k, smat_csr_host* out);
+/*
+ * Edge list (`row<d>col[<d>value]` per line, rows not descending) -> host
+ * CSR, as CSR::read_from_edgelist (preprocess/src/general/csr.cpp:166-290)
+ * in edge-list mode: rows = last row + 1, cols = largest column + 1, values 1
+ * when pattern_only.  delimiter NULL or "" means " ".
+ */
+smat_status smat_read_edgelist(const char* path, const char* delimiter, int32_t pattern_only,
+                               smat_csr_host* out);
+void smat_csr_host_free(smat_csr_host* csr);
+
+/* ------------------------------------------------------------------------ */
 /* BCSR object: 16 x bk blocks, built on the GPU                            */
 /* ------------------------------------------------------------------------ */
 typedef struct smat_bcsr_t* smat_bcsr;
@@ -182,6 +212,27 @@ smat_status smat_grouping_to_permutation(int64_t rows, const int64_t* grouping_h
 smat_status smat_permute_csr_rows(int64_t rows, const int32_t* rowptr, const int32_t* colidx,
                                   const uint16_t* values, const int64_t* perm, int32_t* out_rowptr,
                                   int32_t* out_colidx, uint16_t* out_values, smat_stream stream);
+
+/*
+ * Column pass of the reference's reorder driver (src/reorder.py:143-183):
+ * the pattern (host arrays) is transposed with each transposed row sorted, as
+ * transpose_el_matrix does, grouped by the same Keeper (GPU) and turned into
+ * a column order: new column c is old column col_perm_host[c].  The product
+ * is unchanged when A's columns are relabelled (smat_permute_csr_cols with
+ * the inverse order) and B's rows permuted the same way
+ * (smat_permute_dense_rows with col_perm).
+ */
+smat_status smat_reorder_columns(int64_t rows, int64_t cols, const int32_t* rowptr_host,
+                                 const int32_t* colidx_host, float tau, int64_t col_block,
+                                 int64_t max_rows, int64_t* col_perm_host, int64_t* counters_host,
+                                 smat_stream stream);
+/* out_colidx[q] = inv_perm[colidx[q]] (device arrays; inv_perm: int64[cols]). */
+smat_status smat_permute_csr_cols(int64_t nnz, const int32_t* colidx, const int64_t* inv_perm,
+                                  int32_t* out_colidx, smat_stream stream);
+/* Row r of out (K x N, ld_out) = row perm[r] of in (ld_in); elem_bytes 2 or 4;
+ * perm: device int64[K], a permutation of 0..K-1. */
+smat_status smat_permute_dense_rows(int64_t K, int64_t N, int32_t elem_bytes, const void* in, int64_t ld_in,
+                                    const int64_t* perm, void* out, int64_t ld_out, smat_stream stream);
 
 /* ------------------------------------------------------------------------ */
 /* 2:4 compression                                                          */

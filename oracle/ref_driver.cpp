@@ -170,4 +170,33 @@ void ref_reorder_rows(intT rows, intT cols, const intT* rowptr,
   }
 }
 
+// CSR::read_from_edgelist (csr.cpp:166-290) on a file, edge-list mode.
+// Returns nnz (rowptr/colidx/vals written when they fit in cap), or -1 when
+// the reference rejects the file.
+intT ref_read_edgelist(const char* path, int pattern_only, intT* rows, intT* cols, intT* rowptr, intT* colidx,
+                       float* vals, intT cap_rows, intT cap_nnz) {
+  try {
+    std::ifstream in(path);
+    CSR m(in, " ", pattern_only != 0, el);
+    *rows = m.rows;
+    *cols = m.cols;
+    intT nnz = 0;
+    for (intT i = 0; i < m.rows; i++) nnz += m.nzcount[i];
+    if (m.rows + 1 <= cap_rows && nnz <= cap_nnz) {
+      rowptr[0] = 0;
+      intT at = 0;
+      for (intT i = 0; i < m.rows; i++) {
+        for (intT q = 0; q < m.nzcount[i]; q++, at++) {
+          colidx[at] = m.ja[i][q];
+          vals[at] = pattern_only ? 1.f : m.ma[i][q];
+        }
+        rowptr[i + 1] = at;
+      }
+    }
+    return nnz;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
 }  // extern "C"

@@ -219,6 +219,33 @@ __global__ void k_perm_gather(int64_t rows, const int64_t* __restrict__ perm, co
   }
 }
 
+// Column relabel: new column of an entry = inv[old column].
+__global__ void k_relabel_cols(int64_t nnz, const int32_t* __restrict__ colidx, const int64_t* __restrict__ inv,
+                               int32_t* __restrict__ out) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nnz) out[q] = (int32_t)inv[colidx[q]];
+}
+
+// Dense row gather: out row r = in row perm[r] (elem_bytes-wide elements,
+// 16-byte chunks where the row is 16-byte aligned, else 2-byte units).
+__global__ void k_gather_rows(int64_t K, int64_t row_bytes, const uint8_t* __restrict__ in, int64_t ld_in_bytes,
+                              const int64_t* __restrict__ perm, uint8_t* __restrict__ out, int64_t ld_out_bytes) {
+  const int64_t r = blockIdx.y;
+  if (r >= K) return;
+  const uint8_t* src = in + perm[r] * ld_in_bytes;
+  uint8_t* dst = out + r * ld_out_bytes;
+  const bool wide = ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst) | row_bytes) & 15) == 0;
+  if (wide) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < row_bytes / 16;
+         i += (int64_t)gridDim.x * blockDim.x)
+      reinterpret_cast<uint4*>(dst)[i] = reinterpret_cast<const uint4*>(src)[i];
+  } else {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < row_bytes / 2;
+         i += (int64_t)gridDim.x * blockDim.x)
+      reinterpret_cast<uint16_t*>(dst)[i] = reinterpret_cast<const uint16_t*>(src)[i];
+  }
+}
+
 inline unsigned nblk(int64_t n, int t) {
   const int64_t g = (n + t - 1) / t;
   return (unsigned)(g < 1 ? 1 : g);
@@ -476,6 +503,70 @@ smat_status smat_permute_csr_rows(int64_t rows, const int32_t* rowptr, const int
                      out_colidx, out_values);
   SMAT_LAUNCH_CHECK();
   SMAT_HIP_TRY(hipStreamSynchronize(s));
+  return SMAT_OK;
+}
+
+// Column pass of the reference's reorder driver (src/reorder.py:143-183):
+// transpose the pattern (transpose_el_matrix sorts the swapped edges, so each
+// transposed row lists its original rows in ascending order), run the same
+// Keeper grouping on it (Matrix_Blocking -a 5 -F 1), and turn the grouping
+// into the order of the columns.
+smat_status smat_reorder_columns(int64_t rows, int64_t cols, const int32_t* rowptr_host, const int32_t* colidx_host,
+                                 float tau, int64_t col_block, int64_t max_rows, int64_t* col_perm_host,
+                                 int64_t* counters_host, smat_stream stream) {
+  SMAT_REQUIRE(rows > 0 && cols > 0 && rowptr_host && colidx_host && col_perm_host,
+               "smat_reorder_columns: bad argument");
+  SMAT_REQUIRE(cols < INT32_MAX, "smat_reorder_columns: too many columns");
+  const int64_t nnz = rowptr_host[rows];
+  std::vector<int32_t> trp((size_t)cols + 1, 0), tci((size_t)(nnz > 0 ? nnz : 1));
+  for (int64_t q = 0; q < nnz; q++) {
+    SMAT_REQUIRE(colidx_host[q] >= 0 && colidx_host[q] < cols, "smat_reorder_columns: column out of range");
+    trp[(size_t)colidx_host[q] + 1]++;
+  }
+  for (int64_t c = 0; c < cols; c++) trp[(size_t)c + 1] += trp[(size_t)c];
+  std::vector<int32_t> fill(trp.begin(), trp.end() - 1);
+  for (int64_t r = 0; r < rows; r++)  // rows in ascending order: each transposed row comes out sorted
+    for (int32_t q = rowptr_host[r]; q < rowptr_host[r + 1]; q++) tci[(size_t)fill[(size_t)colidx_host[q]]++] = (int32_t)r;
+  hipStream_t s = smat::to_hip(stream);
+  smat::DevBuf d_rp, d_ci;
+  SMAT_TRY(d_rp.alloc(trp.size() * 4));
+  SMAT_TRY(d_ci.alloc(tci.size() * 4));
+  SMAT_HIP_TRY(hipMemcpyAsync(d_rp.p, trp.data(), trp.size() * 4, hipMemcpyHostToDevice, s));
+  SMAT_HIP_TRY(hipMemcpyAsync(d_ci.p, tci.data(), tci.size() * 4, hipMemcpyHostToDevice, s));
+  std::vector<int64_t> g((size_t)cols), fixed((size_t)cols);
+  SMAT_TRY(smat_reorder_grouping(cols, d_rp.as<int32_t>(), d_ci.as<int32_t>(), trp.data(), tci.data(), tau, col_block,
+                                 max_rows, g.data(), counters_host, stream));
+  SMAT_TRY(smat_fixed_size_grouping(cols, g.data(), max_rows, fixed.data()));
+  return smat_grouping_to_permutation(cols, fixed.data(), col_perm_host);
+}
+
+smat_status smat_permute_csr_cols(int64_t nnz, const int32_t* colidx, const int64_t* inv_perm, int32_t* out_colidx,
+                                  smat_stream stream) {
+  SMAT_REQUIRE(nnz >= 0 && (nnz == 0 || (colidx && inv_perm && out_colidx)), "smat_permute_csr_cols: bad argument");
+  if (nnz == 0) return SMAT_OK;
+  hipLaunchKernelGGL(k_relabel_cols, nblk(nnz, 256), 256, 0, smat::to_hip(stream), nnz, colidx, inv_perm, out_colidx);
+  SMAT_LAUNCH_CHECK();
+  return SMAT_OK;
+}
+
+smat_status smat_permute_dense_rows(int64_t K, int64_t N, int32_t elem_bytes, const void* in, int64_t ld_in,
+                                    const int64_t* perm, void* out, int64_t ld_out, smat_stream stream) {
+  SMAT_REQUIRE(K > 0 && N > 0 && in && perm && out && ld_in >= N && ld_out >= N &&
+                   (elem_bytes == 2 || elem_bytes == 4),
+               "smat_permute_dense_rows: bad argument");
+  SMAT_REQUIRE(K < 65536 * 32768LL, "smat_permute_dense_rows: too many rows");
+  const int64_t row_bytes = N * elem_bytes;
+  const unsigned gx = (unsigned)std::min<int64_t>(8, (row_bytes / 16 + 255) / 256 + 1);
+  // one row per grid.y index, in slabs of 65535 rows (the grid.y limit)
+  hipStream_t s = smat::to_hip(stream);
+  const int64_t step = 65535;
+  for (int64_t r0 = 0; r0 < K; r0 += step) {
+    const int64_t nr = std::min(step, K - r0);
+    hipLaunchKernelGGL(k_gather_rows, dim3(gx, (unsigned)nr), 256, 0, s, nr, row_bytes,
+                       static_cast<const uint8_t*>(in), ld_in * elem_bytes, perm + r0,
+                       static_cast<uint8_t*>(out) + r0 * ld_out * elem_bytes, ld_out * elem_bytes);
+  }
+  SMAT_LAUNCH_CHECK();
   return SMAT_OK;
 }
 

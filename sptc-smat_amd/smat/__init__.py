@@ -76,6 +76,12 @@ class MatrixStats(ctypes.Structure):
         return d
 
 
+class CsrHost(ctypes.Structure):
+    _fields_ = [("rows", _i64), ("cols", _i64), ("nnz", _i64), ("rowptr", ctypes.POINTER(ctypes.c_int32)),
+                ("colidx", ctypes.POINTER(ctypes.c_int32)), ("values_f16", ctypes.POINTER(ctypes.c_uint16)),
+                ("values_f32", ctypes.POINTER(ctypes.c_float)), ("symmetric", _i32)]
+
+
 # name -> (restype, argtypes); must cover every function declared in smat.h
 SIGNATURES = {
     "smat_last_error": (ctypes.c_char_p, []),
@@ -96,6 +102,12 @@ SIGNATURES = {
     "smat_spmm_execute_ex": (_i32, [_p, ctypes.POINTER(SpmmOperands), _p]),
     "smat_spmm_plan_destroy": (_i32, [_p]),
     "smat_bcsr_scatter_values": (_i32, [ctypes.POINTER(BcsrDesc), _p, _p, _p, _p, _p]),
+    "smat_reorder_columns": (_i32, [_i64, _i64, _p, _p, ctypes.c_float, _i64, _i64, _p, _p, _p]),
+    "smat_permute_csr_cols": (_i32, [_i64, _p, _p, _p, _p]),
+    "smat_permute_dense_rows": (_i32, [_i64, _i64, _i32, _p, _i64, _p, _p, _i64, _p]),
+    "smat_read_mtx": (_i32, [ctypes.c_char_p, _i32, ctypes.POINTER(CsrHost)]),
+    "smat_read_edgelist": (_i32, [ctypes.c_char_p, ctypes.c_char_p, _i32, ctypes.POINTER(CsrHost)]),
+    "smat_csr_host_free": (None, [ctypes.POINTER(CsrHost)]),
     "smat_batch_create": (_i32, [_i32, _p, _p, _p, ctypes.POINTER(_p)]),
     "smat_batch_capture": (_i32, [_p, _p]),
     "smat_batch_execute": (_i32, [_p, _p]),
@@ -372,6 +384,49 @@ def reorder_grouping(rowptr, colidx, tau: float, col_block: int, max_rows: int, 
     return g, cnt
 
 
+def reorder_columns(rowptr, colidx, cols: int, tau: float, col_block: int, max_rows: int, stream=None):
+    """Column order from the Keeper on the transposed pattern (host int32
+    arrays); returns (col_perm int64[cols], [comparisons, merges])."""
+    import numpy as np
+    rp = np.ascontiguousarray(rowptr, np.int32)
+    ci = np.ascontiguousarray(colidx, np.int32) if len(colidx) else np.zeros(1, np.int32)
+    perm = np.zeros(cols, np.int64)
+    cnt = np.zeros(2, np.int64)
+    check(lib().smat_reorder_columns(rp.size - 1, cols, rp.ctypes.data, ci.ctypes.data, float(tau), col_block,
+                                     max_rows, perm.ctypes.data, cnt.ctypes.data, stream_ptr(stream)))
+    return perm, cnt
+
+
+REFERENCE_TAUS = (0.9, 0.8, 0.6, 0.5, 0.4, 0.3, 0.2, 0.1, 0.01)
+
+
+def threshold_search(rowptr, colidx, cols: int, col_block: int = 16, row_block: int = 16, taus=REFERENCE_TAUS):
+    """The reference's exhaustive threshold search
+    (preprocess/test/general/reorder_matrix.cpp:17-33): for each tau, Keeper
+    grouping forced to fixed-size groups (BlockingEngine::GetGrouping with
+    -F 1, blocking.cpp:651-654), then the number of nonzero (group, column
+    block) pairs (VBR_nzblocks_count of CollectBlockingInfo,
+    blocking.cpp:562-614); the best count wins, ties going to the later tau
+    (`<=`), starting from the nonzero count.  Returns (best_tau, best_blocks,
+    {tau: blocks})."""
+    import numpy as np
+    rp = np.ascontiguousarray(rowptr, np.int32)
+    ci = np.ascontiguousarray(colidx, np.int32)
+    rows = rp.size - 1
+    row_of = np.repeat(np.arange(rows, dtype=np.int64), np.diff(rp))
+    nbc = (cols + col_block - 1) // col_block
+    best_tau, best = -1.0, int(ci.size)
+    counts = {}
+    for tau in taus:
+        g, _ = reorder_grouping(rp, ci, np.float32(tau), col_block, row_block)
+        fixed = fixed_size_grouping(g, row_block)
+        blocks = int(np.unique(fixed[row_of] * nbc + ci // col_block).size)
+        counts[tau] = blocks
+        if blocks <= best:
+            best_tau, best = tau, blocks
+    return best_tau, best, counts
+
+
 def fixed_size_grouping(grouping, row_block: int):
     import numpy as np
     g = np.ascontiguousarray(grouping, np.int64)
@@ -480,3 +535,32 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ---- readers ------------------------------------------------------------------
+def _csr_from_host(c: CsrHost):
+    import numpy as np
+    try:
+        rows, nnz = int(c.rows), int(c.nnz)
+        rp = np.ctypeslib.as_array(c.rowptr, shape=(rows + 1,)).copy()
+        ci = np.ctypeslib.as_array(c.colidx, shape=(max(nnz, 1),))[:nnz].copy()
+        v16 = np.ctypeslib.as_array(c.values_f16, shape=(max(nnz, 1),))[:nnz].copy()
+        v32 = np.ctypeslib.as_array(c.values_f32, shape=(max(nnz, 1),))[:nnz].copy()
+        return {"rows": rows, "cols": int(c.cols), "nnz": nnz, "rowptr": rp, "colidx": ci, "values_f16": v16,
+                "values_f32": v32, "symmetric": bool(c.symmetric)}
+    finally:
+        lib().smat_csr_host_free(ctypes.byref(c))
+
+
+def read_mtx(path: str, k: int = 16):
+    """Matrix Market file -> dict of host CSR arrays (smat_read_mtx)."""
+    c = CsrHost()
+    check(lib().smat_read_mtx(path.encode(), k, ctypes.byref(c)))
+    return _csr_from_host(c)
+
+
+def read_edgelist(path: str, delimiter: str = " ", pattern_only: bool = False):
+    """Edge-list file -> dict of host CSR arrays (smat_read_edgelist)."""
+    c = CsrHost()
+    check(lib().smat_read_edgelist(path.encode(), delimiter.encode(), int(pattern_only), ctypes.byref(c)))
+    return _csr_from_host(c)

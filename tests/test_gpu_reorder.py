@@ -82,3 +82,76 @@ def test_permute_rows_on_gpu():
     assert np.array_equal(o_rp.cpu().numpy(), erp)
     assert np.array_equal(o_ci.cpu().numpy(), eci)
     assert np.array_equal(o_v.cpu().numpy().view(np.uint16), ev)
+
+
+def _transpose_sorted(M, K, rp, ci):
+    rows = np.repeat(np.arange(M), np.diff(rp))
+    order = np.lexsort((rows, ci))  # by column, then row: each transposed row ascending
+    trp = np.zeros(K + 1, np.int64)
+    np.add.at(trp, ci + 1, 1)
+    return np.cumsum(trp), rows[order].astype(np.int64)
+
+
+@pytest.mark.parametrize("scale,tau", [(10, 0.5), (11, 0.3)])
+def test_column_pass_matches_oracle(scale, tau):
+    """The column order equals the Keeper (oracle, itself pinned to the
+    compiled reference) run on the sorted transpose (src/reorder.py:143-183)."""
+    _need_gpu()
+    import smat
+    M, K, rp, ci, _ = gen.rmat(scale=scale, edge_factor=8, seed=300 + scale)
+    print(f"column pass scale {scale} tau {tau}", flush=True)
+    perm, cnt = smat.reorder_columns(rp, ci, K, np.float32(tau), 16, 16)
+    trp, tci = _transpose_sorted(M, K, rp, ci)
+    g, ecnt = of.keeper(trp, tci, np.float32(tau), 16, 16)
+    eperm = of.get_permutation(of.fixed_size_grouping(g, 16))
+    assert np.array_equal(cnt, ecnt)
+    assert np.array_equal(perm, eperm)
+
+
+def test_column_reordered_spmm_equals_original():
+    _need_gpu()
+    import smat
+    M, K, rp, ci, v = gen.rmat(scale=11, edge_factor=8, seed=9)
+    N = 64
+    perm, _ = smat.reorder_columns(rp, ci, K, np.float32(0.5), 16, 16)
+    inv = np.argsort(perm).astype(np.int64)
+    d_ci = torch.from_numpy(ci).cuda()
+    d_ci2 = torch.empty_like(d_ci)
+    smat.check(smat.lib().smat_permute_csr_cols(ci.size, d_ci.data_ptr(), torch.from_numpy(inv).cuda().data_ptr(),
+                                                d_ci2.data_ptr(), smat.stream_ptr()))
+    B = gen.dense_b(K, N, seed=4)
+    d_B = torch.from_numpy(B.view(np.int16)).cuda()
+    d_B2 = torch.empty_like(d_B)
+    smat.check(smat.lib().smat_permute_dense_rows(K, N, 2, d_B.data_ptr(), N, torch.from_numpy(perm).cuda().data_ptr(),
+                                                  d_B2.data_ptr(), N, smat.stream_ptr()))
+    torch.cuda.synchronize()
+    assert np.array_equal(d_ci2.cpu().numpy(), inv[ci])
+    assert np.array_equal(d_B2.cpu().numpy().view(np.uint16), B[perm])
+    for bk in (2, 16):
+        A = smat.Bcsr(M, K, torch.from_numpy(rp).cuda(), d_ci2, torch.from_numpy(v.view(np.int16)).cuda(), bk=bk)
+        C = torch.empty((M, N), dtype=torch.float32, device="cuda")
+        smat.SpmmPlan(A, N)(d_B2, C)
+        torch.cuda.synchronize()
+        R = of.spmm_csr_f64(M, rp, ci, v, B)
+        err = np.linalg.norm(C.cpu().numpy() - R) / np.linalg.norm(R)
+        assert err < 1e-5, (bk, err)
+
+
+def test_threshold_search_matches_oracle_blocking_info():
+    """Per-tau block counts equal the oracle's CollectBlockingInfo (pinned to
+    the reference KAT) on the oracle's fixed-size Keeper grouping, and the
+    winner follows reorder_matrix.cpp's `<=` rule."""
+    _need_gpu()
+    import smat
+    M, K, rp, ci, _ = gen.rmat(scale=10, edge_factor=8, seed=41)
+    taus = (0.9, 0.5, 0.3, 0.1)
+    best_tau, best, counts = smat.threshold_search(rp, ci, K, 16, 16, taus)
+    want_best_tau, want_best = -1.0, ci.size
+    for tau in taus:
+        g, _ = of.keeper(rp.astype(np.int64), ci.astype(np.int64), np.float32(tau), 16, 16)
+        fixed = of.fixed_size_grouping(g, 16)
+        info, _ = of.collect_blocking_info(rp.astype(np.int64), ci.astype(np.int64), fixed, K, 16)
+        assert counts[tau] == int(info[1]), tau  # VBR_nzblocks_count
+        if info[1] <= want_best:
+            want_best_tau, want_best = tau, int(info[1])
+    assert (best_tau, best) == (want_best_tau, want_best)

@@ -340,17 +340,19 @@ smat_status smat_reorder_grouping(int64_t rows, const int32_t* rowptr, const int
   // Pinned, device-mapped host buffers: the round kernels write the compared
   // list and its header straight into host memory (one sync per round);
   // marks hold the rows to flag as grouped (each row is written once).
-  smat::HostBuf h_j, h_d, h_hdr, h_marks, h_pat;
-  SMAT_TRY(h_j.alloc(rows * 4));
-  SMAT_TRY(h_d.alloc(rows * 4));
-  SMAT_TRY(h_hdr.alloc(16));
+  // Two result areas: while the host replays round r from one, the GPU
+  // already runs round r + 1 into the other.
+  smat::HostBuf h_j[2], h_d[2], h_hdr, h_marks, h_pat;
+  for (int b = 0; b < 2; b++) {
+    SMAT_TRY(h_j[b].alloc(rows * 4));
+    SMAT_TRY(h_d[b].alloc(rows * 4));
+  }
+  SMAT_TRY(h_hdr.alloc(32));
   SMAT_TRY(h_marks.alloc(rows * 8));
   hipLaunchKernelGGL(k_init, nblk(rows, 256), 256, 0, s, rows, d.as<float>(), grouped.as<uint8_t>());
   hipLaunchKernelGGL(k_reset_first, 1, 1, 0, s, first.as<unsigned long long>());
   SMAT_LAUNCH_CHECK();
 
-  const int32_t* oj = h_j.as<int32_t>();
-  const float* od = h_d.as<float>();
   volatile int64_t* hdr = h_hdr.as<int64_t>();
   int64_t* marks = h_marks.as<int64_t>();
   int64_t marks_used = 0;
@@ -377,9 +379,10 @@ smat_status smat_reorder_grouping(int64_t rows, const int32_t* rowptr, const int
     int64_t gsize = 1;
     grouping[i] = group;
     mark(&i, 1);
-    int64_t j0 = i + 1;
     bool pattern_dirty = true;
-    while (j0 < rows && gsize != max_rows) {
+    // One round: every candidate of [j0, rows) against the current pattern,
+    // results into area b.
+    auto launch_round = [&](int b, int64_t j0) -> smat_status {
       if (pattern_dirty) {
         // the pattern goes through a pinned staging area so the copy is asynchronous
         if (pattern.size() + 1 > pat_cap) {
@@ -399,14 +402,48 @@ smat_status smat_reorder_grouping(int64_t rows, const int32_t* rowptr, const int
                          status.as<uint8_t>(), dist.as<float>(), lpos.as<int32_t>(), bcount.as<int32_t>(),
                          first.as<unsigned long long>());
       hipLaunchKernelGGL(k_round_offsets, 1, 1024, 0, s, nb, bcount.as<int32_t>(), boff.as<int32_t>(),
-                         first.as<unsigned long long>(), h_hdr.dev<int64_t>());
+                         first.as<unsigned long long>(), h_hdr.dev<int64_t>() + 2 * b);
       hipLaunchKernelGGL(k_round_write, nb, kScanThreads, 0, s, rows, j0, status.as<uint8_t>(), dist.as<float>(),
-                         lpos.as<int32_t>(), boff.as<int32_t>(), h_j.dev<int32_t>(), h_d.dev<float>());
+                         lpos.as<int32_t>(), boff.as<int32_t>(), h_j[b].dev<int32_t>(), h_d[b].dev<float>());
       SMAT_LAUNCH_CHECK();
+      return SMAT_OK;
+    };
+    int64_t j0 = i + 1;
+    int b = 0;
+    bool launched = false;
+    if (j0 < rows && gsize != max_rows) {
+      SMAT_TRY(launch_round(b, j0));
+      launched = true;
+    }
+    while (launched) {
       SMAT_HIP_TRY(hipStreamSynchronize(s));
-      const int64_t ncmp = hdr[0];
-      const int64_t jm = hdr[1];
-      // compared candidates strictly before the merge (or all of them)
+      const int64_t ncmp = hdr[2 * b];
+      const int64_t jm = hdr[2 * b + 1];
+      // the truncation bound of this round's candidates: before its merge
+      const size_t keep = (size_t)(max_rows - (int64_t)merged.size());
+      const int64_t jend = jm == INT64_MAX ? rows : jm + 1;
+      hipLaunchKernelGGL(k_commit, nblk(jend - j0, 256), 256, 0, s, j0, jend, status.as<uint8_t>(),
+                         dist.as<float>(), d.as<float>());
+      SMAT_LAUNCH_CHECK();
+      launched = false;
+      if (jm != INT64_MAX) {  // merge row jm, then start the next round at once
+        comparisons++;
+        merges++;
+        grouping[jm] = group;
+        merged.push_back(jm);
+        pattern = merge_rows(pattern, colidx_host + rowptr_host[jm], rowptr_host[jm + 1] - rowptr_host[jm]);
+        pattern_dirty = true;
+        gsize++;
+        mark(&jm, 1);
+        if (jm + 1 < rows && gsize != max_rows) {
+          SMAT_TRY(launch_round(1 - b, jm + 1));
+          launched = true;
+        }
+      }
+      // replay this round's compared candidates strictly before the merge (or
+      // all of them) while the GPU works on the next round
+      const int32_t* oj = h_j[b].as<int32_t>();
+      const float* od = h_d[b].as<float>();
       for (int64_t q = 0; q < ncmp && oj[q] < jm; q++) {
         comparisons++;
         const std::pair<float, int64_t> key(od[q], (int64_t)oj[q]);
@@ -416,28 +453,14 @@ smat_status smat_reorder_grouping(int64_t rows, const int32_t* rowptr, const int
         // exactly those of a plain insert.
         if (!best.empty() && *best.rbegin() < key) best.insert(best.end(), key);
         else best.insert(key);
-        const size_t keep = (size_t)(max_rows - (int64_t)merged.size());
         if (best.size() > keep) {
           auto it = best.end();  // blocking.cpp:505-507
           std::advance(it, keep);
           best.erase(it, best.end());
         }
       }
-      const int64_t jend = jm == INT64_MAX ? rows : jm + 1;
-      hipLaunchKernelGGL(k_commit, nblk(jend - j0, 256), 256, 0, s, j0, jend, status.as<uint8_t>(),
-                         dist.as<float>(), d.as<float>());
-      SMAT_LAUNCH_CHECK();
-      if (jm == INT64_MAX) break;
-      // merge row jm
-      comparisons++;
-      merges++;
-      grouping[jm] = group;
-      merged.push_back(jm);
-      pattern = merge_rows(pattern, colidx_host + rowptr_host[jm], rowptr_host[jm + 1] - rowptr_host[jm]);
-      pattern_dirty = true;
-      gsize++;
-      mark(&jm, 1);
-      j0 = jm + 1;
+      j0 = jend;
+      b = 1 - b;
     }
     // top-up from the candidate set (blocking.cpp:513-521)
     std::vector<int64_t> topped;

@@ -31,6 +31,13 @@ sys.path.insert(0, os.path.join(ROOT, "sptc-smat_amd"))
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per SpMM call measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (calibrated with a 1 GiB copy: x2.0 / x1.0) for configurations that have a
+# committed profile: (workload, scale, N, bk, n_tile) -> (bytes, profile file).
+MEASURED_TRAFFIC = {
+    ("rmat", 18, 256, 2, "auto"): (3.286e9, "profiles/spmm_lds_bk2_rmat18_n256_hbm_traffic.md"),
+}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -555,6 +562,8 @@ def main():
             cpu = cpu_baseline(M, rp, ci, v16, ctx.N, args.cpu_seconds)
         except Exception as ex:  # the baseline is a report, never the measurement
             cpu = {"error": str(ex)}
+    traffic = MEASURED_TRAFFIC.get((args.workload, args.scale, ctx.N, tuned["bk"], tuned["n_tile"])) \
+        if tuned["kernel"] == "dense" and args.dtype == "f16" else None
     out = {
         "metric": "SpMM useful GFLOP/s (2*nnz*N / t), learned-tuned vs plain FP16 BCSR",
         "value": tuned["useful_gflops"],
@@ -582,7 +591,9 @@ def main():
         "stored_block_gflops": tuned["stored_gflops"],
         "build_ms": tuned["build_ms"],
         "roofline": {"bound": "hbm", "achieved": tuned["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(tuned["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(tuned["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": traffic[0] if traffic else None,
+                     "traffic_source": traffic[1] if traffic else None,
                      "kernel": "SpMM call (main kernel + split-row reduction)",
                      "alg_bytes_per_call": tuned["alg_bytes"], "event_ms_avg": tuned["event_ms"]},
         "sweep": [{k: s[k] for k in ("bk", "n_tile", "kernel", "event_ms", "useful_gflops")} for s in sweep],

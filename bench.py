@@ -562,8 +562,10 @@ def main():
             cpu = cpu_baseline(M, rp, ci, v16, ctx.N, args.cpu_seconds)
         except Exception as ex:  # the baseline is a report, never the measurement
             cpu = {"error": str(ex)}
-    traffic = MEASURED_TRAFFIC.get((args.workload, args.scale, ctx.N, tuned["bk"], tuned["n_tile"])) \
-        if tuned["kernel"] == "dense" and args.dtype == "f16" else None
+    # "auto" N-tiles are 64 columns for N >= 64
+    nt_key = "auto" if tuned["n_tile"] == "auto" or (tuned["n_tile"] == 64 and ctx.N >= 64) else tuned["n_tile"]
+    traffic = MEASURED_TRAFFIC.get((args.workload, args.scale, ctx.N, tuned["bk"], nt_key)) \
+        if tuned["kernel"] == "dense" and args.dtype == "f16" and args.split == 0 else None
     out = {
         "metric": "SpMM useful GFLOP/s (2*nnz*N / t), learned-tuned vs plain FP16 BCSR",
         "value": tuned["useful_gflops"],

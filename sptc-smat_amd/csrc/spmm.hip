@@ -646,26 +646,43 @@ struct Reduce {
   int32_t pad;
 };
 
-// Sums the partial tiles of each split block row in slice order.
-template <bool OUT_F16>
+// Sums the partial tiles of each split block row in slice order.  With
+// N % 4 == 0 (and 16-byte aligned C rows for FP32 output) a thread handles
+// four consecutive columns with 16-byte loads; the summation order per
+// element is the same either way.
+template <bool OUT_F16, bool VEC4>
 __global__ void k_reduce(const Reduce* __restrict__ red, int64_t n_red, int32_t chunks,
                          const float* __restrict__ scratch, int64_t N, void* __restrict__ C, int64_t ldc,
                          const int32_t* __restrict__ rowmap) {
   const int64_t r = blockIdx.x / chunks;
   if (r >= n_red) return;
   const Reduce rd = red[r];
-  const int64_t total = 16 * N;
+  constexpr int V = VEC4 ? 4 : 1;
+  const int64_t per_row = N / V, total = 16 * per_row;
   for (int64_t i = (int64_t)(blockIdx.x % chunks) * blockDim.x + threadIdx.x; i < total;
        i += (int64_t)chunks * blockDim.x) {
-    const int64_t row = i / N, n = i % N;
-    float s = 0.f;
-    for (int q = 0; q < rd.slot_cnt; q++) s += scratch[((int64_t)(rd.slot_beg + q) * 16 + row) * N + n];
+    const int64_t row = i / per_row, n = (i % per_row) * V;
     int64_t crow = (int64_t)rd.brow * 16 + row;
     if (rowmap) crow = rowmap[crow];
-    if (OUT_F16)
-      reinterpret_cast<_Float16*>(C)[crow * ldc + n] = (_Float16)s;
-    else
-      reinterpret_cast<float*>(C)[crow * ldc + n] = s;
+    if constexpr (VEC4) {
+      smat_f4 s = smat_f4{0.f, 0.f, 0.f, 0.f};
+      for (int q = 0; q < rd.slot_cnt; q++)
+        s += *reinterpret_cast<const smat_f4*>(scratch + ((int64_t)(rd.slot_beg + q) * 16 + row) * N + n);
+      if (OUT_F16) {
+        _Float16* c = reinterpret_cast<_Float16*>(C) + crow * ldc + n;
+#pragma unroll
+        for (int e = 0; e < 4; e++) c[e] = (_Float16)s[e];
+      } else {
+        *reinterpret_cast<smat_f4*>(reinterpret_cast<float*>(C) + crow * ldc + n) = s;
+      }
+    } else {
+      float s = 0.f;
+      for (int q = 0; q < rd.slot_cnt; q++) s += scratch[((int64_t)(rd.slot_beg + q) * 16 + row) * N + n];
+      if (OUT_F16)
+        reinterpret_cast<_Float16*>(C)[crow * ldc + n] = (_Float16)s;
+      else
+        reinterpret_cast<float*>(C)[crow * ldc + n] = s;
+    }
   }
 }
 
@@ -1202,14 +1219,21 @@ smat_status smat_spmm_execute_ex(smat_plan P, const smat_spmm_operands* ops, sma
     dispatch_bk(P, op, s);
   }
   if (P->n_red > 0) {
-    const int32_t chunks = (int32_t)((16 * P->N + 255) / 256);
+    const bool f16 = P->opts.out_dtype == SMAT_F16;
+    // 16-byte scratch loads need N % 4 == 0; FP32 C also needs 16-byte aligned rows
+    const bool vec = P->N % 4 == 0 &&
+                     (f16 || (ldc % 4 == 0 && (reinterpret_cast<uintptr_t>(ops->c) & 15) == 0));
+    const int32_t chunks = (int32_t)((16 * P->N / (vec ? 4 : 1) + 255) / 256);
     const unsigned g = (unsigned)(chunks * P->n_red);
-    if (P->opts.out_dtype == SMAT_F16)
-      hipLaunchKernelGGL(k_reduce<true>, g, 256, 0, s, P->red.as<Reduce>(), P->n_red, chunks,
-                         P->scratch.as<float>(), P->N, ops->c, ldc, ops->c_rowmap);
-    else
-      hipLaunchKernelGGL(k_reduce<false>, g, 256, 0, s, P->red.as<Reduce>(), P->n_red, chunks,
-                         P->scratch.as<float>(), P->N, ops->c, ldc, ops->c_rowmap);
+    const Reduce* rd = P->red.as<Reduce>();
+    const float* sc = P->scratch.as<float>();
+    if (vec) {
+      if (f16) hipLaunchKernelGGL((k_reduce<true, true>), g, 256, 0, s, rd, P->n_red, chunks, sc, P->N, ops->c, ldc, ops->c_rowmap);
+      else hipLaunchKernelGGL((k_reduce<false, true>), g, 256, 0, s, rd, P->n_red, chunks, sc, P->N, ops->c, ldc, ops->c_rowmap);
+    } else {
+      if (f16) hipLaunchKernelGGL((k_reduce<true, false>), g, 256, 0, s, rd, P->n_red, chunks, sc, P->N, ops->c, ldc, ops->c_rowmap);
+      else hipLaunchKernelGGL((k_reduce<false, false>), g, 256, 0, s, rd, P->n_red, chunks, sc, P->N, ops->c, ldc, ops->c_rowmap);
+    }
   }
   SMAT_LAUNCH_CHECK();
   return SMAT_OK;

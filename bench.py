@@ -51,6 +51,7 @@ def parse():
     ap.add_argument("--n-tile", type=int, default=0)
     ap.add_argument("--stage", type=int, default=0, help="B staging: 0 auto, 1 direct, 2 LDS")
     ap.add_argument("--split", type=int, default=0, help="max blocks per work item (0 = the plan's default)")
+    ap.add_argument("--schedule", type=int, default=0, help="0 auto, 1 block rows (XCD ranges), 2 longest first")
     ap.add_argument("--kernel", default="dense", choices=["dense", "mixed24"])
     ap.add_argument("--dtype", default="f16", choices=["f16", "bf16"])
     ap.add_argument("--single", action="store_true", help="time only --bk/--n-tile (no sweep, no plain leg)")
@@ -188,7 +189,7 @@ class Run:
         torch.cuda.synchronize()
         self.build_ms = (time.perf_counter() - t0) * 1e3
         self.plan = smat.SpmmPlan(self.A, ctx.N, n_tile=n_tile, b_staging=ctx.args.stage,
-                                  max_blocks_per_item=ctx.args.split,
+                                  max_blocks_per_item=ctx.args.split, schedule=ctx.args.schedule,
                                   kernel=smat.SMAT_KERNEL_MIXED24 if kernel == "mixed24" else smat.SMAT_KERNEL_DENSE,
                                   in_dtype=smat.SMAT_BF16 if ctx.args.dtype == "bf16" else smat.SMAT_F16)
         self.ctx = ctx

@@ -274,8 +274,15 @@ typedef struct {
   int32_t max_blocks_per_item; /* split block rows longer than this; 0 = auto */
   int32_t b_staging;       /* SMAT_BSTAGE_* (row-major B only) */
   int32_t in_dtype;        /* SMAT_F16 (0 = F16), SMAT_BF16, SMAT_FP8_E4M3 or SMAT_SPLIT_F16 */
-  int32_t reserved;
+  int32_t schedule;        /* SMAT_SCHED_* */
 } smat_spmm_opts;
+
+/* Order in which work items reach the GPU. */
+#define SMAT_SCHED_AUTO 0          /* longest first when block-row lengths are skewed (longest over
+                                      8x the mean), block-row order otherwise */
+#define SMAT_SCHED_BLOCK_ROWS 1    /* block-row order; each XCD takes a contiguous range (L2 locality) */
+#define SMAT_SCHED_LONGEST_FIRST 2 /* items sorted by length, longest dispatched first: hub rows start
+                                      early, so only rows over half a wave's fair share are split */
 
 typedef struct smat_plan_t* smat_plan;
 

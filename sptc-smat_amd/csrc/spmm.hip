@@ -186,10 +186,10 @@ __global__ __launch_bounds__(256) void k_spmm_dense(
     const Item* __restrict__ items, int64_t n_items, int32_t ntiles,
     const int32_t* __restrict__ colidx, const uint16_t* __restrict__ values,
     const uint16_t* __restrict__ B, int64_t ldb, int64_t N, void* __restrict__ C, int64_t ldc,
-    float* __restrict__ scratch, const int32_t* __restrict__ rowmap) {
+    float* __restrict__ scratch, const int32_t* __restrict__ rowmap, int32_t remap) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-  const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
   const int64_t task = (int64_t)wg * kWaves + wave;
   const int64_t item_id = task / ntiles;
   if (item_id >= n_items) return;
@@ -482,12 +482,12 @@ __device__ __forceinline__ void lds_tile(const Item& it, int64_t n0, const Opera
 
 template <int BK, int NSUB, int DT, bool OUT_F16>
 __global__ __launch_bounds__(256) void k_spmm_lds(const Item* __restrict__ items, int64_t n_items, int32_t ntiles,
-                                                  Operands op) {
+                                                  Operands op, int32_t remap) {
   constexpr int WB = lds_wave_bytes<NSUB, DT>();
   __shared__ __attribute__((aligned(16))) uint8_t lds_all[kWaves * WB];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-  const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
   const int64_t task = (int64_t)wg * kWaves + wave;
   const int64_t item_id = task / ntiles;
   if (item_id >= n_items) return;
@@ -625,12 +625,12 @@ __device__ __forceinline__ void mixed_tile(const Item& it, int64_t n0, const Ope
 
 template <int NSUB, int DT, bool OUT_F16>
 __global__ __launch_bounds__(256) void k_spmm_mixed(const Item* __restrict__ items, int64_t n_items, int32_t ntiles,
-                                                    Operands op, Mixed24 mx) {
+                                                    Operands op, Mixed24 mx, int32_t remap) {
   constexpr int WB = mixed_wave_bytes<NSUB>();
   __shared__ __attribute__((aligned(16))) uint8_t lds_all[kWaves * WB];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
   const int64_t task = (int64_t)wg * kWaves + wave;
   const int64_t item_id = task / ntiles;
   if (item_id >= n_items) return;
@@ -881,6 +881,45 @@ __global__ void k_plan_emit(int64_t nbrows, const int32_t* __restrict__ rp1, int
   red[red_off[r]] = Reduce{(int32_t)r, sbase, k, 0};
 }
 
+// Longest and total block-row length (blocks) of a BCSR.
+__global__ void k_row_len_stats(int64_t nbrows, const int32_t* __restrict__ rowptr,
+                                unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long mx, tot;
+  if (threadIdx.x == 0) mx = tot = 0;
+  __syncthreads();
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < nbrows) {
+    const unsigned long long len = (unsigned long long)(rowptr[r + 1] - rowptr[r]);
+    atomicMax(&mx, len);
+    atomicAdd(&tot, len);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicMax(&out[0], mx);
+    atomicAdd(&out[1], tot);
+  }
+}
+
+__global__ void k_zero2(unsigned long long* p) {
+  if (threadIdx.x < 2) p[threadIdx.x] = 0;
+}
+
+// Longest-first schedules: sort key (MFMA steps of the item) and identity index.
+__global__ void k_item_keys(int64_t n, const Item* __restrict__ items, int32_t step1, int32_t step2,
+                            int32_t* __restrict__ key, int32_t* __restrict__ idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Item it = items[i];
+  key[i] = ceil_div(it.pend - it.pbeg, step1) + ceil_div(it.send - it.sbeg, step2);
+  idx[i] = (int32_t)i;
+}
+
+__global__ void k_item_gather(int64_t n, const Item* __restrict__ items, const int32_t* __restrict__ idx,
+                              Item* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = items[idx[i]];
+}
+
 // Mixed plans: per-block flags and list positions of tag-2 and tag-1 blocks.
 __global__ void k_list_flags(int64_t nnzb, const int32_t* __restrict__ tags, int32_t* __restrict__ fd,
                              int32_t* __restrict__ fs) {
@@ -919,6 +958,7 @@ struct smat_plan_t {
   smat_spmm_opts opts{};
   int32_t nsub = 4;
   bool mixed = false;
+  bool remap = true;  // XCD-contiguous task ranges (off for longest-first schedules)
   int64_t n_items = 0, n_red = 0, n_slots = 0;
   int64_t n_dense = 0, n_sparse = 0;
   smat::DevBuf items, red, scratch, dense_ids, sparse_ids;
@@ -934,14 +974,14 @@ void launch_dense(const smat_plan_t* P, const Operands& op, hipStream_t s) {
   hipLaunchKernelGGL((k_spmm_dense<BK, NSUB, BLAYOUT, DT, OUT_F16>), blocks_for(P->n_items * ntiles, kWaves),
                      64 * kWaves, 0, s, P->items.as<Item>(), P->n_items, ntiles, op.colidx,
                      static_cast<const uint16_t*>(op.a), static_cast<const uint16_t*>(op.b), op.ldb, op.N, op.c,
-                     op.ldc, op.scratch, op.rowmap);
+                     op.ldc, op.scratch, op.rowmap, (int32_t)P->remap);
 }
 
 template <int BK, int NSUB, int DT, bool OUT_F16>
 void launch_lds(const smat_plan_t* P, const Operands& op, hipStream_t s) {
   const int32_t ntiles = ntiles_of(P, NSUB);
   hipLaunchKernelGGL((k_spmm_lds<BK, NSUB, DT, OUT_F16>), blocks_for(P->n_items * ntiles, kWaves), 64 * kWaves,
-                     0, s, P->items.as<Item>(), P->n_items, ntiles, op);
+                     0, s, P->items.as<Item>(), P->n_items, ntiles, op, (int32_t)P->remap);
 }
 
 template <int NSUB, int DT, bool OUT_F16>
@@ -950,7 +990,7 @@ void launch_mixed(const smat_plan_t* P, const uint16_t* v24, const uint16_t* i24
   const int32_t ntiles = ntiles_of(P, NSUB);
   const Mixed24 mx{P->dense_ids.as<int32_t>(), P->sparse_ids.as<int32_t>(), v24, i24};
   hipLaunchKernelGGL((k_spmm_mixed<NSUB, DT, OUT_F16>), blocks_for(P->n_items * ntiles, kWaves), 64 * kWaves, 0,
-                     s, P->items.as<Item>(), P->n_items, ntiles, op, mx);
+                     s, P->items.as<Item>(), P->n_items, ntiles, op, mx, (int32_t)P->remap);
 }
 
 inline bool lds_aligned(const smat_plan_t* P, const Operands& op, int elems_per_chunk) {
@@ -1034,6 +1074,21 @@ void dispatch_mixed(const smat_plan_t* P, const uint16_t* v24, const uint16_t* i
   }
 }
 
+smat_status row_length_stats(const smat_bcsr_desc* A, int64_t* mx, int64_t* tot, hipStream_t s) {
+  smat::DevBuf acc;
+  SMAT_TRY(acc.alloc(16));
+  hipLaunchKernelGGL(k_zero2, 1, 64, 0, s, acc.as<unsigned long long>());
+  hipLaunchKernelGGL(k_row_len_stats, blocks_for(A->nbrows, 256), 256, 0, s, A->nbrows, A->rowptr,
+                     acc.as<unsigned long long>());
+  SMAT_LAUNCH_CHECK();
+  unsigned long long h[2] = {0, 0};
+  SMAT_HIP_TRY(hipMemcpyAsync(h, acc.p, 16, hipMemcpyDeviceToHost, s));
+  SMAT_HIP_TRY(hipStreamSynchronize(s));
+  *mx = (int64_t)h[0];
+  *tot = (int64_t)h[1];
+  return SMAT_OK;
+}
+
 // Exclusive scan helper over n int32 with a reusable temp buffer.
 smat_status exscan(smat::DevBuf& tmp, const int32_t* in, int32_t* out, int64_t n, hipStream_t s) {
   size_t tb = 0;
@@ -1075,6 +1130,29 @@ smat_status build_items(smat_plan_t* P, const int32_t* rp1, int32_t step1, const
                      off_i.as<int32_t>(), off_s.as<int32_t>(), off_f.as<int32_t>(), P->items.as<Item>(),
                      P->red.as<Reduce>());
   SMAT_LAUNCH_CHECK();
+  if (!P->remap && P->n_items > 1) {
+    // longest first: items sorted by their MFMA steps, descending (stable)
+    const int64_t n = P->n_items;
+    smat::DevBuf k0, k1, i0, i1, sorted, tmp2;
+    SMAT_TRY(k0.alloc(n * 4));
+    SMAT_TRY(k1.alloc(n * 4));
+    SMAT_TRY(i0.alloc(n * 4));
+    SMAT_TRY(i1.alloc(n * 4));
+    SMAT_TRY(sorted.alloc(n * sizeof(Item)));
+    hipLaunchKernelGGL(k_item_keys, blocks_for(n, 256), 256, 0, s, n, P->items.as<Item>(), step1, step2,
+                       k0.as<int32_t>(), i0.as<int32_t>());
+    size_t tb = 0;
+    SMAT_HIP_TRY(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tb, k0.as<int32_t>(), k1.as<int32_t>(),
+                                                              i0.as<int32_t>(), i1.as<int32_t>(), (int)n, 0, 32, s));
+    SMAT_TRY(tmp2.alloc(tb));
+    SMAT_HIP_TRY(hipcub::DeviceRadixSort::SortPairsDescending(tmp2.p, tb, k0.as<int32_t>(), k1.as<int32_t>(),
+                                                              i0.as<int32_t>(), i1.as<int32_t>(), (int)n, 0, 32, s));
+    hipLaunchKernelGGL(k_item_gather, blocks_for(n, 256), 256, 0, s, n, P->items.as<Item>(), i1.as<int32_t>(),
+                       sorted.as<Item>());
+    SMAT_LAUNCH_CHECK();
+    SMAT_HIP_TRY(hipMemcpyAsync(P->items.p, sorted.p, n * sizeof(Item), hipMemcpyDeviceToDevice, s));
+    SMAT_HIP_TRY(hipStreamSynchronize(s));
+  }
   SMAT_HIP_TRY(hipStreamSynchronize(s));
   return SMAT_OK;
 }
@@ -1146,26 +1224,52 @@ smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat
                "smat_spmm_plan_create: n_tile must be 0, 16, 32 or 64");
   SMAT_REQUIRE(o.b_staging >= SMAT_BSTAGE_AUTO && o.b_staging <= SMAT_BSTAGE_LDS,
                "smat_spmm_plan_create: unknown b_staging %d", o.b_staging);
+  SMAT_REQUIRE(o.schedule == SMAT_SCHED_AUTO || o.schedule == SMAT_SCHED_BLOCK_ROWS ||
+                   o.schedule == SMAT_SCHED_LONGEST_FIRST,
+               "smat_spmm_plan_create: unknown schedule %d", o.schedule);
   hipStream_t s = smat::to_hip(stream);
+  if (o.schedule == SMAT_SCHED_AUTO) {
+    // skewed block rows (the longest over 8x the mean) go longest first
+    int64_t mx = 0, tot = 0;
+    SMAT_TRY(row_length_stats(A, &mx, &tot, s));
+    o.schedule = (tot > 0 && mx * A->nbrows > 8 * tot) ? SMAT_SCHED_LONGEST_FIRST : SMAT_SCHED_BLOCK_ROWS;
+  }
 
   smat_plan_t* P = new smat_plan_t();
   P->A = *A;
   P->N = N;
   P->opts = o;
   P->mixed = o.kernel == SMAT_KERNEL_MIXED24;
+  P->remap = o.schedule != SMAT_SCHED_LONGEST_FIRST;
   int nt = o.n_tile ? o.n_tile : (N >= 64 ? 64 : (N > 16 ? 32 : 16));
   if (o.in_dtype == SMAT_FP8_E4M3 && nt < 32) nt = 32;  // FP8 tiles are at least 32 columns wide
   P->nsub = nt / 16;
   const int32_t step = 32 / A->bk;  // blocks per MFMA step
   int32_t S = o.max_blocks_per_item;  // in blocks
   if (S <= 0) {
-    // aim for >= 64k wave tasks, but never slices under 12 MFMA steps: on
-    // mid-size matrices finer slices only add partial tiles to reduce
     const int64_t ntl = (N + nt - 1) / nt;
-    const int64_t target_items = (65536 + ntl - 1) / ntl;
-    const int64_t s_auto = (A->nnzb + target_items - 1) / target_items;
-    const int64_t s_min = 12 * (int64_t)step;
-    S = (int32_t)(s_auto < s_min ? s_min : (s_auto > 4096 ? 4096 : s_auto));
+    const int64_t s_min = 12 * (int64_t)step;  // finer slices only add partial tiles to reduce
+    int64_t s_auto;
+    if (P->remap) {
+      // block-row order: aim for >= 64k wave tasks so the tail stays short
+      const int64_t target_items = (65536 + ntl - 1) / ntl;
+      s_auto = (A->nnzb + target_items - 1) / target_items;
+    } else {
+      // longest first: a slice may be as long as half of one resident
+      // wave's fair share of the work (total steps x tiles / resident
+      // waves); longer rows start first and end within the kernel's span
+      // without being cut
+      int dev = 0, cus = 256;
+      if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        cus = 256;
+      }
+      // (half of it: the last rows to start still end with the rest)
+      const int64_t waves = (int64_t)cus * 16;
+      const int64_t steps = (A->nnzb + step - 1) / step;
+      s_auto = (steps * ntl + 2 * waves - 1) / (2 * waves) * step;
+    }
+    S = (int32_t)(s_auto < s_min ? s_min : (s_auto > 65536 ? 65536 : s_auto));
   }
   smat_status st;
   if (P->mixed)

@@ -54,11 +54,12 @@ class SpmmOperands(ctypes.Structure):
 class SpmmOpts(ctypes.Structure):
     _fields_ = [
         ("kernel", _i32), ("out_dtype", _i32), ("b_layout", _i32), ("n_tile", _i32),
-        ("max_blocks_per_item", _i32), ("b_staging", _i32), ("in_dtype", _i32), ("reserved", _i32),
+        ("max_blocks_per_item", _i32), ("b_staging", _i32), ("in_dtype", _i32), ("schedule", _i32),
     ]
 
 
 SMAT_BSTAGE_AUTO, SMAT_BSTAGE_DIRECT, SMAT_BSTAGE_LDS = 0, 1, 2
+SMAT_SCHED_AUTO, SMAT_SCHED_BLOCK_ROWS, SMAT_SCHED_LONGEST_FIRST = 0, 1, 2
 
 
 class MatrixStats(ctypes.Structure):
@@ -311,10 +312,11 @@ class PlanInfo:
 class SpmmPlan:
     def __init__(self, A: Bcsr | BcsrDesc, N: int, *, kernel: int = SMAT_KERNEL_AUTO,
                  out_dtype: int = SMAT_F32, b_layout: int = SMAT_B_ROW_MAJOR, n_tile: int = 0,
-                 max_blocks_per_item: int = 0, b_staging: int = 0, in_dtype: int = SMAT_F16, stream=None):
+                 max_blocks_per_item: int = 0, b_staging: int = 0, in_dtype: int = SMAT_F16, schedule: int = 0,
+                 stream=None):
         desc = A.desc if isinstance(A, Bcsr) else A
         self._keep = A
-        o = SpmmOpts(kernel, out_dtype, b_layout, n_tile, max_blocks_per_item, b_staging, in_dtype)
+        o = SpmmOpts(kernel, out_dtype, b_layout, n_tile, max_blocks_per_item, b_staging, in_dtype, schedule)
         h = _p()
         check(lib().smat_spmm_plan_create(ctypes.byref(desc), N, ctypes.byref(o), stream_ptr(stream),
                                           ctypes.byref(h)))

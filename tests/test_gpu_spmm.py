@@ -193,3 +193,31 @@ def test_invalid_arguments_raise():
         smat.SpmmPlan(A, 0)
     with pytest.raises(smat.SmatError):
         smat.SpmmPlan(A, 64, n_tile=48)
+
+
+@pytest.mark.parametrize("schedule", [0, 1, 2])
+@pytest.mark.parametrize("which", ["rmat", "band"])
+def test_schedules_match_oracle(schedule, which):
+    """Block-row order, longest first and the automatic choice all give the
+    FP64 oracle's product (the slicing differs, so only the FP32 summation
+    order changes)."""
+    _need_gpu()
+    import smat
+    M, K, rp, ci, v = (gen.rmat(scale=12, edge_factor=12, seed=12) if which == "rmat"
+                       else gen.band(n=1024, hb=24, p=0.5, seed=3))
+    N = 64
+    B = gen.dense_b(K, N, seed=8)
+    R = of.spmm_csr_f64(M, rp, ci, v, B)
+    for bk in (2, 16):
+        for split in (0, 32):
+            print(f"schedule {schedule} {which} bk {bk} split {split}", flush=True)
+            A = smat.Bcsr(M, K, _dev(rp), _dev(ci), _dev(v.view(np.int16)), bk=bk)
+            plan = smat.SpmmPlan(A, N, schedule=schedule, max_blocks_per_item=split)
+            C = torch.full((M, N), float("nan"), dtype=torch.float32, device="cuda")
+            plan(_dev(B.view(np.int16)), C)
+            torch.cuda.synchronize()
+            Cn = C.cpu().numpy().astype(np.float64)
+            assert np.all(np.isfinite(Cn))
+            assert np.linalg.norm(Cn - R) / np.linalg.norm(R) < 1e-5
+            plan.close()
+            A.close()

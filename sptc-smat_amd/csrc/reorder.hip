@@ -40,6 +40,7 @@
 namespace {
 
 constexpr int kScanThreads = 256;
+constexpr int32_t kLongRow = 64;  // rows with more entries are compared by a whole wave
 constexpr uint8_t ST_SKIP = 0, ST_PRUNED = 1, ST_COMPARED = 2;
 
 // JaccardDistanceGroup(pattern, gA, row, 1, block) over sorted column lists.
@@ -104,11 +105,13 @@ __global__ __launch_bounds__(kScanThreads) void k_round_scan(
       st = ST_SKIP;
     } else {
       const int32_t b = rowptr[j], e = rowptr[j + 1];
-      const float x = jaccard(pattern, plen, gsize, colidx + b, e - b, block);
-      dist[j] = x;
       st = ST_COMPARED;
       flag = 1;
-      if (x <= tau) atomicMin(first_merge, (unsigned long long)j);
+      if (e - b <= kLongRow) {  // longer rows get a whole wave in k_round_long
+        const float x = jaccard(pattern, plen, gsize, colidx + b, e - b, block);
+        dist[j] = x;
+        if (x <= tau) atomicMin(first_merge, (unsigned long long)j);
+      }
     }
     status[j] = st;
   }
@@ -126,6 +129,55 @@ __global__ __launch_bounds__(kScanThreads) void k_round_scan(
   }
   if (flag) local_pos[j] = before + in_wave;
   if (t == 0) block_count[blockIdx.x] = total;
+}
+
+// Round, long candidates: one wave per row of `long_rows` (ascending row
+// indices, each longer than kLongRow) that k_round_scan marked compared.  The
+// distance needs only three counts — distinct column blocks of the pattern
+// (n_blk_pat, from the host), of the row, and of the row's blocks present in
+// the pattern — and the same final expression as jaccard(), so it is
+// bit-identical to the per-thread path.
+__global__ __launch_bounds__(256) void k_round_long(const int64_t* __restrict__ long_rows, int64_t n_long,
+                                                    const int32_t* __restrict__ rowptr,
+                                                    const int32_t* __restrict__ colidx,
+                                                    const int32_t* __restrict__ pattern, int32_t plen,
+                                                    int64_t n_blk_pat, int64_t gsize, float tau, int64_t block,
+                                                    const uint8_t* __restrict__ status, float* __restrict__ dist,
+                                                    unsigned long long* __restrict__ first_merge) {
+  const int64_t w = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
+  if (w >= n_long) return;
+  const int64_t j = long_rows[w];
+  if (status[j] != ST_COMPARED) return;  // wave-uniform
+  const int32_t b = rowptr[j], e = rowptr[j + 1];
+  int64_t nb = 0, inter = 0;
+  for (int32_t q = b + lane; q < e; q += 64) {
+    const int64_t blk = colidx[q] / block;
+    if (q > b && colidx[q - 1] / block == blk) continue;  // count each block at its first entry
+    nb++;
+    // does the pattern hold a column of this block?  (first pattern column >= blk * block)
+    int32_t lo = 0, hi = plen;
+    const int64_t c0 = blk * block;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi) >> 1;
+      if (pattern[mid] < c0) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo < plen && pattern[lo] / block == blk) inter++;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    nb += __shfl_xor(nb, off);
+    inter += __shfl_xor(inter, off);
+  }
+  if (lane != 0) return;
+  float x;
+  if (plen == 0) x = 1.f;  // the row is not empty (longer than kLongRow)
+  else {
+    const int64_t count = (n_blk_pat - inter) * 1 + (nb - inter) * gsize;
+    x = (float)((2.0 * (double)count) / (double)(n_blk_pat * gsize + nb * 1 + count));
+  }
+  dist[j] = x;
+  if (x <= tau) atomicMin(first_merge, (unsigned long long)j);
 }
 
 // Round, kernel 2 (one block): block offsets, total and first merge ->
@@ -349,6 +401,14 @@ smat_status smat_reorder_grouping(int64_t rows, const int32_t* rowptr, const int
   }
   SMAT_TRY(h_hdr.alloc(32));
   SMAT_TRY(h_marks.alloc(rows * 8));
+  // rows longer than kLongRow, ascending: their distances take a wave each
+  std::vector<int64_t> long_rows;
+  for (int64_t r = 0; r < rows; r++)
+    if (rowptr_host[r + 1] - rowptr_host[r] > kLongRow) long_rows.push_back(r);
+  smat::DevBuf d_long;
+  SMAT_TRY(d_long.alloc((long_rows.size() + 1) * 8));
+  if (!long_rows.empty())
+    SMAT_HIP_TRY(hipMemcpyAsync(d_long.p, long_rows.data(), long_rows.size() * 8, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_init, nblk(rows, 256), 256, 0, s, rows, d.as<float>(), grouped.as<uint8_t>());
   hipLaunchKernelGGL(k_reset_first, 1, 1, 0, s, first.as<unsigned long long>());
   SMAT_LAUNCH_CHECK();
@@ -380,6 +440,7 @@ smat_status smat_reorder_grouping(int64_t rows, const int32_t* rowptr, const int
     grouping[i] = group;
     mark(&i, 1);
     bool pattern_dirty = true;
+    int64_t n_blk_pat = 0;  // distinct column blocks of the pattern
     // One round: every candidate of [j0, rows) against the current pattern,
     // results into area b.
     auto launch_round = [&](int b, int64_t j0) -> smat_status {
@@ -393,6 +454,9 @@ smat_status smat_reorder_grouping(int64_t rows, const int32_t* rowptr, const int
         std::copy(pattern.begin(), pattern.end(), h_pat.as<int32_t>());
         if (!pattern.empty())
           SMAT_HIP_TRY(hipMemcpyAsync(pat.p, h_pat.p, pattern.size() * 4, hipMemcpyHostToDevice, s));
+        n_blk_pat = 0;
+        for (size_t q = 0; q < pattern.size(); q++)
+          if (q == 0 || pattern[q] / col_block != pattern[q - 1] / col_block) n_blk_pat++;
         pattern_dirty = false;
       }
       const int64_t n = rows - j0;
@@ -401,6 +465,12 @@ smat_status smat_reorder_grouping(int64_t rows, const int32_t* rowptr, const int
                          (int32_t)pattern.size(), gsize, tau, col_block, d.as<float>(), grouped.as<uint8_t>(),
                          status.as<uint8_t>(), dist.as<float>(), lpos.as<int32_t>(), bcount.as<int32_t>(),
                          first.as<unsigned long long>());
+      const int64_t lb = std::lower_bound(long_rows.begin(), long_rows.end(), j0) - long_rows.begin();
+      const int64_t n_long = (int64_t)long_rows.size() - lb;
+      if (n_long > 0)
+        hipLaunchKernelGGL(k_round_long, nblk(n_long, 4), 256, 0, s, d_long.as<int64_t>() + lb, n_long, rowptr,
+                           colidx, pat.as<int32_t>(), (int32_t)pattern.size(), n_blk_pat, gsize, tau, col_block,
+                           status.as<uint8_t>(), dist.as<float>(), first.as<unsigned long long>());
       hipLaunchKernelGGL(k_round_offsets, 1, 1024, 0, s, nb, bcount.as<int32_t>(), boff.as<int32_t>(),
                          first.as<unsigned long long>(), h_hdr.dev<int64_t>() + 2 * b);
       hipLaunchKernelGGL(k_round_write, nb, kScanThreads, 0, s, rows, j0, status.as<uint8_t>(), dist.as<float>(),

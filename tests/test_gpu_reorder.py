@@ -155,3 +155,17 @@ def test_threshold_search_matches_oracle_blocking_info():
         if info[1] <= want_best:
             want_best_tau, want_best = tau, int(info[1])
     assert (best_tau, best) == (want_best_tau, want_best)
+
+
+@pytest.mark.parametrize("tau,cb", [(0.5, 16), (0.3, 4), (0.9, 32)])
+def test_keeper_long_rows_vs_oracle(tau, cb):
+    """Power-law rows (up to 600 entries): distances of rows over 64 entries
+    come from the wave-per-row kernel and must equal the per-thread ones."""
+    _need_gpu()
+    import smat
+    M, K, rp, ci, _ = gen.power_law(n=3000, max_len=600, seed=17)
+    print(f"power-law keeper tau {tau} cb {cb}, longest row {int(np.diff(rp).max())}", flush=True)
+    g, cnt = smat.reorder_grouping(rp, ci, np.float32(tau), cb, 16)
+    eg, ecnt = of.keeper(rp.astype(np.int64), ci.astype(np.int64), np.float32(tau), cb, 16)
+    assert np.array_equal(cnt, ecnt)
+    assert np.array_equal(g, eg)

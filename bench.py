@@ -34,9 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per SpMM call measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (calibrated with a 1 GiB copy: x2.0 / x1.0) for configurations that have a
 # committed profile: (workload, scale, N, bk, n_tile) -> (bytes, profile file).
-MEASURED_TRAFFIC = {
-    ("rmat", 18, 256, 2, "auto"): (3.286e9, "profiles/spmm_lds_bk2_rmat18_n256_hbm_traffic.md"),
-}
+MEASURED_TRAFFIC = {}
 
 
 def parse():
@@ -51,7 +49,7 @@ def parse():
     ap.add_argument("--n-tile", type=int, default=0)
     ap.add_argument("--stage", type=int, default=0, help="B staging: 0 auto, 1 direct, 2 LDS")
     ap.add_argument("--split", type=int, default=0, help="max blocks per work item (0 = the plan's default)")
-    ap.add_argument("--schedule", type=int, default=0, help="0 auto, 1 block rows (XCD ranges), 2 longest first")
+    ap.add_argument("--schedule", type=int, default=0, help="0 auto, 1 block rows (XCD ranges), 2 longest first, 3 longest first + XCD tile affinity")
     ap.add_argument("--kernel", default="dense", choices=["dense", "mixed24"])
     ap.add_argument("--dtype", default="f16", choices=["f16", "bf16"])
     ap.add_argument("--single", action="store_true", help="time only --bk/--n-tile (no sweep, no plain leg)")

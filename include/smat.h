@@ -270,7 +270,8 @@ typedef struct {
   int32_t kernel;          /* SMAT_KERNEL_* */
   int32_t out_dtype;       /* SMAT_F32 or SMAT_F16 */
   int32_t b_layout;        /* SMAT_B_ROW_MAJOR or SMAT_B_REFERENCE */
-  int32_t n_tile;          /* columns per wave: 16, 32 or 64; 0 = auto */
+  int32_t n_tile;          /* columns per wave: 16, 32, 64 or 128 (128: dense kernel, FP16 / BF16,
+                              row-major B staged through LDS); 0 = auto */
   int32_t max_blocks_per_item; /* split block rows longer than this; 0 = auto */
   int32_t b_staging;       /* SMAT_BSTAGE_* (row-major B only) */
   int32_t in_dtype;        /* SMAT_F16 (0 = F16), SMAT_BF16, SMAT_FP8_E4M3 or SMAT_SPLIT_F16 */
@@ -278,11 +279,15 @@ typedef struct {
 } smat_spmm_opts;
 
 /* Order in which work items reach the GPU. */
-#define SMAT_SCHED_AUTO 0          /* longest first when block-row lengths are skewed (longest over
-                                      8x the mean), block-row order otherwise */
+#define SMAT_SCHED_AUTO 0          /* block-row lengths skewed (longest over 8x the mean): tile
+                                      affinity with 2 or 4 N tiles, else longest first; block-row
+                                      order otherwise */
 #define SMAT_SCHED_BLOCK_ROWS 1    /* block-row order; each XCD takes a contiguous range (L2 locality) */
 #define SMAT_SCHED_LONGEST_FIRST 2 /* items sorted by length, longest dispatched first: hub rows start
                                       early, so only rows over half a wave's fair share are split */
+#define SMAT_SCHED_TILE_AFFINITY 3 /* longest first, and each XCD runs only some of the N tiles, so
+                                      its L2 holds one column slice of B; needs 8 % tiles == 0 or
+                                      tiles % 8 == 0 (tiles = ceil(N / n_tile)), else as 2 */
 
 typedef struct smat_plan_t* smat_plan;
 

@@ -32,9 +32,11 @@
 //  * Row-major B is staged through LDS with 128-bit loads and read back as
 //    operands with ds_read_b64_tr_b16 (see k_spmm_lds).
 //
-//  * Workgroups are 4 waves; the workgroup index is remapped so that the 8
-//    XCDs (which receive workgroups round-robin) each get a contiguous range
-//    of tasks, keeping neighbouring block rows in one L2.
+//  * Workgroups are 4 waves.  The 8 XCDs receive workgroups round-robin; the
+//    block-row schedule remaps the workgroup index so that each XCD gets a
+//    contiguous range of tasks (neighbouring block rows in one L2), the
+//    tile-affinity schedule gives each XCD a subset of the N tiles (one
+//    column slice of B per L2); see wave_task.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -64,6 +66,42 @@ __device__ inline uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
   const uint32_t per = nblocks / nx;
   if (bid >= per * nx) return bid;
   return (bid % nx) * per + bid / nx;
+}
+
+// Work order of the one-shot kernels: wave task -> (item, N tile).
+//   kOrderPlain: workgroup w runs tasks 4w..4w+3, task = item * ntiles + tile
+//     (the waves of a workgroup take the N tiles of one item);
+//   kOrderXcd: as kOrderPlain after xcd_remap (contiguous range per XCD);
+//   kOrderTiles: XCD x (workgroups x, x+8, ...) runs only the N tiles
+//     t = x mod 8.  With ntiles | 8, 8/ntiles XCDs share a tile and take its
+//     item groups in turn; with 8 | ntiles, XCD x runs tiles x, x+8, ... one
+//     after the other.  The 4 waves of a workgroup take 4 consecutive items of
+//     one tile, so each L2 sees one column slice of B.
+enum : int32_t { kOrderPlain = 0, kOrderXcd = 1, kOrderTiles = 2 };
+
+__device__ inline bool wave_task(uint32_t bid, uint32_t nblocks, int wave, int64_t n_items, int32_t ntiles,
+                                 int32_t order, int64_t& item, int& nt) {
+  if (order == kOrderTiles) {
+    const uint32_t x = bid & 7, r = bid >> 3;
+    int64_t g;  // item group (kWaves consecutive items)
+    if (ntiles <= 8) {
+      const uint32_t per = 8u / (uint32_t)ntiles;
+      nt = (int)(x % (uint32_t)ntiles);
+      g = (int64_t)r * per + x / (uint32_t)ntiles;
+    } else {
+      const int64_t groups = (n_items + kWaves - 1) / kWaves;
+      nt = (int)(x + 8 * (r / groups));
+      g = r % groups;
+      if (nt >= ntiles) return false;
+    }
+    item = g * kWaves + wave;
+    return item < n_items;
+  }
+  const uint32_t wg = order == kOrderXcd ? xcd_remap(bid, nblocks) : bid;
+  const int64_t task = (int64_t)wg * kWaves + wave;
+  item = task / ntiles;
+  nt = (int)(task % ntiles);
+  return item < n_items;
 }
 
 // ---- element types: DT 0 = FP16, 1 = BF16; operands travel as raw words ---
@@ -186,14 +224,12 @@ __global__ __launch_bounds__(256) void k_spmm_dense(
     const Item* __restrict__ items, int64_t n_items, int32_t ntiles,
     const int32_t* __restrict__ colidx, const uint16_t* __restrict__ values,
     const uint16_t* __restrict__ B, int64_t ldb, int64_t N, void* __restrict__ C, int64_t ldc,
-    float* __restrict__ scratch, const int32_t* __restrict__ rowmap, int32_t remap) {
+    float* __restrict__ scratch, const int32_t* __restrict__ rowmap, int32_t order) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-  const uint32_t wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
-  const int64_t task = (int64_t)wg * kWaves + wave;
-  const int64_t item_id = task / ntiles;
-  if (item_id >= n_items) return;
-  const int nt = (int)(task % ntiles);
+  int64_t item_id;
+  int nt;
+  if (!wave_task(blockIdx.x, gridDim.x, wave, n_items, ntiles, order, item_id, nt)) return;
   const Item it = items[item_id];
   const int64_t n0 = (int64_t)nt * 16 * NSUB;
 
@@ -482,16 +518,14 @@ __device__ __forceinline__ void lds_tile(const Item& it, int64_t n0, const Opera
 
 template <int BK, int NSUB, int DT, bool OUT_F16>
 __global__ __launch_bounds__(256) void k_spmm_lds(const Item* __restrict__ items, int64_t n_items, int32_t ntiles,
-                                                  Operands op, int32_t remap) {
+                                                  Operands op, int32_t order) {
   constexpr int WB = lds_wave_bytes<NSUB, DT>();
   __shared__ __attribute__((aligned(16))) uint8_t lds_all[kWaves * WB];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-  const uint32_t wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
-  const int64_t task = (int64_t)wg * kWaves + wave;
-  const int64_t item_id = task / ntiles;
-  if (item_id >= n_items) return;
-  const int nt = (int)(task % ntiles);
+  int64_t item_id;
+  int nt;
+  if (!wave_task(blockIdx.x, gridDim.x, wave, n_items, ntiles, order, item_id, nt)) return;
   const Item it = items[item_id];
   lds_tile<BK, NSUB, DT, OUT_F16>(it, (int64_t)nt * 16 * NSUB, op, lds_all + wave * WB, lane);
 }
@@ -625,16 +659,14 @@ __device__ __forceinline__ void mixed_tile(const Item& it, int64_t n0, const Ope
 
 template <int NSUB, int DT, bool OUT_F16>
 __global__ __launch_bounds__(256) void k_spmm_mixed(const Item* __restrict__ items, int64_t n_items, int32_t ntiles,
-                                                    Operands op, Mixed24 mx, int32_t remap) {
+                                                    Operands op, Mixed24 mx, int32_t order) {
   constexpr int WB = mixed_wave_bytes<NSUB>();
   __shared__ __attribute__((aligned(16))) uint8_t lds_all[kWaves * WB];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint32_t wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
-  const int64_t task = (int64_t)wg * kWaves + wave;
-  const int64_t item_id = task / ntiles;
-  if (item_id >= n_items) return;
-  const int nt = (int)(task % ntiles);
+  int64_t item_id;
+  int nt;
+  if (!wave_task(blockIdx.x, gridDim.x, wave, n_items, ntiles, order, item_id, nt)) return;
   const Item it = items[item_id];
   mixed_tile<NSUB, DT, OUT_F16>(it, (int64_t)nt * 16 * NSUB, op, mx, lds_all + wave * WB, lane);
 }
@@ -958,7 +990,7 @@ struct smat_plan_t {
   smat_spmm_opts opts{};
   int32_t nsub = 4;
   bool mixed = false;
-  bool remap = true;  // XCD-contiguous task ranges (off for longest-first schedules)
+  int32_t order = kOrderXcd;  // work order of the one-shot kernels (kOrder*)
   int64_t n_items = 0, n_red = 0, n_slots = 0;
   int64_t n_dense = 0, n_sparse = 0;
   smat::DevBuf items, red, scratch, dense_ids, sparse_ids;
@@ -968,20 +1000,37 @@ namespace {
 
 inline int32_t ntiles_of(const smat_plan_t* P, int nsub) { return (int32_t)((P->N + 16 * nsub - 1) / (16 * nsub)); }
 
+// Work order for a launch with ntiles N tiles (tile affinity needs ntiles | 8
+// or 8 | ntiles; otherwise the plain order, which the sorted items keep
+// longest first).
+inline int32_t order_of(const smat_plan_t* P, int32_t ntiles) {
+  if (P->order == kOrderTiles && !(8 % ntiles == 0 || ntiles % 8 == 0)) return kOrderPlain;
+  return P->order;
+}
+
+// Workgroups of a one-shot launch in the given order.
+inline unsigned grid_of(const smat_plan_t* P, int32_t ntiles, int32_t order) {
+  if (order != kOrderTiles) return blocks_for(P->n_items * ntiles, kWaves);
+  const int64_t groups = (P->n_items + kWaves - 1) / kWaves;
+  const int64_t rounds = ntiles <= 8 ? (groups + 8 / ntiles - 1) / (8 / ntiles) : (int64_t)(ntiles / 8) * groups;
+  return (unsigned)(rounds < 1 ? 8 : 8 * rounds);
+}
+
 template <int BK, int NSUB, int BLAYOUT, int DT, bool OUT_F16>
 void launch_dense(const smat_plan_t* P, const Operands& op, hipStream_t s) {
   const int32_t ntiles = ntiles_of(P, NSUB);
-  hipLaunchKernelGGL((k_spmm_dense<BK, NSUB, BLAYOUT, DT, OUT_F16>), blocks_for(P->n_items * ntiles, kWaves),
-                     64 * kWaves, 0, s, P->items.as<Item>(), P->n_items, ntiles, op.colidx,
-                     static_cast<const uint16_t*>(op.a), static_cast<const uint16_t*>(op.b), op.ldb, op.N, op.c,
-                     op.ldc, op.scratch, op.rowmap, (int32_t)P->remap);
+  const int32_t order = order_of(P, ntiles);
+  hipLaunchKernelGGL((k_spmm_dense<BK, NSUB, BLAYOUT, DT, OUT_F16>), grid_of(P, ntiles, order), 64 * kWaves, 0, s,
+                     P->items.as<Item>(), P->n_items, ntiles, op.colidx, static_cast<const uint16_t*>(op.a),
+                     static_cast<const uint16_t*>(op.b), op.ldb, op.N, op.c, op.ldc, op.scratch, op.rowmap, order);
 }
 
 template <int BK, int NSUB, int DT, bool OUT_F16>
 void launch_lds(const smat_plan_t* P, const Operands& op, hipStream_t s) {
   const int32_t ntiles = ntiles_of(P, NSUB);
-  hipLaunchKernelGGL((k_spmm_lds<BK, NSUB, DT, OUT_F16>), blocks_for(P->n_items * ntiles, kWaves), 64 * kWaves,
-                     0, s, P->items.as<Item>(), P->n_items, ntiles, op, (int32_t)P->remap);
+  const int32_t order = order_of(P, ntiles);
+  hipLaunchKernelGGL((k_spmm_lds<BK, NSUB, DT, OUT_F16>), grid_of(P, ntiles, order), 64 * kWaves, 0, s,
+                     P->items.as<Item>(), P->n_items, ntiles, op, order);
 }
 
 template <int NSUB, int DT, bool OUT_F16>
@@ -989,8 +1038,9 @@ void launch_mixed(const smat_plan_t* P, const uint16_t* v24, const uint16_t* i24
                   hipStream_t s) {
   const int32_t ntiles = ntiles_of(P, NSUB);
   const Mixed24 mx{P->dense_ids.as<int32_t>(), P->sparse_ids.as<int32_t>(), v24, i24};
-  hipLaunchKernelGGL((k_spmm_mixed<NSUB, DT, OUT_F16>), blocks_for(P->n_items * ntiles, kWaves), 64 * kWaves, 0,
-                     s, P->items.as<Item>(), P->n_items, ntiles, op, mx, (int32_t)P->remap);
+  const int32_t order = order_of(P, ntiles);
+  hipLaunchKernelGGL((k_spmm_mixed<NSUB, DT, OUT_F16>), grid_of(P, ntiles, order), 64 * kWaves, 0, s,
+                     P->items.as<Item>(), P->n_items, ntiles, op, mx, order);
 }
 
 inline bool lds_aligned(const smat_plan_t* P, const Operands& op, int elems_per_chunk) {
@@ -1004,9 +1054,12 @@ void dispatch_nsub(const smat_plan_t* P, const Operands& op, hipStream_t s) {
     switch (P->nsub) {
       case 1: launch_lds<BK, 1, DT, OUT_F16>(P, op, s); return;
       case 2: launch_lds<BK, 2, DT, OUT_F16>(P, op, s); return;
-      default: launch_lds<BK, 4, DT, OUT_F16>(P, op, s); return;
+      case 4: launch_lds<BK, 4, DT, OUT_F16>(P, op, s); return;
+      default: launch_lds<BK, 8, DT, OUT_F16>(P, op, s); return;
     }
   }
+  // (B not stageable at this call: 128-column plans run 64-column tiles; the
+  // items do not depend on the tile width)
   switch (P->nsub) {
     case 1: launch_dense<BK, 1, BLAYOUT, DT, OUT_F16>(P, op, s); break;
     case 2: launch_dense<BK, 2, BLAYOUT, DT, OUT_F16>(P, op, s); break;
@@ -1130,7 +1183,7 @@ smat_status build_items(smat_plan_t* P, const int32_t* rp1, int32_t step1, const
                      off_i.as<int32_t>(), off_s.as<int32_t>(), off_f.as<int32_t>(), P->items.as<Item>(),
                      P->red.as<Reduce>());
   SMAT_LAUNCH_CHECK();
-  if (!P->remap && P->n_items > 1) {
+  if (P->order != kOrderXcd && P->n_items > 1) {
     // longest first: items sorted by their MFMA steps, descending (stable)
     const int64_t n = P->n_items;
     smat::DevBuf k0, k1, i0, i1, sorted, tmp2;
@@ -1220,19 +1273,30 @@ smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat
   if (o.kernel == SMAT_KERNEL_MIXED24)
     SMAT_REQUIRE(A->bk == 16 && o.b_layout == SMAT_B_ROW_MAJOR,
                  "smat_spmm_plan_create: the 2:4 kernel needs bk = 16 and row-major B");
-  SMAT_REQUIRE(o.n_tile == 0 || o.n_tile == 16 || o.n_tile == 32 || o.n_tile == 64,
-               "smat_spmm_plan_create: n_tile must be 0, 16, 32 or 64");
+  SMAT_REQUIRE(o.n_tile == 0 || o.n_tile == 16 || o.n_tile == 32 || o.n_tile == 64 || o.n_tile == 128,
+               "smat_spmm_plan_create: n_tile must be 0, 16, 32, 64 or 128");
+  if (o.n_tile == 128)
+    SMAT_REQUIRE(o.kernel != SMAT_KERNEL_MIXED24 && (o.in_dtype == SMAT_F16 || o.in_dtype == SMAT_BF16) &&
+                     o.b_layout == SMAT_B_ROW_MAJOR && o.b_staging != SMAT_BSTAGE_DIRECT,
+                 "smat_spmm_plan_create: n_tile 128 is for the dense kernel with FP16 / BF16 row-major B "
+                 "staged through LDS");
   SMAT_REQUIRE(o.b_staging >= SMAT_BSTAGE_AUTO && o.b_staging <= SMAT_BSTAGE_LDS,
                "smat_spmm_plan_create: unknown b_staging %d", o.b_staging);
-  SMAT_REQUIRE(o.schedule == SMAT_SCHED_AUTO || o.schedule == SMAT_SCHED_BLOCK_ROWS ||
-                   o.schedule == SMAT_SCHED_LONGEST_FIRST,
+  SMAT_REQUIRE(o.schedule >= SMAT_SCHED_AUTO && o.schedule <= SMAT_SCHED_TILE_AFFINITY,
                "smat_spmm_plan_create: unknown schedule %d", o.schedule);
   hipStream_t s = smat::to_hip(stream);
+  int nt = o.n_tile ? o.n_tile : (N >= 64 ? 64 : (N > 16 ? 32 : 16));
+  if (o.in_dtype == SMAT_FP8_E4M3 && nt < 32) nt = 32;  // FP8 tiles are at least 32 columns wide
   if (o.schedule == SMAT_SCHED_AUTO) {
-    // skewed block rows (the longest over 8x the mean) go longest first
+    // skewed block rows (the longest over 8x the mean) go longest first,
+    // with tile affinity when every N tile is shared by 2+ XCDs (with more
+    // tiles each XCD would re-read all of A for its one slice of B)
     int64_t mx = 0, tot = 0;
     SMAT_TRY(row_length_stats(A, &mx, &tot, s));
-    o.schedule = (tot > 0 && mx * A->nbrows > 8 * tot) ? SMAT_SCHED_LONGEST_FIRST : SMAT_SCHED_BLOCK_ROWS;
+    const int64_t ntl = (N + nt - 1) / nt;
+    o.schedule = !(tot > 0 && mx * A->nbrows > 8 * tot) ? SMAT_SCHED_BLOCK_ROWS
+                 : (ntl == 2 || ntl == 4)              ? SMAT_SCHED_TILE_AFFINITY
+                                                       : SMAT_SCHED_LONGEST_FIRST;
   }
 
   smat_plan_t* P = new smat_plan_t();
@@ -1240,9 +1304,8 @@ smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat
   P->N = N;
   P->opts = o;
   P->mixed = o.kernel == SMAT_KERNEL_MIXED24;
-  P->remap = o.schedule != SMAT_SCHED_LONGEST_FIRST;
-  int nt = o.n_tile ? o.n_tile : (N >= 64 ? 64 : (N > 16 ? 32 : 16));
-  if (o.in_dtype == SMAT_FP8_E4M3 && nt < 32) nt = 32;  // FP8 tiles are at least 32 columns wide
+  P->order = o.schedule == SMAT_SCHED_BLOCK_ROWS ? kOrderXcd
+             : o.schedule == SMAT_SCHED_TILE_AFFINITY ? kOrderTiles : kOrderPlain;
   P->nsub = nt / 16;
   const int32_t step = 32 / A->bk;  // blocks per MFMA step
   int32_t S = o.max_blocks_per_item;  // in blocks
@@ -1250,7 +1313,7 @@ smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat
     const int64_t ntl = (N + nt - 1) / nt;
     const int64_t s_min = 12 * (int64_t)step;  // finer slices only add partial tiles to reduce
     int64_t s_auto;
-    if (P->remap) {
+    if (P->order == kOrderXcd) {
       // block-row order: aim for >= 64k wave tasks so the tail stays short
       const int64_t target_items = (65536 + ntl - 1) / ntl;
       s_auto = (A->nnzb + target_items - 1) / target_items;
@@ -1441,7 +1504,7 @@ smat_status smat_batch_create(int32_t count, const smat_plan* plans, const smat_
   std::vector<BatchMat> mats((size_t)count);
   std::vector<BatchRed> reds;
   const int dt = plans[0] ? plans[0]->opts.in_dtype : 0;
-  // matrices ordered by N-tile width (16, 32, 64), input order inside a group
+  // matrices ordered by N-tile width (16, 32, 64 and wider), input order inside a group
   std::vector<int32_t> order;
   int32_t g_cnt[3] = {0, 0, 0};
   for (int g = 0; g < 3; g++)
@@ -1475,7 +1538,7 @@ smat_status smat_batch_create(int32_t count, const smat_plan* plans, const smat_
     SMAT_REQUIRE(lds_aligned(P, m.op, 8), "smat_batch_create: entry %d: N and ldb must be multiples of 8 and B "
                                           "16-byte aligned", i);
     m.items = P->items.as<Item>();
-    m.ntiles = ntiles_of(P, P->nsub);
+    m.ntiles = ntiles_of(P, g == 0 ? 1 : g == 1 ? 2 : 4);  // 128-column plans run as 64-column tiles here
     m.kind = P->mixed ? 6 : bk_code(P->A.bk);
     m.mx = P->mixed ? Mixed24{P->dense_ids.as<int32_t>(), P->sparse_ids.as<int32_t>(), o.values24,
                               static_cast<const uint16_t*>(o.index24)}

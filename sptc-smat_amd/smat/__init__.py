@@ -59,7 +59,7 @@ class SpmmOpts(ctypes.Structure):
 
 
 SMAT_BSTAGE_AUTO, SMAT_BSTAGE_DIRECT, SMAT_BSTAGE_LDS = 0, 1, 2
-SMAT_SCHED_AUTO, SMAT_SCHED_BLOCK_ROWS, SMAT_SCHED_LONGEST_FIRST = 0, 1, 2
+SMAT_SCHED_AUTO, SMAT_SCHED_BLOCK_ROWS, SMAT_SCHED_LONGEST_FIRST, SMAT_SCHED_TILE_AFFINITY = 0, 1, 2, 3
 
 
 class MatrixStats(ctypes.Structure):

@@ -36,7 +36,8 @@ from . import gen
 
 NF = 26                      # SMAT_TUNER_NF
 BKS = (1, 2, 4, 8, 16, 32)
-NTILES = (16, 32, 64)
+NTILES = (16, 32, 64, 128)
+NTILES_24 = (16, 32, 64)     # tile widths of the mixed dense / 2:4 kernel
 TAUS = (None, 0.5)           # None: no reordering; else Keeper threshold (16-row groups, 16-column blocks)
 REORDER_MAX_ROWS = 16384     # the exact Keeper's host replay grows ~quadratically; larger matrices skip it
 HIDDEN = 64
@@ -60,7 +61,7 @@ def configs():
         for bk in BKS:
             for nt in NTILES:
                 out.append((tau, bk, nt, "dense"))
-        for nt in NTILES:
+        for nt in NTILES_24:
             out.append((tau, 16, nt, "mixed24"))
     return out
 
@@ -188,7 +189,7 @@ def sweep(M, K, rp, ci, v, N, B=None, reps=5, taus=TAUS, log=None):
                 v24 = i24 = None
                 if kernel == "mixed24":
                     v24, i24 = A.compress_24(SMAT_24_SMFMAC)
-                for nt in NTILES:
+                for nt in NTILES_24 if kernel == "mixed24" else NTILES:
                     plan = SpmmPlan(A, N, n_tile=nt,
                                     kernel=SMAT_KERNEL_MIXED24 if kernel == "mixed24" else SMAT_KERNEL_DENSE)
                     t = time_call(lambda: plan.execute(B, C, values24=v24, index24=i24, c_rowmap=rowmap), reps=reps)

@@ -195,7 +195,7 @@ def test_invalid_arguments_raise():
         smat.SpmmPlan(A, 64, n_tile=48)
 
 
-@pytest.mark.parametrize("schedule", [0, 1, 2])
+@pytest.mark.parametrize("schedule", [0, 1, 2, 3])
 @pytest.mark.parametrize("which", ["rmat", "band"])
 def test_schedules_match_oracle(schedule, which):
     """Block-row order, longest first and the automatic choice all give the
@@ -221,3 +221,93 @@ def test_schedules_match_oracle(schedule, which):
             assert np.linalg.norm(Cn - R) / np.linalg.norm(R) < 1e-5
             plan.close()
             A.close()
+
+
+@pytest.mark.parametrize("N,n_tile", [(32, 32), (64, 16), (128, 16), (256, 16), (192, 64), (256, 32)])
+@pytest.mark.parametrize("kernel", ["lds", "direct", "mixed24"])
+def test_tile_affinity_covers_every_tile(N, n_tile, kernel):
+    """Tile-affinity order (each XCD a subset of the N tiles) for tile counts
+    1, 4, 8, 16 (affinity) and 3 (falls back to the plain order), on the LDS,
+    direct-B and mixed dense/2:4 kernels: the FP64 oracle's product, every C
+    element written."""
+    _need_gpu()
+    import smat
+    M, K, rp, ci, v = gen.rmat(scale=11, edge_factor=10, seed=31)
+    B = gen.dense_b(K, N, seed=9)
+    R = of.spmm_csr_f64(M, rp, ci, v, B)
+    print(f"tile affinity N {N} n_tile {n_tile} {kernel}", flush=True)
+    rule = smat.SMAT_TAG_EXACT24 if kernel == "mixed24" else smat.SMAT_TAG_REFERENCE
+    A = smat.Bcsr(M, K, _dev(rp), _dev(ci), _dev(v.view(np.int16)), bk=16 if kernel == "mixed24" else 2,
+                  tag_rule=rule)
+    v24 = i24 = None
+    if kernel == "mixed24":
+        v24, i24 = A.compress_24(smat.SMAT_24_SMFMAC)
+    stage = smat.SMAT_BSTAGE_DIRECT if kernel == "direct" else smat.SMAT_BSTAGE_AUTO
+    plan = smat.SpmmPlan(A, N, n_tile=n_tile, b_staging=stage, schedule=smat.SMAT_SCHED_TILE_AFFINITY,
+                         kernel=smat.SMAT_KERNEL_MIXED24 if kernel == "mixed24" else smat.SMAT_KERNEL_DENSE)
+    C = torch.full((M, N), float("nan"), dtype=torch.float32, device="cuda")
+    plan(_dev(B.view(np.int16)), C, values24=v24, index24=i24)
+    torch.cuda.synchronize()
+    Cn = C.cpu().numpy().astype(np.float64)
+    assert np.all(np.isfinite(Cn))
+    assert np.linalg.norm(Cn - R) / np.linalg.norm(R) < 1e-5
+    plan.close()
+    A.close()
+
+
+@pytest.mark.parametrize("bk", [1, 2, 4, 8, 16, 32])
+@pytest.mark.parametrize("N", [128, 200, 384])
+@pytest.mark.parametrize("schedule", [1, 3])
+def test_n_tile_128_bit_identical_to_64(bk, N, schedule):
+    """128-column tiles run the same MFMA sequence per 16-column sub-tile as
+    64-column tiles (same items, same step order), so C is bit-identical;
+    and both are the FP64 oracle's product.  N = 200 leaves a partial tile;
+    384 gives 3 tiles (tile affinity falls back to the plain order)."""
+    _need_gpu()
+    import smat
+    M, K, rp, ci, v = gen.rmat(scale=11, edge_factor=10, seed=41)
+    B = gen.dense_b(K, N, seed=10)
+    R = of.spmm_csr_f64(M, rp, ci, v, B)
+    print(f"n_tile 128 bk {bk} N {N} schedule {schedule}", flush=True)
+    A = _bcsr(M, K, rp, ci, v, bk=bk)
+    out = []
+    for nt in (64, 128):
+        plan = smat.SpmmPlan(A, N, n_tile=nt, schedule=schedule, max_blocks_per_item=8 * (32 // bk))
+        C = torch.full((M, N), float("nan"), dtype=torch.float32, device="cuda")
+        plan(_dev(B.view(np.int16)), C)
+        torch.cuda.synchronize()
+        out.append(C.cpu().numpy())
+        plan.close()
+    assert np.array_equal(out[0], out[1])
+    assert _rel_fro(out[1].astype(np.float64), R) < 1e-5
+    A.close()
+
+
+def test_n_tile_128_unaligned_b_and_restrictions():
+    """B that cannot be staged through LDS at call time (16-byte misaligned)
+    runs 64-column direct tiles with the same result; plans that cannot use
+    128-column tiles at all are refused."""
+    _need_gpu()
+    import smat
+    M, K, rp, ci, v = gen.rmat(scale=10, edge_factor=8, seed=42)
+    N = 256
+    B = gen.dense_b(K, N, seed=11)
+    R = of.spmm_csr_f64(M, rp, ci, v, B)
+    A = _bcsr(M, K, rp, ci, v, bk=4)
+    plan = smat.SpmmPlan(A, N, n_tile=128)
+    Bd = torch.zeros(K * N + 1, dtype=torch.int16, device="cuda")
+    Bd[1:] = _dev(B.view(np.int16)).reshape(-1)
+    C = torch.full((M, N), float("nan"), dtype=torch.float32, device="cuda")
+    plan(Bd[1:], C, ldb=N)
+    torch.cuda.synchronize()
+    assert _rel_fro(C.cpu().numpy().astype(np.float64), R) < 1e-5
+    plan.close()
+    for kw in (dict(b_staging=smat.SMAT_BSTAGE_DIRECT), dict(in_dtype=smat.SMAT_FP8_E4M3),
+               dict(b_layout=smat.SMAT_B_REFERENCE)):
+        with pytest.raises(smat.SmatError):
+            smat.SpmmPlan(A, N, n_tile=128, **kw)
+    A.close()
+    A16 = _bcsr(M, K, rp, ci, v, bk=16, tag_rule=smat.SMAT_TAG_EXACT24)
+    with pytest.raises(smat.SmatError):
+        smat.SpmmPlan(A16, N, n_tile=128, kernel=smat.SMAT_KERNEL_MIXED24)
+    A16.close()

@@ -203,6 +203,21 @@ smat_status smat_fixed_size_grouping(int64_t rows, const int64_t* grouping_host,
 smat_status smat_grouping_to_permutation(int64_t rows, const int64_t* grouping_host, int64_t* perm_host);
 
 /*
+ * VBR blocking statistics of a row grouping on the GPU: replaces
+ * BlockingEngine::CollectBlockingInfo (preprocess/src/general/blocking.cpp:
+ * 562-614) with get_partition (utilities.cpp:23-43).  Rows with equal
+ * grouping values form one block row; out_host[0] = VBR_nzcount,
+ * out_host[1] = VBR_nzblocks_count, out_host[2] = VBR_longest_row;
+ * *avg_height_host = VBR_average_height (NaN when there are no blocks, as
+ * the reference's 0/0).  rowptr/colidx/grouping are device arrays; grouping
+ * values must lie in [0, 2 * rows) (as GetGrouping and
+ * smat_fixed_size_grouping produce).  Synchronous.
+ */
+smat_status smat_blocking_info(int64_t rows, int64_t cols, int64_t nnz, const int32_t* rowptr, const int32_t* colidx,
+                               const int64_t* grouping, int64_t col_block, int64_t* out_host, float* avg_height_host,
+                               smat_stream stream);
+
+/*
  * CSR::reorder / permute_rows (csr.cpp:65-74, 97-105) on the GPU: row r of
  * the output is row perm[r] of the input (perm: device int64[rows]); values
  * may be NULL (pattern only).  Unlike the reference pipeline, which writes

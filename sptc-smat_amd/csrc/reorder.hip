@@ -365,6 +365,82 @@ struct PoolAlloc {
   bool operator!=(const PoolAlloc<U>& o) const { return pool != o.pool; }
 };
 
+// ---- VBR blocking statistics (BlockingEngine::CollectBlockingInfo) --------
+// A block row is the set of rows sharing one grouping value g; its height
+// h[g] comes from a histogram over the values (which lie in [0, 2 rows) for
+// GetGrouping and get_fixed_size_grouping output).  Its nonzero blocks are
+// the distinct keys g * nbc + column block among its entries: one radix sort
+// of the per-entry keys, then the heads of equal runs.  Every figure is an
+// integer sum or maximum, so the result does not depend on execution order.
+__global__ void k_group_heights(int64_t rows, const int64_t* __restrict__ grouping, int32_t* __restrict__ h,
+                                int32_t* __restrict__ bad) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const int64_t g = grouping[r];
+  if (g < 0 || g >= 2 * rows) {
+    atomicOr(bad, 1);
+    return;
+  }
+  atomicAdd(&h[g], 1);
+}
+
+__global__ void k_block_keys(int64_t rows, int64_t nnz, int64_t cols, const int32_t* __restrict__ rowptr,
+                             const int32_t* __restrict__ colidx, const int64_t* __restrict__ grouping, int64_t nbc,
+                             int64_t col_block, uint64_t* __restrict__ keys, int32_t* __restrict__ bad) {
+  const int64_t nz = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (nz >= nnz) return;
+  // row of entry nz: the last r with rowptr[r] <= nz (rowptr[0] = 0 <= nz < rowptr[rows] = nnz)
+  int64_t lo = 0, hi = rows;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (rowptr[mid] <= nz) lo = mid;
+    else hi = mid;
+  }
+  const int64_t c = colidx[nz];
+  const int64_t g = grouping[lo];
+  if (c < 0 || c >= cols) atomicOr(bad, 2);
+  const bool ok = c >= 0 && c < cols && g >= 0 && g < 2 * rows;
+  keys[nz] = ok ? (uint64_t)g * (uint64_t)nbc + (uint64_t)(c / col_block) : ~0ull;  // invalid keys sort last
+}
+
+__device__ inline int64_t wave_sum(int64_t v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// sums: [0] VBR_nzcount, [1] VBR_nzblocks_count, [2] total block height
+__global__ void k_vbr_heads(int64_t n, const uint64_t* __restrict__ keys, int64_t nbc, int64_t col_block,
+                            int64_t cols, int64_t rows, const int32_t* __restrict__ h, int32_t* __restrict__ cntb,
+                            unsigned long long* __restrict__ sums) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t nzc = 0, nzb = 0, th = 0;
+  if (i < n && keys[i] != ~0ull && (i == 0 || keys[i] != keys[i - 1])) {
+    const int64_t g = (int64_t)(keys[i] / (uint64_t)nbc);
+    const int64_t cb = (int64_t)(keys[i] % (uint64_t)nbc);
+    const int64_t hh = h[g];
+    nzb = 1;
+    th = hh;
+    nzc = col_block * hh;
+    if (cb == nbc - 1 && cols % col_block != 0) nzc -= hh * (col_block - cols % col_block);  // short last block
+    atomicAdd(&cntb[g], 1);
+  }
+  nzc = wave_sum(nzc);
+  nzb = wave_sum(nzb);
+  th = wave_sum(th);
+  if ((threadIdx.x & 63) == 0 && (nzb | nzc | th) != 0) {
+    atomicAdd(&sums[0], (unsigned long long)nzc);
+    atomicAdd(&sums[1], (unsigned long long)nzb);
+    atomicAdd(&sums[2], (unsigned long long)th);
+  }
+}
+
+__global__ void k_max_i32(int64_t n, const int32_t* __restrict__ v, unsigned int* __restrict__ out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int32_t m = i < n ? v[i] : 0;
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_down(m, o, 64));
+  if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(out, (unsigned int)m);
+}
+
 }  // namespace
 
 extern "C" {
@@ -660,6 +736,67 @@ smat_status smat_permute_dense_rows(int64_t K, int64_t N, int32_t elem_bytes, co
                        static_cast<uint8_t*>(out) + r0 * ld_out * elem_bytes, ld_out * elem_bytes);
   }
   SMAT_LAUNCH_CHECK();
+  return SMAT_OK;
+}
+
+// BlockingEngine::CollectBlockingInfo (blocking.cpp:562-614) with the row
+// partition of get_partition (utilities.cpp:23-43): out[0] = VBR_nzcount,
+// out[1] = VBR_nzblocks_count, out[2] = VBR_longest_row; *avg_height =
+// VBR_average_height, (float)total height / (float)blocks as the reference
+// divides (NaN with no blocks).  Column blocks are counted with the exact
+// ceil(cols / col_block); the reference's float ceil agrees below 2^24 columns.
+smat_status smat_blocking_info(int64_t rows, int64_t cols, int64_t nnz, const int32_t* rowptr, const int32_t* colidx,
+                               const int64_t* grouping, int64_t col_block, int64_t* out_host, float* avg_height_host,
+                               smat_stream stream) {
+  SMAT_REQUIRE(rows > 0 && cols > 0 && nnz >= 0 && col_block > 0 && rowptr && grouping && out_host &&
+                   avg_height_host && (nnz == 0 || colidx),
+               "smat_blocking_info: bad argument");
+  SMAT_REQUIRE(2 * rows < (int64_t)INT32_MAX && nnz < (int64_t)INT32_MAX, "smat_blocking_info: too large");
+  const int64_t nbc = (cols + col_block - 1) / col_block;
+  SMAT_REQUIRE(nbc <= (int64_t)(~0ull >> 2) / (2 * rows), "smat_blocking_info: rows x column blocks too large");
+  hipStream_t s = smat::to_hip(stream);
+  const int64_t ng = 2 * rows;
+  smat::DevBuf h, cntb, flags, sums, keys, keys2, tmp;
+  SMAT_TRY(h.alloc(ng * 4));
+  SMAT_TRY(cntb.alloc(ng * 4));
+  SMAT_TRY(flags.alloc(64));
+  SMAT_TRY(sums.alloc(64));
+  SMAT_HIP_TRY(hipMemsetAsync(h.p, 0, ng * 4, s));
+  SMAT_HIP_TRY(hipMemsetAsync(cntb.p, 0, ng * 4, s));
+  SMAT_HIP_TRY(hipMemsetAsync(flags.p, 0, 64, s));
+  SMAT_HIP_TRY(hipMemsetAsync(sums.p, 0, 64, s));
+  hipLaunchKernelGGL(k_group_heights, nblk(rows, 256), 256, 0, s, rows, grouping, h.as<int32_t>(),
+                     flags.as<int32_t>());
+  SMAT_LAUNCH_CHECK();
+  if (nnz > 0) {
+    SMAT_TRY(keys.alloc(nnz * 8));
+    SMAT_TRY(keys2.alloc(nnz * 8));
+    hipLaunchKernelGGL(k_block_keys, nblk(nnz, 256), 256, 0, s, rows, nnz, cols, rowptr, colidx, grouping, nbc,
+                       col_block, keys.as<uint64_t>(), flags.as<int32_t>());
+    SMAT_LAUNCH_CHECK();
+    size_t tb = 0;
+    SMAT_HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, keys.as<uint64_t>(), keys2.as<uint64_t>(), (int)nnz,
+                                                   0, 64, s));
+    SMAT_TRY(tmp.alloc(tb));
+    SMAT_HIP_TRY(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, keys.as<uint64_t>(), keys2.as<uint64_t>(), (int)nnz, 0,
+                                                   64, s));
+    hipLaunchKernelGGL(k_vbr_heads, nblk(nnz, 256), 256, 0, s, nnz, keys2.as<uint64_t>(), nbc, col_block, cols,
+                       rows, h.as<int32_t>(), cntb.as<int32_t>(), sums.as<unsigned long long>());
+    hipLaunchKernelGGL(k_max_i32, nblk(ng, 256), 256, 0, s, ng, cntb.as<int32_t>(),
+                       reinterpret_cast<unsigned int*>(sums.as<unsigned long long>() + 3));
+    SMAT_LAUNCH_CHECK();
+  }
+  unsigned long long hs[4] = {0, 0, 0, 0};
+  int32_t hf = 0;
+  SMAT_HIP_TRY(hipMemcpyAsync(hs, sums.p, sizeof hs, hipMemcpyDeviceToHost, s));
+  SMAT_HIP_TRY(hipMemcpyAsync(&hf, flags.p, 4, hipMemcpyDeviceToHost, s));
+  SMAT_HIP_TRY(hipStreamSynchronize(s));
+  SMAT_REQUIRE(!(hf & 1), "smat_blocking_info: grouping value outside [0, 2 * rows)");
+  SMAT_REQUIRE(!(hf & 2), "smat_blocking_info: column index outside [0, cols)");
+  out_host[0] = (int64_t)hs[0];
+  out_host[1] = (int64_t)hs[1];
+  out_host[2] = (int64_t)(hs[3] & 0xffffffffull);
+  *avg_height_host = (float)(int64_t)hs[2] / (float)(int64_t)hs[1];
   return SMAT_OK;
 }
 

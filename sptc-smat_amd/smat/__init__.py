@@ -142,6 +142,7 @@ SIGNATURES = {
     "smat_reorder_grouping": (_i32, [_i64, _p, _p, _p, _p, ctypes.c_float, _i64, _i64, _p, _p, _p]),
     "smat_fixed_size_grouping": (_i32, [_i64, _p, _i64, _p]),
     "smat_grouping_to_permutation": (_i32, [_i64, _p, _p]),
+    "smat_blocking_info": (_i32, [_i64, _i64, _i64, _p, _p, _p, _i64, _p, _p, _p]),
     "smat_permute_csr_rows": (_i32, [_i64, _p, _p, _p, _p, _p, _p, _p, _p]),
     "mmaOBTKernel": (None, [_p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
     "mmaOBTSKernel": (None, [_p, _p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
@@ -402,6 +403,20 @@ def reorder_columns(rowptr, colidx, cols: int, tau: float, col_block: int, max_r
 REFERENCE_TAUS = (0.9, 0.8, 0.6, 0.5, 0.4, 0.3, 0.2, 0.1, 0.01)
 
 
+def blocking_info(rows: int, cols: int, d_rowptr, d_colidx, d_grouping, col_block: int, stream=None):
+    """VBR statistics of a row grouping on the GPU (BlockingEngine::
+    CollectBlockingInfo, preprocess/src/general/blocking.cpp:562-614): device
+    int32 rowptr / colidx and int64 grouping (torch tensors); returns
+    ([VBR_nzcount, VBR_nzblocks_count, VBR_longest_row], VBR_average_height)."""
+    import numpy as np
+    out = np.zeros(3, np.int64)
+    avg = ctypes.c_float(0.0)
+    nnz = int(d_colidx.numel()) if d_colidx is not None else 0
+    check(lib().smat_blocking_info(rows, cols, nnz, ptr(d_rowptr), ptr(d_colidx), ptr(d_grouping), col_block,
+                                   out.ctypes.data, ctypes.byref(avg), stream_ptr(stream)))
+    return out, float(avg.value)
+
+
 def threshold_search(rowptr, colidx, cols: int, col_block: int = 16, row_block: int = 16, taus=REFERENCE_TAUS):
     """The reference's exhaustive threshold search
     (preprocess/test/general/reorder_matrix.cpp:17-33): for each tau, Keeper
@@ -412,17 +427,18 @@ def threshold_search(rowptr, colidx, cols: int, col_block: int = 16, row_block: 
     (`<=`), starting from the nonzero count.  Returns (best_tau, best_blocks,
     {tau: blocks})."""
     import numpy as np
+    import torch
     rp = np.ascontiguousarray(rowptr, np.int32)
     ci = np.ascontiguousarray(colidx, np.int32)
     rows = rp.size - 1
-    row_of = np.repeat(np.arange(rows, dtype=np.int64), np.diff(rp))
-    nbc = (cols + col_block - 1) // col_block
+    d_rp = torch.from_numpy(rp).cuda()
+    d_ci = torch.from_numpy(ci).cuda() if ci.size else None
     best_tau, best = -1.0, int(ci.size)
     counts = {}
     for tau in taus:
         g, _ = reorder_grouping(rp, ci, np.float32(tau), col_block, row_block)
-        fixed = fixed_size_grouping(g, row_block)
-        blocks = int(np.unique(fixed[row_of] * nbc + ci // col_block).size)
+        fixed = torch.from_numpy(fixed_size_grouping(g, row_block)).cuda()
+        blocks = int(blocking_info(rows, cols, d_rp, d_ci, fixed, col_block)[0][1])
         counts[tau] = blocks
         if blocks <= best:
             best_tau, best = tau, blocks

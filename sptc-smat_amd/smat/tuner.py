@@ -103,15 +103,18 @@ def features(st: MatrixStats, N: int) -> np.ndarray:
 
 
 # ---- synthetic matrices (the batched workload's distribution) ------------------
-def make_matrix(seed: int):
+def make_matrix(seed: int, large: bool = False):
     """Seeded matrix of the batched workload: n log-uniform in [2k, 200k], nnz
     log-uniform in [1e4, 5e6] (clamped by what the generator can place), the
-    generator cycling over band, R-MAT, block-clustered and power-law.
-    Returns (kind, (M, K, rowptr, colidx, FP16 bits))."""
+    generator cycling over band, R-MAT, block-clustered and power-law.  With
+    large=True: n in [100k, 600k], nnz in [1e6, 1.2e7] (single-matrix sizes
+    such as R-MAT scale 17-19).  Returns (kind, (M, K, rowptr, colidx, FP16
+    bits))."""
     rng = np.random.default_rng(seed)
     kind = ("band", "rmat", "clustered", "powerlaw")[seed % 4]
-    n = int(math.exp(rng.uniform(math.log(2000), math.log(200000))))
-    nnz = math.exp(rng.uniform(math.log(1e4), math.log(5e6)))
+    n_lo, n_hi, z_lo, z_hi = (1e5, 6e5, 1e6, 1.2e7) if large else (2e3, 2e5, 1e4, 5e6)
+    n = int(math.exp(rng.uniform(math.log(n_lo), math.log(n_hi))))
+    nnz = math.exp(rng.uniform(math.log(z_lo), math.log(z_hi)))
     nnz = min(nnz, 64.0 * n)
     gs = int(rng.integers(0, 2 ** 31))
     if kind == "band":
@@ -378,7 +381,12 @@ def _gen_worker(seed):
     return seed, kind, M, K, rp, ci, v
 
 
-def collect(first, count, out, N=128, workers=8, log=print, n_list=None):
+def _gen_worker_large(seed):
+    kind, (M, K, rp, ci, v) = make_matrix(seed, large=True)
+    return seed, kind, M, K, rp, ci, v
+
+
+def collect(first, count, out, N=128, workers=8, log=print, n_list=None, large=False):
     import multiprocessing as mp
 
     import torch
@@ -387,7 +395,8 @@ def collect(first, count, out, N=128, workers=8, log=print, n_list=None):
     t0 = time.time()
     ctx = mp.get_context("fork")
     with ctx.Pool(workers) as pool:
-        for k, (seed, kind, M, K, rp, ci, v) in enumerate(pool.imap(_gen_worker, range(first, first + count))):
+        for k, (seed, kind, M, K, rp, ci, v) in enumerate(pool.imap(_gen_worker_large if large else _gen_worker,
+                                                                                   range(first, first + count))):
             n_cols = n_list[seed % len(n_list)] if n_list else N
             d_rp, d_ci = torch.from_numpy(rp).cuda(), torch.from_numpy(ci).cuda()
             d_v = torch.from_numpy(v.view(np.int16)).cuda()
@@ -424,6 +433,7 @@ def main(argv=None):
     c.add_argument("--n", type=int, default=128)
     c.add_argument("--n-list", default="", help="comma-separated N values cycled over the seeds")
     c.add_argument("--workers", type=int, default=8)
+    c.add_argument("--large", action="store_true", help="single-matrix sizes (n 100k-600k, nnz 1e6-1.2e7)")
     t = sub.add_parser("train")
     t.add_argument("--data", nargs="+", required=True)
     t.add_argument("--out", required=True)
@@ -435,7 +445,7 @@ def main(argv=None):
     log = lambda *x: print(*x, flush=True)  # noqa: E731
     if a.cmd == "collect":
         n_list = [int(x) for x in a.n_list.split(",")] if a.n_list else None
-        collect(a.first, a.count, a.out, N=a.n, workers=a.workers, log=log, n_list=n_list)
+        collect(a.first, a.count, a.out, N=a.n, workers=a.workers, log=log, n_list=n_list, large=a.large)
     elif a.cmd == "train":
         fs = [np.load(p) for p in a.data]
         feats = np.concatenate([f["feats"] for f in fs])

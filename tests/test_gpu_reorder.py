@@ -169,3 +169,85 @@ def test_keeper_long_rows_vs_oracle(tau, cb):
     eg, ecnt = of.keeper(rp.astype(np.int64), ci.astype(np.int64), np.float32(tau), cb, 16)
     assert np.array_equal(cnt, ecnt)
     assert np.array_equal(g, eg)
+
+
+def test_blocking_info_golden_and_kat(golden_dir):
+    """GPU CollectBlockingInfo against the compiled reference's figures: the
+    60 golden Keeper cases (fixed-size groupings) and the test-matrix KAT
+    (raw algorithm-0 grouping, 3-column blocks over 10 columns)."""
+    _need_gpu()
+    import json
+    import smat
+    d = np.load(os.path.join(golden_dir, "keeper_golden.npz"))
+    for key in [str(c) for c in d["cases"]]:
+        name, params = key.split("__")
+        cb = int(params.split("_")[1][1:])
+        rp, ci = d[f"{name}__rowptr"].astype(np.int32), d[f"{name}__colidx"].astype(np.int32)
+        cols = int(d[f"{name}__cols"][0])
+        fixed = torch.from_numpy(d[key + "__fixed"].astype(np.int64)).cuda()
+        info, avg = smat.blocking_info(rp.size - 1, cols, torch.from_numpy(rp).cuda(), torch.from_numpy(ci).cuda(),
+                                       fixed, cb)
+        exp = d[key + "__info"]
+        assert info.tolist() == exp[:3].tolist(), key
+        assert round(avg * 1e6) == exp[3], key
+    kat = json.load(open(os.path.join(golden_dir, "kat_test_matrix.json")))
+    e = np.array(kat["edges"])
+    rows = int(e[:, 0].max()) + 1
+    rp = np.zeros(rows + 1, np.int64)
+    np.add.at(rp, e[:, 0].astype(np.int64) + 1, 1)
+    rp = np.cumsum(rp).astype(np.int32)
+    ci = e[:, 1].astype(np.int32)
+    g = torch.tensor(kat["compiled_reference"]["grouping"], dtype=torch.int64).cuda()
+    info, avg = smat.blocking_info(rows, kat["cols"], torch.from_numpy(rp).cuda(), torch.from_numpy(ci).cuda(), g, 3)
+    assert info.tolist() == [kat["VBR_nzcount"], kat["VBR_nzblocks_count"], kat["VBR_longest_row"]]
+    assert avg == pytest.approx(kat["VBR_average_height"], abs=1e-6)
+
+
+@pytest.mark.parametrize("which", ["rmat", "powerlaw", "band_empty_rows"])
+@pytest.mark.parametrize("cb", [5, 8, 16])
+def test_blocking_info_vs_oracle(which, cb):
+    """Random, Keeper (values seed / seed + rows) and fixed-size groupings,
+    column blocks that leave a short last block (cb = 5), empty rows: the
+    GPU figures equal the oracle's exactly, the average height bit for bit."""
+    _need_gpu()
+    import smat
+    rng = np.random.default_rng(7)
+    if which == "rmat":
+        M, K, rp, ci, _ = gen.rmat(scale=11, edge_factor=8, seed=51)
+    elif which == "powerlaw":
+        M, K, rp, ci, _ = gen.power_law(n=3000, max_len=600, seed=52)
+    else:
+        M, K, rp, ci, _ = gen.band(n=1000, hb=6, p=0.5, seed=53)
+        keep = rng.random(M) < 0.7  # drop whole rows
+        lens = np.diff(rp) * keep
+        ci = np.concatenate([ci[rp[r]:rp[r + 1]] for r in range(M) if keep[r]] + [np.zeros(0, ci.dtype)])
+        rp = np.concatenate([[0], np.cumsum(lens)]).astype(rp.dtype)
+    K = int(K) + 3  # a few empty trailing columns
+    g_keeper, _ = smat.reorder_grouping(rp.astype(np.int32), ci.astype(np.int32), np.float32(0.5), cb, 16)
+    groupings = {"random": rng.integers(0, 2 * M, size=M, dtype=np.int64), "keeper": g_keeper,
+                 "fixed": smat.fixed_size_grouping(g_keeper, 16)}
+    d_rp = torch.from_numpy(rp.astype(np.int32)).cuda()
+    d_ci = torch.from_numpy(ci.astype(np.int32)).cuda()
+    for gname, g in groupings.items():
+        print(f"blocking_info {which} cb {cb} {gname}", flush=True)
+        info, avg = smat.blocking_info(M, K, d_rp, d_ci, torch.from_numpy(g).cuda(), cb)
+        ref, ref_avg = of.collect_blocking_info(rp.astype(np.int64), ci.astype(np.int64), g, K, cb)
+        assert info.tolist() == ref.tolist(), gname
+        assert np.float32(avg).tobytes() == np.float32(ref_avg).tobytes(), gname
+
+
+def test_blocking_info_errors_and_empty():
+    _need_gpu()
+    import smat
+    rp = torch.tensor([0, 2, 3], dtype=torch.int32).cuda()
+    ci = torch.tensor([0, 4, 1], dtype=torch.int32).cuda()
+    with pytest.raises(smat.SmatError):
+        smat.blocking_info(2, 5, rp, ci, torch.tensor([0, 4], dtype=torch.int64).cuda(), 2)   # 4 >= 2 * rows
+    with pytest.raises(smat.SmatError):
+        smat.blocking_info(2, 4, rp, ci, torch.tensor([0, 1], dtype=torch.int64).cuda(), 2)   # column 4 >= cols
+    info, avg = smat.blocking_info(2, 5, rp, ci, torch.tensor([1, 1], dtype=torch.int64).cuda(), 2)
+    # one block row of height 2, blocks {0, 2} (last, 1 wide) -> 2*2 + 2*2 - 2*1 = 6 entries, 2 blocks
+    assert info.tolist() == [6, 2, 2] and avg == 2.0
+    z = torch.zeros(3, dtype=torch.int32).cuda()
+    info, avg = smat.blocking_info(2, 5, z, None, torch.tensor([0, 1], dtype=torch.int64).cuda(), 2)
+    assert info.tolist() == [0, 0, 0] and np.isnan(avg)

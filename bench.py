@@ -33,8 +33,11 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per SpMM call measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (calibrated with a 1 GiB copy: x2.0 / x1.0) for configurations that have a
-# committed profile: (workload, scale, N, bk, n_tile) -> (bytes, profile file).
-MEASURED_TRAFFIC = {}
+# committed profile, under the automatic work schedule:
+# (workload, scale, N, bk, n_tile) -> (bytes, profile file).
+MEASURED_TRAFFIC = {
+    ("rmat", 18, 256, 2, "auto"): (2159e6, "profiles/spmm_lds_bk2_rmat18_n256_hbm_traffic.md"),
+}
 
 
 def parse():
@@ -316,7 +319,7 @@ def batched_workload(args, log):
     model = tuner.Model()
     useful = sum(2.0 * m[1][3].size * N for m in mats)
     Bs, Cs, R = [], [], []
-    picks, sweep_best, within = [], [], []
+    picks, sweep_best, within, single_ms = [], [], [], [0.0, 0.0]
     t_sweep = time.time()
     for i, (kind, (M, K, rp, ci, v)) in enumerate(mats):
         Bbits = gen.dense_b(K, N, seed=100 + i)
@@ -333,6 +336,8 @@ def batched_workload(args, log):
         sb = int(np.argmin(times))
         sweep_best.append(tuner.CONFIGS[sb])
         within.append(float(times[b] / times[sb]))
+        single_ms[0] += float(times[b])
+        single_ms[1] += float(times[sb])
     log(f"features, model picks and exhaustive sweeps in {time.time() - t_sweep:.1f}s")
 
     def build(cfgs):
@@ -387,6 +392,9 @@ def batched_workload(args, log):
     return {"useful_flop": useful, "results": res,
             "model": {"within_10pct_of_sweep_best": float(np.mean(within <= 1.10)),
                       "median_ratio_to_sweep_best": float(np.median(within)),
+                      "sum_of_single_calls_ms": {"tuned": round(single_ms[0], 4),
+                                                 "sweep_best": round(single_ms[1], 4)},
+                      "ratio_per_matrix": [round(x, 3) for x in within],
                       "picks": [str(c) for c in picks]},
             "kinds": [m[0] for m in mats], "rows": [int(m[1][0]) for m in mats],
             "nnz": [int(m[1][3].size) for m in mats]}
@@ -564,7 +572,7 @@ def main():
     # "auto" N-tiles are 64 columns for N >= 64
     nt_key = "auto" if tuned["n_tile"] == "auto" or (tuned["n_tile"] == 64 and ctx.N >= 64) else tuned["n_tile"]
     traffic = MEASURED_TRAFFIC.get((args.workload, args.scale, ctx.N, tuned["bk"], nt_key)) \
-        if tuned["kernel"] == "dense" and args.dtype == "f16" and args.split == 0 else None
+        if tuned["kernel"] == "dense" and args.dtype == "f16" and args.split == 0 and args.schedule == 0 else None
     out = {
         "metric": "SpMM useful GFLOP/s (2*nnz*N / t), learned-tuned vs plain FP16 BCSR",
         "value": tuned["useful_gflops"],

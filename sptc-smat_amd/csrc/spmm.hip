@@ -745,7 +745,7 @@ struct BatchRed {
 };
 
 __global__ void k_batch_reset(unsigned long long* __restrict__ counters) {
-  if (threadIdx.x < 24) counters[threadIdx.x] = 0ull;  // 3 tile widths x 8 XCD ranges
+  if (threadIdx.x < 32) counters[threadIdx.x] = 0ull;  // 4 tile widths x 8 XCD ranges
 }
 
 // Wave-uniform copies (scalar registers) of values every lane holds equally.
@@ -783,7 +783,9 @@ __device__ __forceinline__ void batch_tile(int bkc, const Item& it, int nt, cons
                                            uint8_t* lds, int lane) {
   const int64_t n0 = (int64_t)nt * 16 * NS;
   switch (bkc) {
-    case 6: mixed_tile<NS, DT, false>(it, n0, op, mx, lds, lane); break;
+    case 6:
+      if constexpr (NS <= 4) mixed_tile<NS, DT, false>(it, n0, op, mx, lds, lane);  // 2:4 tiles are <= 64 wide
+      break;
     case 0: lds_tile<1, NS, DT, false>(it, n0, op, lds, lane); break;
     case 1: lds_tile<2, NS, DT, false>(it, n0, op, lds, lane); break;
     case 2: lds_tile<4, NS, DT, false>(it, n0, op, lds, lane); break;
@@ -802,11 +804,11 @@ __device__ __forceinline__ void batch_tile(int bkc, const Item& it, int nt, cons
 // the other ranges in turn.  Exit: counters only grow, so after at most one
 // failed draw per range every wave leaves the loop.
 template <int DT, int NS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NS == 4 ? 3 : 4)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NS == 8 ? 2 : NS == 4 ? 3 : 4)))
 void k_batch(const BatchMat* __restrict__ mats, int32_t nmats, int64_t total, int64_t chunk,
              unsigned long long* __restrict__ counters) {
-  constexpr int WB = lds_wave_bytes<NS, DT>() > mixed_wave_bytes<NS>() ? lds_wave_bytes<NS, DT>()
-                                                                          : mixed_wave_bytes<NS>();
+  constexpr int WB = NS > 4 || lds_wave_bytes<NS, DT>() > mixed_wave_bytes<NS>() ? lds_wave_bytes<NS, DT>()
+                                                                                  : mixed_wave_bytes<NS>();
   constexpr int NX = 8;
   __shared__ __attribute__((aligned(16))) uint8_t lds_all[kWaves * WB];
   const int lane = threadIdx.x & 63;
@@ -1440,15 +1442,25 @@ struct smat_batch_t {
   int32_t count = 0;
   int32_t dt = 0;
   int64_t total = 0, n_red = 0;
-  int32_t g_beg[3] = {0, 0, 0}, g_cnt[3] = {0, 0, 0};  // matrices per N-tile width 16, 32, 64
-  int64_t g_tasks[3] = {0, 0, 0};
-  unsigned grid[3] = {0, 0, 0};
+  int32_t g_beg[4] = {0, 0, 0, 0}, g_cnt[4] = {0, 0, 0, 0};  // matrices per N-tile width 16, 32, 64, 128
+  int64_t g_tasks[4] = {0, 0, 0, 0};
+  unsigned grid[4] = {0, 0, 0, 0};
   smat::DevBuf mats, red, counter;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // groups after the first run on side streams forked from the caller's
+  // stream (parallel graph branches when captured), joined before the
+  // split-row reduction
+  hipStream_t side[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[3] = {nullptr, nullptr, nullptr};
   ~smat_batch_t() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    for (int i = 0; i < 3; i++) {
+      if (side[i]) (void)hipStreamDestroy(side[i]);
+      if (join[i]) (void)hipEventDestroy(join[i]);
+    }
+    if (fork) (void)hipEventDestroy(fork);
   }
 };
 
@@ -1474,16 +1486,30 @@ void batch_group(const smat_batch_t* b, int g, hipStream_t s) {
   unsigned long long* c = b->counter.as<unsigned long long>() + 8 * g;
   if (g == 0) hipLaunchKernelGGL((k_batch<DT, 1>), b->grid[g], 64 * kWaves, 0, s, m, b->g_cnt[g], b->g_tasks[g], chunk, c);
   else if (g == 1) hipLaunchKernelGGL((k_batch<DT, 2>), b->grid[g], 64 * kWaves, 0, s, m, b->g_cnt[g], b->g_tasks[g], chunk, c);
-  else hipLaunchKernelGGL((k_batch<DT, 4>), b->grid[g], 64 * kWaves, 0, s, m, b->g_cnt[g], b->g_tasks[g], chunk, c);
+  else if (g == 2) hipLaunchKernelGGL((k_batch<DT, 4>), b->grid[g], 64 * kWaves, 0, s, m, b->g_cnt[g], b->g_tasks[g], chunk, c);
+  else hipLaunchKernelGGL((k_batch<DT, 8>), b->grid[g], 64 * kWaves, 0, s, m, b->g_cnt[g], b->g_tasks[g], chunk, c);
 }
+
+inline int nsub_group(int nsub) { return nsub == 1 ? 0 : nsub == 2 ? 1 : nsub == 4 ? 2 : 3; }
 
 smat_status batch_launch(smat_batch_t* b, hipStream_t s) {
   hipLaunchKernelGGL(k_batch_reset, 1, 64, 0, s, b->counter.as<unsigned long long>());
-  for (int g = 0; g < 3; g++) {
+  // the first non-empty group runs on s, the others on side streams that
+  // start after the counter reset; s waits for them before the reduction
+  int used = 0, nside = 0;
+  for (int g = 0; g < 4; g++) {
     if (b->g_cnt[g] == 0) continue;
-    if (b->dt == SMAT_BF16) batch_group<1>(b, g, s);
-    else batch_group<0>(b, g, s);
+    hipStream_t gs = s;
+    if (used++ > 0) {
+      if (nside == 0) SMAT_HIP_TRY(hipEventRecord(b->fork, s));
+      gs = b->side[nside];
+      SMAT_HIP_TRY(hipStreamWaitEvent(gs, b->fork, 0));
+    }
+    if (b->dt == SMAT_BF16) batch_group<1>(b, g, gs);
+    else batch_group<0>(b, g, gs);
+    if (gs != s) SMAT_HIP_TRY(hipEventRecord(b->join[nside++], gs));
   }
+  for (int i = 0; i < nside; i++) SMAT_HIP_TRY(hipStreamWaitEvent(s, b->join[i], 0));
   if (b->n_red > 0)
     hipLaunchKernelGGL(k_batch_reduce, (unsigned)b->n_red, 256, 0, s, b->red.as<BatchRed>(), b->mats.as<BatchMat>());
   SMAT_LAUNCH_CHECK();
@@ -1504,22 +1530,22 @@ smat_status smat_batch_create(int32_t count, const smat_plan* plans, const smat_
   std::vector<BatchMat> mats((size_t)count);
   std::vector<BatchRed> reds;
   const int dt = plans[0] ? plans[0]->opts.in_dtype : 0;
-  // matrices ordered by N-tile width (16, 32, 64 and wider), input order inside a group
+  // matrices ordered by N-tile width (16, 32, 64, 128), input order inside a group
   std::vector<int32_t> order;
-  int32_t g_cnt[3] = {0, 0, 0};
-  for (int g = 0; g < 3; g++)
+  int32_t g_cnt[4] = {0, 0, 0, 0};
+  for (int g = 0; g < 4; g++)
     for (int32_t i = 0; i < count; i++)
-      if (plans[i] && (plans[i]->nsub == 1 ? 0 : plans[i]->nsub == 2 ? 1 : 2) == g) {
+      if (plans[i] && nsub_group(plans[i]->nsub) == g) {
         order.push_back(i);
         g_cnt[g]++;
       }
   SMAT_REQUIRE((int32_t)order.size() == count, "smat_batch_create: null plan");
-  int64_t g_tasks[3] = {0, 0, 0};
+  int64_t g_tasks[4] = {0, 0, 0, 0};
   for (int32_t pos = 0; pos < count; pos++) {
     const int32_t i = order[(size_t)pos];
     const smat_plan_t* P = plans[i];
     const smat_spmm_operands& o = ops[i];
-    const int g = P->nsub == 1 ? 0 : P->nsub == 2 ? 1 : 2;
+    const int g = nsub_group(P->nsub);
     SMAT_REQUIRE(P && o.b && o.c, "smat_batch_create: entry %d: null plan or operand", i);
     SMAT_REQUIRE(P->opts.in_dtype == dt && (dt == SMAT_F16 || dt == SMAT_BF16) &&
                      P->opts.b_layout == SMAT_B_ROW_MAJOR && P->opts.out_dtype == SMAT_F32 &&
@@ -1538,7 +1564,7 @@ smat_status smat_batch_create(int32_t count, const smat_plan* plans, const smat_
     SMAT_REQUIRE(lds_aligned(P, m.op, 8), "smat_batch_create: entry %d: N and ldb must be multiples of 8 and B "
                                           "16-byte aligned", i);
     m.items = P->items.as<Item>();
-    m.ntiles = ntiles_of(P, g == 0 ? 1 : g == 1 ? 2 : 4);  // 128-column plans run as 64-column tiles here
+    m.ntiles = ntiles_of(P, P->nsub);
     m.kind = P->mixed ? 6 : bk_code(P->A.bk);
     m.mx = P->mixed ? Mixed24{P->dense_ids.as<int32_t>(), P->sparse_ids.as<int32_t>(), o.values24,
                               static_cast<const uint16_t*>(o.index24)}
@@ -1555,8 +1581,8 @@ smat_status smat_batch_create(int32_t count, const smat_plan* plans, const smat_
   smat_batch_t* B = new smat_batch_t();
   B->count = count;
   B->dt = dt;
-  B->total = g_tasks[0] + g_tasks[1] + g_tasks[2];
-  for (int g = 0, beg = 0; g < 3; g++) {
+  B->total = g_tasks[0] + g_tasks[1] + g_tasks[2] + g_tasks[3];
+  for (int g = 0, beg = 0; g < 4; g++) {
     B->g_beg[g] = beg;
     B->g_cnt[g] = g_cnt[g];
     B->g_tasks[g] = g_tasks[g];
@@ -1565,7 +1591,7 @@ smat_status smat_batch_create(int32_t count, const smat_plan* plans, const smat_
   B->n_red = (int64_t)reds.size();
   smat_status st = B->mats.alloc(mats.size() * sizeof(BatchMat));
   if (st == SMAT_OK) st = B->red.alloc((reds.size() + 1) * sizeof(BatchRed));
-  if (st == SMAT_OK) st = B->counter.alloc(24 * sizeof(unsigned long long));
+  if (st == SMAT_OK) st = B->counter.alloc(32 * sizeof(unsigned long long));
   if (st != SMAT_OK) {
     delete B;
     return st;
@@ -1581,6 +1607,18 @@ smat_status smat_batch_create(int32_t count, const smat_plan* plans, const smat_
   B->grid[0] = dt == SMAT_BF16 ? batch_grid<1, 1>() : batch_grid<0, 1>();
   B->grid[1] = dt == SMAT_BF16 ? batch_grid<1, 2>() : batch_grid<0, 2>();
   B->grid[2] = dt == SMAT_BF16 ? batch_grid<1, 4>() : batch_grid<0, 4>();
+  B->grid[3] = dt == SMAT_BF16 ? batch_grid<1, 8>() : batch_grid<0, 8>();
+  int used = 0;
+  for (int g = 0; g < 4; g++) used += B->g_cnt[g] > 0;
+  hipError_t ce = hipEventCreateWithFlags(&B->fork, hipEventDisableTiming);
+  for (int i = 0; i + 1 < used && ce == hipSuccess; i++) {
+    ce = hipStreamCreateWithFlags(&B->side[i], hipStreamNonBlocking);
+    if (ce == hipSuccess) ce = hipEventCreateWithFlags(&B->join[i], hipEventDisableTiming);
+  }
+  if (ce != hipSuccess) {
+    delete B;
+    return smat::set_error(SMAT_ERR_HIP, "smat_batch_create: stream setup failed: %s", hipGetErrorString(ce));
+  }
   *out = B;
   return SMAT_OK;
 }
@@ -1627,7 +1665,7 @@ smat_status smat_batch_info(smat_batch B, int64_t* tasks, int64_t* split_rows, i
   if (split_rows) *split_rows = B->n_red;
   if (grid) {  // waves the largest group's launch keeps resident
     int32_t gmax = 0;
-    for (int g = 0; g < 3; g++)
+    for (int g = 0; g < 4; g++)
       if (B->g_cnt[g] > 0 && (int32_t)B->grid[g] > gmax) gmax = (int32_t)B->grid[g];
     *grid = gmax;
   }

@@ -1,5 +1,5 @@
 """GPU parity of the batched persistent launch: every matrix of a batch (mixed
-block widths, N-tiles, split rows, reordered rows) equals its own SpMM call,
+block widths, N-tiles 16-128, split rows, reordered rows) equals its own SpMM call,
 eagerly and through repeated HIP-graph replays, and the FP64 oracle."""
 import numpy as np
 import pytest
@@ -48,7 +48,7 @@ def _build(mats, N, seed=1, with_rowmap=False):
     entries, refs, keep = [], [], []
     for i, (M, K, rp, ci, v) in enumerate(mats):
         bk = [1, 2, 4, 8, 16, 32][i % 6]
-        n_tile = [0, 16, 32, 64][(i // 6) % 4]
+        n_tile = [0, 16, 32, 64, 128][i % 5]
         split = [0, 0, 16][i % 3]
         perm = None
         if with_rowmap and i % 2 == 1:
@@ -66,7 +66,8 @@ def _build(mats, N, seed=1, with_rowmap=False):
             A = smat.Bcsr(M, K, _dev(rp_a), _dev(ci_a), _dev(v_a.view(np.int16)), bk=16,
                           tag_rule=smat.SMAT_TAG_EXACT24)
             v24, i24 = A.compress_24(smat.SMAT_24_SMFMAC)
-            plan = smat.SpmmPlan(A, N, n_tile=n_tile, max_blocks_per_item=split, kernel=smat.SMAT_KERNEL_MIXED24)
+            plan = smat.SpmmPlan(A, N, n_tile=min(n_tile, 64), max_blocks_per_item=split,
+                                 kernel=smat.SMAT_KERNEL_MIXED24)  # 2:4 tiles are at most 64 wide
         else:
             A = smat.Bcsr(M, K, _dev(rp_a), _dev(ci_a), _dev(v_a.view(np.int16)), bk=bk)
             v24 = i24 = None
@@ -83,13 +84,13 @@ def _build(mats, N, seed=1, with_rowmap=False):
 
 
 @pytest.mark.parametrize("with_rowmap", [False, True])
-def test_batch_matches_each_call_and_oracle(with_rowmap):
+@pytest.mark.parametrize("N", [64, 256])
+def test_batch_matches_each_call_and_oracle(with_rowmap, N):
     _need_gpu()
     import smat
-    N = 64
     mats = _mats(24, seed=3)
     entries, refs, keep = _build(mats, N, with_rowmap=with_rowmap)
-    print(f"batch of {len(entries)} rowmap={with_rowmap}", flush=True)
+    print(f"batch of {len(entries)} N={N} rowmap={with_rowmap}", flush=True)
     # one call each
     singles = []
     for (plan, ops), (A, Bd, C, rm, v24, i24) in zip(entries, keep):

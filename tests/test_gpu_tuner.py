@@ -165,3 +165,8 @@ def test_trained_model_on_held_out_matrices():
     assert r["within_10pct"] >= 0.9
     assert r["geo_speedup_vs_plain"] > 1.3
     assert gain_vs_fixed > 1.0
+    for f, d in zip(test_files, te):  # each held-out distribution on its own (batched sizes, large)
+        rf = tuner.evaluate(tuner.Model(), d["feats"], d["times"], d["rows"])
+        print(os.path.basename(f), json.dumps({k: rf[k] for k in ("n", "within_10pct", "geo_speedup_vs_plain")}),
+              flush=True)
+        assert rf["within_10pct"] >= 0.9, f

@@ -376,6 +376,13 @@ def batched_workload(args, log):
                                        "max_rel_fro_err": errors()}
             for p in seq_plans:
                 p.close()
+        if name == "tuned":  # the same plans, one launch per matrix
+            with torch.cuda.stream(s):
+                ms, wall = time_fn(lambda: [b["plan"].execute(Bs[i], Cs[i], values24=b["v24"], index24=b["i24"],
+                                                              c_rowmap=b["rowmap"]) for i, b in enumerate(built)], s)
+            res["tuned_sequential"] = {"event_ms": round(ms, 4), "wall_ms": round(wall, 4),
+                                       "useful_gflops": round(useful / (ms * 1e-3) / 1e9, 1),
+                                       "max_rel_fro_err": errors()}
         batch = smat.Batch(ops(built))
         with torch.cuda.stream(s):
             batch.capture(s)

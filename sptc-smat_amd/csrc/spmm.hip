@@ -832,17 +832,39 @@ void k_batch(const BatchMat* __restrict__ mats, int32_t nmats, int64_t total, in
       if (mats[mid].task_beg <= t0) lo = mid;
       else hi = mid - 1;
     }
+    // The chunk's tasks are consecutive: the matrix fields are loaded when the
+    // walk enters a matrix and the item when it enters an item (the N tiles of
+    // one item are consecutive tasks), not per task.
+    int32_t cur = -1;
+    int64_t tbeg = 0, next_beg = 0, cur_item = -1;
+    uint32_t ntl = 1;
+    int kind = 0;
+    const Item* items = nullptr;
+    Operands op{};
+    Mixed24 mx{nullptr, nullptr, nullptr, nullptr};
+    Item it{};
     for (int64_t task = t0; task < t1; task++) {
-      while (lo + 1 < nmats && mats[lo + 1].task_beg <= task) lo++;
-      const BatchMat& m = mats[lo];
-      const uint32_t local = (uint32_t)(task - uni(m.task_beg));  // < 2^31 per matrix
-      const uint32_t ntl = (uint32_t)__builtin_amdgcn_readfirstlane(m.ntiles);
-      const Item it = uni(m.items)[local / ntl];
-      const int kind = __builtin_amdgcn_readfirstlane(m.kind);
-      const Mixed24 mx = kind == 6 ? Mixed24{uni(m.mx.dense_ids), uni(m.mx.sparse_ids), uni(m.mx.values24),
-                                             uni(m.mx.index24)}
-                                   : Mixed24{nullptr, nullptr, nullptr, nullptr};
-      batch_tile<DT, NS>(kind, it, (int)(local % ntl), uni(m.op), mx, lds, lane);
+      if (cur < 0 || task >= next_beg) {
+        while (lo + 1 < nmats && mats[lo + 1].task_beg <= task) lo++;
+        const BatchMat& m = mats[lo];
+        cur = lo;
+        tbeg = uni(m.task_beg);
+        next_beg = lo + 1 < nmats ? uni(mats[lo + 1].task_beg) : INT64_MAX;
+        ntl = (uint32_t)__builtin_amdgcn_readfirstlane(m.ntiles);
+        kind = __builtin_amdgcn_readfirstlane(m.kind);
+        items = uni(m.items);
+        op = uni(m.op);
+        mx = kind == 6 ? Mixed24{uni(m.mx.dense_ids), uni(m.mx.sparse_ids), uni(m.mx.values24), uni(m.mx.index24)}
+                       : Mixed24{nullptr, nullptr, nullptr, nullptr};
+        cur_item = -1;
+      }
+      const uint32_t local = (uint32_t)(task - tbeg);  // < 2^31 per matrix
+      const int64_t iid = local / ntl;
+      if (iid != cur_item) {
+        it = items[iid];
+        cur_item = iid;
+      }
+      batch_tile<DT, NS>(kind, it, (int)(local % ntl), op, mx, lds, lane);
     }
   }
 }
@@ -1495,13 +1517,15 @@ inline int nsub_group(int nsub) { return nsub == 1 ? 0 : nsub == 2 ? 1 : nsub ==
 smat_status batch_launch(smat_batch_t* b, hipStream_t s) {
   hipLaunchKernelGGL(k_batch_reset, 1, 64, 0, s, b->counter.as<unsigned long long>());
   // the first non-empty group runs on s, the others on side streams that
-  // start after the counter reset; s waits for them before the reduction
-  int used = 0, nside = 0;
+  // start right after the counter reset (the fork is recorded before any
+  // group is launched); s waits for them before the reduction
+  int groups = 0, used = 0, nside = 0;
+  for (int g = 0; g < 4; g++) groups += b->g_cnt[g] > 0;
+  if (groups > 1) SMAT_HIP_TRY(hipEventRecord(b->fork, s));
   for (int g = 0; g < 4; g++) {
     if (b->g_cnt[g] == 0) continue;
     hipStream_t gs = s;
     if (used++ > 0) {
-      if (nside == 0) SMAT_HIP_TRY(hipEventRecord(b->fork, s));
       gs = b->side[nside];
       SMAT_HIP_TRY(hipStreamWaitEvent(gs, b->fork, 0));
     }

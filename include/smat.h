@@ -381,24 +381,39 @@ smat_status smat_error_sums(const float* C, int64_t ldc, const double* R, int64_
                             int64_t N, double* out, double* work, smat_stream stream);
 
 /* ------------------------------------------------------------------------ */
-/* Batches: many SpMMs in one persistent launch                              */
+/* Batches: many SpMMs in one launch (one HIP graph when captured)           */
 /* ------------------------------------------------------------------------ */
 /*
- * A batch runs `count` planned SpMMs (each with its own block width, N-tile
- * and operands) as one persistent kernel: a grid sized to fill the GPU
- * draws (matrix, work item, N-tile) tasks from one device counter, then one
- * kernel reduces the split rows of every matrix.  Plans (dense or mixed 2:4)
- * must share one 16-bit input type (F16 or BF16) and use row-major B staged
- * through LDS and FP32 C; the plans and every operand buffer must outlive the
- * batch (their addresses are baked in).  No reference counterpart: the
- * reference launches one kernel per matrix (tester.h:944-1011).
+ * A batch runs `count` planned SpMMs (each with its own block width, N-tile,
+ * work schedule and operands).  Two ways to run it:
+ *   SMAT_BATCH_PERSISTENT: one persistent queue kernel per N-tile width
+ *     (concurrent), drawing (matrix, work item, N-tile) tasks from per-XCD
+ *     device counters, then one kernel reducing the split rows of every
+ *     matrix.  Plans (dense or mixed 2:4) must share one 16-bit input type
+ *     (F16 or BF16), use row-major B staged through LDS and FP32 C.
+ *   SMAT_BATCH_BRANCHES: each matrix's own call (its specialised kernels and
+ *     work order), the matrices spread over 4 streams balanced by estimated
+ *     work, longest first: parallel branches of the captured graph.  Any plan.
+ * SMAT_BATCH_AUTO is SMAT_BATCH_BRANCHES.  Entries may share a plan only if
+ * it has no split rows (split rows use the plan's scratch).  The plans and
+ * every operand buffer must outlive the batch (their addresses are baked
+ * in).  No reference counterpart: the reference launches one kernel per
+ * matrix (tester.h:944-1011).
  */
 typedef struct smat_batch_t* smat_batch;
+#define SMAT_BATCH_AUTO 0
+#define SMAT_BATCH_PERSISTENT 1
+#define SMAT_BATCH_BRANCHES 2
+smat_status smat_batch_create_ex(int32_t count, const smat_plan* plans, const smat_spmm_operands* ops,
+                                 int32_t mode, smat_stream stream, smat_batch* out);
+/* smat_batch_create_ex with SMAT_BATCH_AUTO. */
 smat_status smat_batch_create(int32_t count, const smat_plan* plans, const smat_spmm_operands* ops,
                               smat_stream stream, smat_batch* out);
-/* Captures the batch's launches (counter reset, queue kernel, split-row
- * reduction; no memset nodes) into a HIP graph on `stream` (not the default
- * stream); later smat_batch_execute calls replay the graph. */
+/* Captures the batch's launches (no memset nodes) into a HIP graph on
+ * `stream` (not the default stream); later smat_batch_execute calls replay
+ * the graph.  smat_batch_info: tasks = wave tasks of all plans, split_rows =
+ * rows reduced from partial tiles, grid = resident workgroups of the largest
+ * persistent group (persistent) or branches (branches). */
 smat_status smat_batch_capture(smat_batch batch, smat_stream stream);
 smat_status smat_batch_execute(smat_batch batch, smat_stream stream);
 smat_status smat_batch_info(smat_batch batch, int64_t* tasks, int64_t* split_rows, int32_t* grid,

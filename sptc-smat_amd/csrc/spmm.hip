@@ -1371,7 +1371,13 @@ smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat
   return SMAT_OK;
 }
 
-smat_status smat_spmm_execute_ex(smat_plan P, const smat_spmm_operands* ops, smat_stream stream) {
+}  // extern "C"
+
+namespace {
+
+// Checks the operands of one call against its plan and fills the kernels'
+// view of them (no launch).
+smat_status prepare_operands(smat_plan P, const smat_spmm_operands* ops, Operands* out) {
   SMAT_REQUIRE(P && ops && ops->b && ops->c, "smat_spmm_execute: null argument");
   const int64_t ldb = ops->ldb, ldc = ops->ldc;
   const int dt = P->opts.in_dtype;
@@ -1401,7 +1407,14 @@ smat_status smat_spmm_execute_ex(smat_plan P, const smat_spmm_operands* ops, sma
   if (P->mixed)
     SMAT_REQUIRE(ops->values24 && ops->index24 && (reinterpret_cast<uintptr_t>(ops->values24) & 15) == 0,
                  "smat_spmm_execute: the 2:4 kernel needs 16-byte aligned values24 and index24");
-  hipStream_t s = smat::to_hip(stream);
+  *out = op;
+  return SMAT_OK;
+}
+
+// Launches one call's kernels (main kernel, then the split-row reduction).
+smat_status launch_call(smat_plan P, const smat_spmm_operands* ops, const Operands& op, hipStream_t s) {
+  const int dt = P->opts.in_dtype;
+  const int64_t ldc = ops->ldc;
   if (P->mixed) {
     const uint16_t* i24 = static_cast<const uint16_t*>(ops->index24);
     if (dt == SMAT_BF16) dispatch_mixed<1>(P, ops->values24, i24, op, s);
@@ -1428,6 +1441,16 @@ smat_status smat_spmm_execute_ex(smat_plan P, const smat_spmm_operands* ops, sma
   }
   SMAT_LAUNCH_CHECK();
   return SMAT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+smat_status smat_spmm_execute_ex(smat_plan P, const smat_spmm_operands* ops, smat_stream stream) {
+  Operands op{};
+  SMAT_TRY(prepare_operands(P, ops, &op));
+  return launch_call(P, ops, op, smat::to_hip(stream));
 }
 
 smat_status smat_spmm_execute(smat_plan P, const uint16_t* values24, const void* index24, const uint16_t* B,
@@ -1475,6 +1498,14 @@ struct smat_batch_t {
   // split-row reduction
   hipStream_t side[3] = {nullptr, nullptr, nullptr};
   hipEvent_t fork = nullptr, join[3] = {nullptr, nullptr, nullptr};
+  // SMAT_BATCH_BRANCHES: the entries, and per branch the entries it runs in
+  // order (lane l: lane_order[lane_beg[l] .. lane_beg[l + 1]))
+  int32_t mode = SMAT_BATCH_PERSISTENT;
+  int32_t lanes = 0;
+  std::vector<smat_plan> plans;
+  std::vector<smat_spmm_operands> ops;
+  std::vector<Operands> prepared;
+  std::vector<int32_t> lane_order, lane_beg;
   ~smat_batch_t() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -1514,7 +1545,25 @@ void batch_group(const smat_batch_t* b, int g, hipStream_t s) {
 
 inline int nsub_group(int nsub) { return nsub == 1 ? 0 : nsub == 2 ? 1 : nsub == 4 ? 2 : 3; }
 
+// Branches: lane 0 runs on s, lane l > 0 on side stream l - 1, forked from s
+// before any work and joined back at the end.
+smat_status batch_launch_branches(smat_batch_t* b, hipStream_t s) {
+  if (b->lanes > 1) SMAT_HIP_TRY(hipEventRecord(b->fork, s));
+  for (int l = 0; l < b->lanes; l++) {
+    hipStream_t ls = l == 0 ? s : b->side[l - 1];
+    if (l > 0) SMAT_HIP_TRY(hipStreamWaitEvent(ls, b->fork, 0));
+    for (int32_t k = b->lane_beg[l]; k < b->lane_beg[l + 1]; k++) {
+      const int32_t i = b->lane_order[k];
+      SMAT_TRY(launch_call(b->plans[i], &b->ops[i], b->prepared[i], ls));
+    }
+    if (l > 0) SMAT_HIP_TRY(hipEventRecord(b->join[l - 1], ls));
+  }
+  for (int l = 1; l < b->lanes; l++) SMAT_HIP_TRY(hipStreamWaitEvent(s, b->join[l - 1], 0));
+  return SMAT_OK;
+}
+
 smat_status batch_launch(smat_batch_t* b, hipStream_t s) {
+  if (b->mode == SMAT_BATCH_BRANCHES) return batch_launch_branches(b, s);
   hipLaunchKernelGGL(k_batch_reset, 1, 64, 0, s, b->counter.as<unsigned long long>());
   // the first non-empty group runs on s, the others on side streams that
   // start right after the counter reset (the fork is recorded before any
@@ -1542,14 +1591,79 @@ smat_status batch_launch(smat_batch_t* b, hipStream_t s) {
 
 int bk_code(int bk) { return bk == 1 ? 0 : bk == 2 ? 1 : bk == 4 ? 2 : bk == 8 ? 3 : bk == 16 ? 4 : 5; }
 
+smat_status batch_create_branches(int32_t count, const smat_plan* plans, const smat_spmm_operands* ops,
+                                  smat_batch* out) {
+  smat_batch_t* B = new smat_batch_t();
+  B->mode = SMAT_BATCH_BRANCHES;
+  B->count = count;
+  B->plans.assign(plans, plans + count);
+  B->ops.assign(ops, ops + count);
+  B->prepared.resize((size_t)count);
+  std::vector<double> work((size_t)count);
+  for (int32_t i = 0; i < count; i++) {
+    const smat_status st = prepare_operands(plans[i], &ops[i], &B->prepared[(size_t)i]);
+    if (st != SMAT_OK) {
+      delete B;
+      return st;
+    }
+    const smat_plan_t* P = plans[i];
+    B->total += P->n_items * (int64_t)ntiles_of(P, P->nsub);
+    B->n_red += P->n_red;
+    // stored-block MFMA work plus the C tile written, in elements
+    work[(size_t)i] = (double)P->A.nnzb * P->A.bk * 16.0 * (double)P->N + (double)P->A.nbrows * 16.0 * (double)P->N;
+  }
+  // longest first onto the least loaded branch
+  B->lanes = count < 4 ? count : 4;
+  std::vector<int32_t> idx((size_t)count);
+  for (int32_t i = 0; i < count; i++) idx[(size_t)i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return work[(size_t)a] > work[(size_t)b]; });
+  std::vector<std::vector<int32_t>> lane((size_t)B->lanes);
+  std::vector<double> load((size_t)B->lanes, 0.0);
+  for (int32_t i : idx) {
+    const size_t l = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+    lane[l].push_back(i);
+    load[l] += work[(size_t)i];
+  }
+  B->lane_beg.push_back(0);
+  for (auto& v : lane) {
+    B->lane_order.insert(B->lane_order.end(), v.begin(), v.end());
+    B->lane_beg.push_back((int32_t)B->lane_order.size());
+  }
+  hipError_t ce = hipEventCreateWithFlags(&B->fork, hipEventDisableTiming);
+  for (int i = 0; i + 1 < B->lanes && ce == hipSuccess; i++) {
+    ce = hipStreamCreateWithFlags(&B->side[i], hipStreamNonBlocking);
+    if (ce == hipSuccess) ce = hipEventCreateWithFlags(&B->join[i], hipEventDisableTiming);
+  }
+  if (ce != hipSuccess) {
+    delete B;
+    return smat::set_error(SMAT_ERR_HIP, "smat_batch_create: stream setup failed: %s", hipGetErrorString(ce));
+  }
+  *out = B;
+  return SMAT_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
 smat_status smat_batch_create(int32_t count, const smat_plan* plans, const smat_spmm_operands* ops,
                               smat_stream stream, smat_batch* out) {
+  return smat_batch_create_ex(count, plans, ops, SMAT_BATCH_AUTO, stream, out);
+}
+
+smat_status smat_batch_create_ex(int32_t count, const smat_plan* plans, const smat_spmm_operands* ops,
+                                 int32_t mode, smat_stream stream, smat_batch* out) {
   SMAT_REQUIRE(out && count > 0 && plans && ops, "smat_batch_create: bad argument");
+  SMAT_REQUIRE(mode == SMAT_BATCH_AUTO || mode == SMAT_BATCH_PERSISTENT || mode == SMAT_BATCH_BRANCHES,
+               "smat_batch_create: unknown mode %d", mode);
   *out = nullptr;
+  for (int32_t i = 0; i < count; i++) {
+    SMAT_REQUIRE(plans[i], "smat_batch_create: entry %d: null plan", i);
+    for (int32_t j = 0; j < i; j++)
+      SMAT_REQUIRE(plans[j] != plans[i] || plans[i]->n_red == 0,
+                   "smat_batch_create: entries %d and %d share a plan with split rows (one scratch)", j, i);
+  }
+  if (mode != SMAT_BATCH_PERSISTENT) return batch_create_branches(count, plans, ops, out);
   hipStream_t s = smat::to_hip(stream);
   std::vector<BatchMat> mats((size_t)count);
   std::vector<BatchRed> reds;
@@ -1687,7 +1801,9 @@ smat_status smat_batch_info(smat_batch B, int64_t* tasks, int64_t* split_rows, i
   SMAT_REQUIRE(B, "smat_batch_info: null batch");
   if (tasks) *tasks = B->total;
   if (split_rows) *split_rows = B->n_red;
-  if (grid) {  // waves the largest group's launch keeps resident
+  if (grid && B->mode == SMAT_BATCH_BRANCHES) {
+    *grid = B->lanes;
+  } else if (grid) {  // workgroups the largest group's launch keeps resident
     int32_t gmax = 0;
     for (int g = 0; g < 4; g++)
       if (B->g_cnt[g] > 0 && (int32_t)B->grid[g] > gmax) gmax = (int32_t)B->grid[g];

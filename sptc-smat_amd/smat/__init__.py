@@ -60,6 +60,7 @@ class SpmmOpts(ctypes.Structure):
 
 SMAT_BSTAGE_AUTO, SMAT_BSTAGE_DIRECT, SMAT_BSTAGE_LDS = 0, 1, 2
 SMAT_SCHED_AUTO, SMAT_SCHED_BLOCK_ROWS, SMAT_SCHED_LONGEST_FIRST, SMAT_SCHED_TILE_AFFINITY = 0, 1, 2, 3
+SMAT_BATCH_AUTO, SMAT_BATCH_PERSISTENT, SMAT_BATCH_BRANCHES = 0, 1, 2
 
 
 class MatrixStats(ctypes.Structure):
@@ -110,6 +111,7 @@ SIGNATURES = {
     "smat_read_edgelist": (_i32, [ctypes.c_char_p, ctypes.c_char_p, _i32, ctypes.POINTER(CsrHost)]),
     "smat_csr_host_free": (None, [ctypes.POINTER(CsrHost)]),
     "smat_batch_create": (_i32, [_i32, _p, _p, _p, ctypes.POINTER(_p)]),
+    "smat_batch_create_ex": (_i32, [_i32, _p, _p, _i32, _p, ctypes.POINTER(_p)]),
     "smat_batch_capture": (_i32, [_p, _p]),
     "smat_batch_execute": (_i32, [_p, _p]),
     "smat_batch_info": (_i32, [_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i32),
@@ -518,18 +520,19 @@ def rel_fro_error(C, R, stream=None) -> float:
 
 
 class Batch:
-    """Many planned SpMMs in one persistent launch (smat_batch_*).  `entries`
-    is a list of (SpmmPlan, SpmmOperands); plans and buffers must outlive the
-    batch."""
+    """Many planned SpMMs in one launch (smat_batch_*): `mode` SMAT_BATCH_AUTO
+    / SMAT_BATCH_BRANCHES (each matrix's own kernels on 4 parallel branches)
+    or SMAT_BATCH_PERSISTENT (persistent queue kernels).  `entries` is a list
+    of (SpmmPlan, SpmmOperands); plans and buffers must outlive the batch."""
 
-    def __init__(self, entries, stream=None):
+    def __init__(self, entries, stream=None, mode: int = 0):
         self._keep = entries
         n = len(entries)
         plans = (_p * n)(*[e[0].handle for e in entries])
         ops = (SpmmOperands * n)(*[e[1] for e in entries])
         h = _p()
-        check(lib().smat_batch_create(n, ctypes.cast(plans, _p), ctypes.cast(ops, _p), stream_ptr(stream),
-                                      ctypes.byref(h)))
+        check(lib().smat_batch_create_ex(n, ctypes.cast(plans, _p), ctypes.cast(ops, _p), mode, stream_ptr(stream),
+                                         ctypes.byref(h)))
         self.handle = h
 
     def capture(self, stream):

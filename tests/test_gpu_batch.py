@@ -85,12 +85,13 @@ def _build(mats, N, seed=1, with_rowmap=False):
 
 @pytest.mark.parametrize("with_rowmap", [False, True])
 @pytest.mark.parametrize("N", [64, 256])
-def test_batch_matches_each_call_and_oracle(with_rowmap, N):
+@pytest.mark.parametrize("mode", [1, 2])
+def test_batch_matches_each_call_and_oracle(with_rowmap, N, mode):
     _need_gpu()
     import smat
     mats = _mats(24, seed=3)
     entries, refs, keep = _build(mats, N, with_rowmap=with_rowmap)
-    print(f"batch of {len(entries)} N={N} rowmap={with_rowmap}", flush=True)
+    print(f"batch of {len(entries)} N={N} rowmap={with_rowmap} mode={mode}", flush=True)
     # one call each
     singles = []
     for (plan, ops), (A, Bd, C, rm, v24, i24) in zip(entries, keep):
@@ -98,7 +99,7 @@ def test_batch_matches_each_call_and_oracle(with_rowmap, N):
         torch.cuda.synchronize()
         singles.append(C.cpu().numpy().copy())
         C.fill_(float("nan"))
-    batch = smat.Batch(entries)
+    batch = smat.Batch(entries, mode=mode)
     info = batch.info()
     assert info["tasks"] > 0 and info["grid"] > 0
     batch()
@@ -110,13 +111,15 @@ def test_batch_matches_each_call_and_oracle(with_rowmap, N):
     batch.close()
 
 
-def test_batch_graph_replays():
+@pytest.mark.parametrize("mode", [1, 2])
+def test_batch_graph_replays(mode):
     _need_gpu()
     import smat
     N = 32
     mats = _mats(12, seed=5)
     entries, refs, keep = _build(mats, N, seed=2)
-    batch = smat.Batch(entries)
+    print(f"graph replays mode={mode}", flush=True)
+    batch = smat.Batch(entries, mode=mode)
     batch()
     torch.cuda.synchronize()
     eager = [C.cpu().numpy().copy() for (_, _, C, *_) in keep]
@@ -156,6 +159,47 @@ def test_batch_rejects_bad_plans():
                dict(in_dtype=smat.SMAT_BF16)]:
         bad = smat.SpmmPlan(A, 64, **kw)
         with pytest.raises(smat.SmatError):
-            smat.Batch([(good, good.operands(B, C)), (bad, bad.operands(B, C))])
-    with pytest.raises(smat.SmatError):  # default stream cannot be captured
-        smat.Batch([(good, good.operands(B, C))]).capture(0)
+            smat.Batch([(good, good.operands(B, C)), (bad, bad.operands(B, C))], mode=smat.SMAT_BATCH_PERSISTENT)
+    for mode in (smat.SMAT_BATCH_PERSISTENT, smat.SMAT_BATCH_BRANCHES):
+        with pytest.raises(smat.SmatError):  # default stream cannot be captured
+            smat.Batch([(good, good.operands(B, C))], mode=mode).capture(0)
+        split = smat.SpmmPlan(A, 64, max_blocks_per_item=2)  # split rows share the plan's scratch
+        with pytest.raises(smat.SmatError):
+            smat.Batch([(split, split.operands(B, C)), (split, split.operands(B, C))], mode=mode)
+    with pytest.raises(smat.SmatError):
+        smat.Batch([(good, good.operands(B, C))], mode=7)
+
+
+def test_batch_branches_take_any_plan():
+    """Branches run each matrix's own call, so plans the persistent kernel
+    cannot take (FP16 C, direct B gathers, BF16, reference-layout B) batch
+    too, with results identical to the single calls."""
+    _need_gpu()
+    import smat
+    M, K, rp, ci, v = gen.rmat(scale=10, edge_factor=8, seed=7)
+    N = 64
+    B = gen.dense_b(K, N, seed=3)
+    Bd = _dev(B.view(np.int16))
+    Bt = _dev(np.ascontiguousarray(B.T).view(np.int16))  # reference layout B[n*K + k]
+    A = smat.Bcsr(M, K, _dev(rp), _dev(ci), _dev(v.view(np.int16)), bk=16)
+    kws = [dict(out_dtype=smat.SMAT_F16), dict(b_staging=smat.SMAT_BSTAGE_DIRECT), dict(in_dtype=smat.SMAT_BF16),
+           dict(b_layout=smat.SMAT_B_REFERENCE), dict()]
+    entries, outs, singles = [], [], []
+    for kw in kws:
+        plan = smat.SpmmPlan(A, N, **kw)
+        f16 = kw.get("out_dtype") == smat.SMAT_F16
+        C = torch.full((M, N), float("nan"), dtype=torch.float16 if f16 else torch.float32, device="cuda")
+        Bx = Bt if kw.get("b_layout") == smat.SMAT_B_REFERENCE else Bd
+        ldb = K if kw.get("b_layout") == smat.SMAT_B_REFERENCE else N
+        plan.execute(Bx, C, ldb=ldb)
+        torch.cuda.synchronize()
+        singles.append(C.cpu().numpy().copy())
+        C.fill_(float("nan"))
+        entries.append((plan, plan.operands(Bx, C, ldb=ldb)))
+        outs.append(C)
+    batch = smat.Batch(entries, mode=smat.SMAT_BATCH_BRANCHES)
+    batch()
+    torch.cuda.synchronize()
+    for i, C in enumerate(outs):
+        assert np.array_equal(C.cpu().numpy(), singles[i]), i
+    batch.close()

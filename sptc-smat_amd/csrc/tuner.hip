@@ -190,46 +190,48 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 // scores and the index of the lowest available one.  Layer widths <= 64.
 constexpr int kMaxIn = 64, kMaxHidden = 64, kMaxCfg = 256;
 
-__global__ __launch_bounds__(256) void k_mlp_predict(
+// Inference: one workgroup per candidate configuration (blockIdx.x), the
+// three layers with one thread per output and a fixed summation order;
+// k_mlp_argmin then takes the first minimum over the available candidates.
+__global__ __launch_bounds__(64) void k_mlp_scores(
     const float* __restrict__ feats, int32_t n_feat, const float* __restrict__ cfg_enc, int32_t n_enc,
-    const uint8_t* __restrict__ avail, int32_t n_cfg, const float* __restrict__ mu, const float* __restrict__ sigma,
-    const float* __restrict__ W1, const float* __restrict__ b1, const float* __restrict__ W2,
-    const float* __restrict__ b2, const float* __restrict__ W3, const float* __restrict__ b3, int32_t hidden,
-    float* __restrict__ scores, int32_t* __restrict__ best) {
-  __shared__ float x[kMaxIn], h1[kMaxHidden], h2[kMaxHidden], sc[kMaxCfg];
+    const float* __restrict__ mu, const float* __restrict__ sigma, const float* __restrict__ W1,
+    const float* __restrict__ b1, const float* __restrict__ W2, const float* __restrict__ b2,
+    const float* __restrict__ W3, const float* __restrict__ b3, int32_t hidden, float* __restrict__ scores) {
+  __shared__ float x[kMaxIn], h1[kMaxHidden], h2[kMaxHidden];
+  const int c = blockIdx.x;
   const int n_in = n_feat + n_enc;
-  for (int c = 0; c < n_cfg; c++) {
-    for (int i = threadIdx.x; i < n_in; i += blockDim.x) {
-      const float raw = i < n_feat ? feats[i] : cfg_enc[c * n_enc + (i - n_feat)];
-      x[i] = (raw - mu[i]) / sigma[i];
-    }
-    __syncthreads();
-    for (int o = threadIdx.x; o < hidden; o += blockDim.x) {
-      float s = b1[o];
-      for (int i = 0; i < n_in; i++) s = fmaf(x[i], W1[o * n_in + i], s);
-      h1[o] = s > 0.f ? s : 0.f;
-    }
-    __syncthreads();
-    for (int o = threadIdx.x; o < hidden; o += blockDim.x) {
-      float s = b2[o];
-      for (int i = 0; i < hidden; i++) s = fmaf(h1[i], W2[o * hidden + i], s);
-      h2[o] = s > 0.f ? s : 0.f;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      float s = b3[0];
-      for (int i = 0; i < hidden; i++) s = fmaf(h2[i], W3[i], s);
-      sc[c] = s;
-      scores[c] = s;
-    }
-    __syncthreads();
+  for (int i = threadIdx.x; i < n_in; i += blockDim.x) {
+    const float raw = i < n_feat ? feats[i] : cfg_enc[c * n_enc + (i - n_feat)];
+    x[i] = (raw - mu[i]) / sigma[i];
   }
+  __syncthreads();
+  for (int o = threadIdx.x; o < hidden; o += blockDim.x) {
+    float s = b1[o];
+    for (int i = 0; i < n_in; i++) s = fmaf(x[i], W1[o * n_in + i], s);
+    h1[o] = s > 0.f ? s : 0.f;
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < hidden; o += blockDim.x) {
+    float s = b2[o];
+    for (int i = 0; i < hidden; i++) s = fmaf(h1[i], W2[o * hidden + i], s);
+    h2[o] = s > 0.f ? s : 0.f;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    int bi = -1;
-    for (int c = 0; c < n_cfg; c++)
-      if (avail[c] && (bi < 0 || sc[c] < sc[bi])) bi = c;
-    best[0] = bi;
+    float s = b3[0];
+    for (int i = 0; i < hidden; i++) s = fmaf(h2[i], W3[i], s);
+    scores[c] = s;
   }
+}
+
+__global__ void k_mlp_argmin(const float* __restrict__ scores, const uint8_t* __restrict__ avail, int32_t n_cfg,
+                             int32_t* __restrict__ best) {
+  if (threadIdx.x != 0) return;
+  int bi = -1;
+  for (int c = 0; c < n_cfg; c++)
+    if (avail[c] && (bi < 0 || scores[c] < scores[bi])) bi = c;
+  best[0] = bi;
 }
 
 }  // namespace
@@ -359,8 +361,10 @@ smat_status smat_tuner_predict(const float* feats, int32_t n_feat, const float* 
   SMAT_REQUIRE(n_feat + n_enc <= kMaxIn && hidden <= kMaxHidden && hidden > 0 && n_cfg > 0 && n_cfg <= kMaxCfg,
                "smat_tuner_predict: sizes exceed the kernel's limits (inputs %d, hidden %d, configs %d)",
                kMaxIn, kMaxHidden, kMaxCfg);
-  hipLaunchKernelGGL(k_mlp_predict, 1, 64, 0, smat::to_hip(stream), feats, n_feat, cfg_enc, n_enc, avail, n_cfg, mu,
-                     sigma, W1, b1, W2, b2, W3, b3, hidden, scores, best);
+  hipStream_t s = smat::to_hip(stream);
+  hipLaunchKernelGGL(k_mlp_scores, (unsigned)n_cfg, 64, 0, s, feats, n_feat, cfg_enc, n_enc, mu, sigma, W1, b1, W2, b2,
+                     W3, b3, hidden, scores);
+  hipLaunchKernelGGL(k_mlp_argmin, 1, 64, 0, s, scores, avail, n_cfg, best);
   SMAT_LAUNCH_CHECK();
   return SMAT_OK;
 }

@@ -441,10 +441,13 @@ __device__ __forceinline__ void lds_tile(const Item& it, int64_t n0, const Opera
       const int boff = row / BK;
       const int32_t c = __shfl(cid, boff);
       const bool ok = p + boff < pe && col_ok;
-      const int64_t off = ((int64_t)(c + row % BK) * ldb + ncol) * ES;
+      // 32-bit byte offset from the uniform base (the launcher checks that B,
+      // hi and lo arrays together, spans under 4 GiB): one VGPR per address
+      // and the SGPR-base load form
+      const uint32_t off = ((uint32_t)(c + row % BK) * (uint32_t)ldb + (uint32_t)ncol) * ES;
 #pragma unroll
       for (int h = 0; h < NARR; h++)
-        bb[h][i] = ok ? *reinterpret_cast<const uint4*>(B + off + h * op.b_lo * ES) : uint4{0, 0, 0, 0};
+        bb[h][i] = ok ? *reinterpret_cast<const uint4*>(B + (uint64_t)off + h * op.b_lo * ES) : uint4{0, 0, 0, 0};
     }
     load_a_any<BK, DT>(op.a, op.a_lo, p, pe, r16, grp, aa);
   };
@@ -1067,9 +1070,15 @@ void launch_mixed(const smat_plan_t* P, const uint16_t* v24, const uint16_t* i24
                      P->items.as<Item>(), P->n_items, ntiles, op, mx, order);
 }
 
+// LDS staging of row-major B: 16-byte chunks (N and ldb multiples of the
+// chunk, B 16-byte aligned) and 32-bit byte offsets (every B array the call
+// reads under 4 GiB).
 inline bool lds_aligned(const smat_plan_t* P, const Operands& op, int elems_per_chunk) {
+  const int es = P->opts.in_dtype == SMAT_FP8_E4M3 ? 1 : 2;
+  const int narr = P->opts.in_dtype == SMAT_SPLIT_F16 ? 2 : 1;
+  const double bytes = (double)P->A.K * (double)op.ldb * es * narr;
   return P->N % elems_per_chunk == 0 && op.ldb % elems_per_chunk == 0 &&
-         (reinterpret_cast<uintptr_t>(op.b) & 15) == 0;
+         (reinterpret_cast<uintptr_t>(op.b) & 15) == 0 && bytes < 4294967296.0;
 }
 
 template <int BK, int BLAYOUT, int DT, bool OUT_F16>
@@ -1394,7 +1403,9 @@ smat_status prepare_operands(smat_plan P, const smat_spmm_operands* ops, Operand
   const int chunk = dt == SMAT_FP8_E4M3 ? 16 : 8;  // elements per 16-byte B load
   const bool aligned = lds_aligned(P, op, chunk);
   if ((P->opts.b_staging == SMAT_BSTAGE_LDS && P->opts.b_layout == SMAT_B_ROW_MAJOR) || P->mixed || scaled)
-    SMAT_REQUIRE(aligned, "smat_spmm_execute: LDS staging needs N and ldb multiples of %d and a 16-byte aligned B",
+    SMAT_REQUIRE(aligned,
+                 "smat_spmm_execute: LDS staging needs N and ldb multiples of %d, a 16-byte aligned B and B "
+                 "arrays under 4 GiB",
                  chunk);
   if (scaled) {
     SMAT_REQUIRE(ops->a_scale && ops->b_scale, "smat_spmm_execute: FP8 and split operands need a_scale and b_scale");
@@ -1699,8 +1710,8 @@ smat_status smat_batch_create_ex(int32_t count, const smat_plan* plans, const sm
     BatchMat& m = mats[(size_t)pos];
     m.op = Operands{P->A.colidx, a, 0, nullptr, o.b, o.ldb, 0, nullptr, P->N, o.c, o.ldc, P->scratch.as<float>(),
                     o.c_rowmap};
-    SMAT_REQUIRE(lds_aligned(P, m.op, 8), "smat_batch_create: entry %d: N and ldb must be multiples of 8 and B "
-                                          "16-byte aligned", i);
+    SMAT_REQUIRE(lds_aligned(P, m.op, 8), "smat_batch_create: entry %d: N and ldb must be multiples of 8, B "
+                                          "16-byte aligned and under 4 GiB", i);
     m.items = P->items.as<Item>();
     m.ntiles = ntiles_of(P, P->nsub);
     m.kind = P->mixed ? 6 : bk_code(P->A.bk);

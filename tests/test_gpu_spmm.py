@@ -312,3 +312,39 @@ def test_n_tile_128_unaligned_b_and_restrictions():
     with pytest.raises(smat.SmatError):
         smat.SpmmPlan(A16, N, n_tile=128, kernel=smat.SMAT_KERNEL_MIXED24)
     A16.close()
+
+
+def test_b_over_4gib_takes_the_64bit_path():
+    """The LDS kernels address B with 32-bit byte offsets; a B of 4 GiB or
+    more must take the 64-bit direct-gather kernel.  Entries in B rows past
+    2^23 (byte offsets past 4 GiB at N = 256) give the oracle's product."""
+    _need_gpu()
+    import smat
+    N = 256
+    K = (1 << 23) + 4096  # B = K x 256 FP16 = 4.3 GB
+    M = 64
+    rng = np.random.default_rng(5)
+    cols_hi = np.sort(rng.choice(np.arange(K - 4096, K), size=48, replace=False))
+    cols_lo = np.sort(rng.choice(np.arange(0, 4096), size=16, replace=False))
+    rows = np.repeat(np.arange(M), 1)
+    ci = np.concatenate([cols_lo, cols_hi]).astype(np.int32)[:M]
+    rp = np.arange(M + 1, dtype=np.int32)  # one entry per row
+    v = np.float16(rng.uniform(-1, 1, M)).view(np.uint16)
+    print("B over 4 GiB: allocating", flush=True)
+    Bd = torch.empty((K, N), dtype=torch.float16, device="cuda")
+    Bd[:4096].uniform_(-1, 1)
+    Bd[4096:K - 4096].zero_()
+    Bd[K - 4096:].uniform_(-1, 1)
+    rows_used = ci.astype(np.int64)
+    Bh = Bd[torch.from_numpy(rows_used).cuda()].float().cpu().numpy().astype(np.float64)
+    R = v.view(np.float16).astype(np.float64)[:, None] * Bh  # row r = a_r * B[ci[r]]
+    A = _bcsr(M, K, rp, ci, v, bk=2)
+    plan = smat.SpmmPlan(A, N)
+    C = torch.full((M, N), float("nan"), dtype=torch.float32, device="cuda")
+    plan(Bd.view(torch.int16), C)
+    torch.cuda.synchronize()
+    assert _rel_fro(C.cpu().numpy().astype(np.float64), R) < 1e-6
+    plan.close()
+    A.close()
+    del Bd
+    torch.cuda.empty_cache()

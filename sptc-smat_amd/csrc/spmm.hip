@@ -152,20 +152,29 @@ __device__ inline uint32_t load_b(const uint16_t* __restrict__ B, int64_t ldb, i
 // for BK >= 8 that is one 16-byte piece of one block row, for smaller BK it
 // is 8/BK pieces of BK halves from consecutive blocks.  Blocks at or beyond
 // `pend` contribute zeros.
-template <int BK>
+// A operand of one MFMA step (k-slots 8*grp .. 8*grp+7 of row r16, from the
+// 32/BK blocks starting at block p).  OFF32: byte offsets in 32 bits from the
+// uniform base (the caller rebases `values` so that they fit).
+template <int BK, bool OFF32 = false>
 __device__ inline uint4 load_a(const uint16_t* __restrict__ values, int32_t p, int32_t pend, int r16,
                                int grp) {
   static_assert(BK == 1 || BK == 2 || BK == 4 || BK == 8 || BK == 16 || BK == 32, "block width");
+  const uint8_t* __restrict__ base = reinterpret_cast<const uint8_t*>(values);
+  // byte address of element e of block blk
+  auto at = [&](int32_t blk, int e) -> const uint8_t* {
+    if constexpr (OFF32) return base + (uint64_t)(((uint32_t)blk * (16u * BK) + (uint32_t)e) * 2u);
+    else return base + ((int64_t)blk * 16 * BK + e) * 2;
+  };
   if constexpr (BK >= 8) {
     const int32_t blk = p + (8 * grp) / BK;
     if (blk >= pend) return uint4{0, 0, 0, 0};
-    return *reinterpret_cast<const uint4*>(values + (int64_t)blk * 16 * BK + r16 * BK + (8 * grp) % BK);
+    return *reinterpret_cast<const uint4*>(at(blk, r16 * BK + (8 * grp) % BK));
   } else if constexpr (BK == 4) {
     uint2 w[2];
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       const int32_t blk = p + 2 * grp + q;
-      w[q] = blk < pend ? *reinterpret_cast<const uint2*>(values + (int64_t)blk * 64 + r16 * 4) : uint2{0, 0};
+      w[q] = blk < pend ? *reinterpret_cast<const uint2*>(at(blk, r16 * 4)) : uint2{0, 0};
     }
     return uint4{w[0].x, w[0].y, w[1].x, w[1].y};
   } else if constexpr (BK == 2) {
@@ -173,7 +182,7 @@ __device__ inline uint4 load_a(const uint16_t* __restrict__ values, int32_t p, i
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int32_t blk = p + 4 * grp + q;
-      w[q] = blk < pend ? *reinterpret_cast<const uint32_t*>(values + (int64_t)blk * 32 + r16 * 2) : 0u;
+      w[q] = blk < pend ? *reinterpret_cast<const uint32_t*>(at(blk, r16 * 2)) : 0u;
     }
     return uint4{w[0], w[1], w[2], w[3]};
   } else {
@@ -181,8 +190,8 @@ __device__ inline uint4 load_a(const uint16_t* __restrict__ values, int32_t p, i
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int32_t b0 = p + 8 * grp + 2 * q, b1 = b0 + 1;
-      const uint32_t lo = b0 < pend ? values[(int64_t)b0 * 16 + r16] : 0u;
-      const uint32_t hi = b1 < pend ? values[(int64_t)b1 * 16 + r16] : 0u;
+      const uint32_t lo = b0 < pend ? *reinterpret_cast<const uint16_t*>(at(b0, r16)) : 0u;
+      const uint32_t hi = b1 < pend ? *reinterpret_cast<const uint16_t*>(at(b1, r16)) : 0u;
       w[q] = lo | (hi << 16);
     }
     return uint4{w[0], w[1], w[2], w[3]};
@@ -340,11 +349,18 @@ __device__ inline uint2 load_a8(const uint8_t* __restrict__ values, int32_t p, i
 
 // A operand of one step for every codec (hi and, for split, lo).
 template <int BK, int DT>
-__device__ inline void load_a_any(const void* __restrict__ values, int64_t lo_off, int32_t p, int32_t pend,
-                                  int r16, int grp, uint4 (&a)[Codec<DT>::NARR]) {
+__device__ inline void load_a_any(const void* __restrict__ values, int64_t lo_off, int32_t pb, int32_t p,
+                                  int32_t pend, int r16, int grp, uint4 (&a)[Codec<DT>::NARR]) {
   if constexpr (DT == 2) {
     const uint2 v = load_a8<BK>(static_cast<const uint8_t*>(values), p, pend, r16, grp);
     a[0] = uint4{v.x, v.y, 0, 0};
+  } else if constexpr (BK == 1 || BK >= 8) {
+    // 32-bit offsets from the item's first block (an item spans at most
+    // 65536 blocks, 64 MiB at bk = 32): measured faster for bk = 1 and
+    // bk >= 8, slower for 2 and 4
+    const uint16_t* v = static_cast<const uint16_t*>(values) + (int64_t)pb * 16 * BK;
+    a[0] = load_a<BK, true>(v, p - pb, pend - pb, r16, grp);
+    if constexpr (DT == 3) a[Codec<DT>::NARR - 1] = load_a<BK, true>(v + lo_off, p - pb, pend - pb, r16, grp);
   } else {
     const uint16_t* v = static_cast<const uint16_t*>(values);
     a[0] = load_a<BK>(v, p, pend, r16, grp);
@@ -449,7 +465,7 @@ __device__ __forceinline__ void lds_tile(const Item& it, int64_t n0, const Opera
       for (int h = 0; h < NARR; h++)
         bb[h][i] = ok ? *reinterpret_cast<const uint4*>(B + (uint64_t)off + h * op.b_lo * ES) : uint4{0, 0, 0, 0};
     }
-    load_a_any<BK, DT>(op.a, op.a_lo, p, pe, r16, grp, aa);
+    load_a_any<BK, DT>(op.a, op.a_lo, it.pbeg, p, pe, r16, grp, aa);
   };
   auto consume = [&](const uint4 (&bb)[NARR][NLD]) {
 #pragma unroll
@@ -1367,6 +1383,9 @@ smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat
     }
     S = (int32_t)(s_auto < s_min ? s_min : (s_auto > 65536 ? 65536 : s_auto));
   }
+  // an item spans at most 65536 blocks (the LDS kernels address an item's A
+  // values with 32-bit offsets from its first block)
+  if (S > 65536) S = 65536;
   smat_status st;
   if (P->mixed)
     st = build_mixed(P, (S + 1) / 2, s);  // S in MFMA steps (a dense pair or a 2:4 quad each)

@@ -285,8 +285,8 @@ typedef struct {
   int32_t kernel;          /* SMAT_KERNEL_* */
   int32_t out_dtype;       /* SMAT_F32 or SMAT_F16 */
   int32_t b_layout;        /* SMAT_B_ROW_MAJOR or SMAT_B_REFERENCE */
-  int32_t n_tile;          /* columns per wave: 16, 32, 64 or 128 (128: dense kernel, FP16 / BF16,
-                              row-major B staged through LDS); 0 = auto */
+  int32_t n_tile;          /* columns per wave: 16, 32, 64 or 128 (128: dense kernel, FP16 / BF16 /
+                              FP8, row-major B staged through LDS); 0 = auto (FP8: 128 when N >= 128) */
   int32_t max_blocks_per_item; /* split block rows longer than this (at most 65536); 0 = auto */
   int32_t b_staging;       /* SMAT_BSTAGE_* (row-major B only) */
   int32_t in_dtype;        /* SMAT_F16 (0 = F16), SMAT_BF16, SMAT_FP8_E4M3 or SMAT_SPLIT_F16 */

@@ -1129,11 +1129,13 @@ void dispatch_layout(const smat_plan_t* P, const Operands& op, hipStream_t s) {
 }
 
 // FP8 and split codecs: row-major B through LDS only, FP32 C.  FP8 tiles are
-// at least 32 columns wide (a 16-byte load then spans at most 32 B rows).
+// at least 32 columns wide (a 16-byte load then spans at most 32 B rows) and
+// 128 by default (each B row segment a whole 128-byte line).
 template <int BK, int DT>
 void dispatch_scaled(const smat_plan_t* P, const Operands& op, hipStream_t s) {
   if (DT == 2) {
-    if (P->nsub == 4) launch_lds<BK, 4, 2, false>(P, op, s);
+    if (P->nsub == 8) launch_lds<BK, 8, 2, false>(P, op, s);
+    else if (P->nsub == 4) launch_lds<BK, 4, 2, false>(P, op, s);
     else launch_lds<BK, 2, 2, false>(P, op, s);
   } else {
     switch (P->nsub) {
@@ -1325,9 +1327,10 @@ smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat
   SMAT_REQUIRE(o.n_tile == 0 || o.n_tile == 16 || o.n_tile == 32 || o.n_tile == 64 || o.n_tile == 128,
                "smat_spmm_plan_create: n_tile must be 0, 16, 32, 64 or 128");
   if (o.n_tile == 128)
-    SMAT_REQUIRE(o.kernel != SMAT_KERNEL_MIXED24 && (o.in_dtype == SMAT_F16 || o.in_dtype == SMAT_BF16) &&
+    SMAT_REQUIRE(o.kernel != SMAT_KERNEL_MIXED24 &&
+                     (o.in_dtype == SMAT_F16 || o.in_dtype == SMAT_BF16 || o.in_dtype == SMAT_FP8_E4M3) &&
                      o.b_layout == SMAT_B_ROW_MAJOR && o.b_staging != SMAT_BSTAGE_DIRECT,
-                 "smat_spmm_plan_create: n_tile 128 is for the dense kernel with FP16 / BF16 row-major B "
+                 "smat_spmm_plan_create: n_tile 128 is for the dense kernel with FP16 / BF16 / FP8 row-major B "
                  "staged through LDS");
   SMAT_REQUIRE(o.b_staging >= SMAT_BSTAGE_AUTO && o.b_staging <= SMAT_BSTAGE_LDS,
                "smat_spmm_plan_create: unknown b_staging %d", o.b_staging);
@@ -1335,7 +1338,12 @@ smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat
                "smat_spmm_plan_create: unknown schedule %d", o.schedule);
   hipStream_t s = smat::to_hip(stream);
   int nt = o.n_tile ? o.n_tile : (N >= 64 ? 64 : (N > 16 ? 32 : 16));
-  if (o.in_dtype == SMAT_FP8_E4M3 && nt < 32) nt = 32;  // FP8 tiles are at least 32 columns wide
+  if (o.in_dtype == SMAT_FP8_E4M3) {
+    // FP8 tiles are at least 32 columns wide; by default 128 (a whole
+    // 128-byte line of each B row) when N allows
+    if (nt < 32) nt = 32;
+    if (o.n_tile == 0 && N >= 128) nt = 128;
+  }
   if (o.schedule == SMAT_SCHED_AUTO) {
     // skewed block rows (the longest over 8x the mean) go longest first,
     // with tile affinity when every N tile is shared by 2+ XCDs (with more

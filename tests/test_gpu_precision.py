@@ -118,7 +118,10 @@ def test_spmm_packed_vs_oracle(bk, dt):
     sign = 0x7F if dt == of.FP8 else 0x7FFF  # clear the sign bits: |A| |B|
     Rabs = of.spmm_scaled_f64(bk, brp, bci, dt, ha & sign, sa.cpu().numpy() if scaled else None, K, N, hb & sign,
                               sb.cpu().numpy() if scaled else None)
-    for n_tile, split in [(0, 0), (32, 0), (16, 64), (64, 32)]:
+    cases = [(0, 0), (32, 0), (16, 64), (64, 32)]
+    if dt != of.SPLIT:
+        cases += [(128, 0), (128, 16)]  # 128-column tiles (half empty at N = 64)
+    for n_tile, split in cases:
         print(f"spmm packed bk={bk} dt={dt} n_tile={n_tile} split={split}", flush=True)
         plan = smat.SpmmPlan(A, N, in_dtype=dt, n_tile=n_tile, max_blocks_per_item=split)
         C = torch.full((M, N), float("nan"), dtype=torch.float32, device="cuda")
@@ -189,3 +192,32 @@ def test_scaled_modes_reject_bad_options():
         plan(B, C)
     with pytest.raises(smat.SmatError):
         plan.execute(B, C)
+
+
+@pytest.mark.parametrize("bk", [1, 2, 16])
+@pytest.mark.parametrize("N", [160, 256])
+def test_fp8_default_128_column_tiles(bk, N):
+    """FP8 plans default to 128-column tiles when N >= 128 (a whole 128-byte
+    line per B row segment): same bound against the oracle's FP64 product of
+    the packed operands as 64-column tiles, every column written."""
+    _need_gpu()
+    import smat
+    A, M, K, rp, ci, v32, x, brp, bci, bv = _matrix(bk, seed=90 + bk)
+    rng = np.random.default_rng(N + bk)
+    Bf = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    a, sa = smat.pack_a(A, x, of.FP8)
+    b, sb = smat.pack_b(_dev(Bf), of.FP8)
+    torch.cuda.synchronize()
+    ha = a.cpu().numpy().view(np.uint8)
+    hb = b.cpu().numpy().view(np.uint8)
+    R = of.spmm_scaled_f64(bk, brp, bci, of.FP8, ha, sa.cpu().numpy(), K, N, hb, sb.cpu().numpy())
+    for n_tile in (0, 64):
+        print(f"fp8 bk={bk} N={N} n_tile={n_tile}", flush=True)
+        plan = smat.SpmmPlan(A, N, in_dtype=of.FP8, n_tile=n_tile)
+        C = torch.full((M, N), float("nan"), dtype=torch.float32, device="cuda")
+        plan.execute(b, C, a_values=a, a_scale=sa, b_scale=sb)
+        torch.cuda.synchronize()
+        Cn = C.cpu().numpy().astype(np.float64)
+        assert np.all(np.isfinite(Cn))
+        assert _rel_fro(Cn, R) < 2e-4, n_tile
+        plan.close()

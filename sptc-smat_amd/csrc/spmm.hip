@@ -1,4 +1,5 @@
-// BCSR x dense SpMM on gfx950 matrix cores (FP16 or BF16 in, FP32 accumulate).
+// BCSR x dense SpMM on gfx950 matrix cores (FP16, BF16, FP8 e4m3 or split
+// FP16 in, FP32 accumulate), its planner and work schedules, and batches.
 //
 // Reference semantics: mmaOBTKernelSparse (src/cuda_hgemm/src/mma/mmaOBT.cu:18-158)
 // computes C = A*B over the blocks of one block row in BCSR order, and

@@ -315,12 +315,29 @@ __device__ inline uint2 tr8(const uint8_t* addr) {
 
 // FP8 counterpart of load_a: 8 bytes (k-slots 8*grp .. 8*grp+7 of row r16)
 // gathered from the 32/BK blocks of the step starting at block p.
-template <int BK>
+template <int BK, bool OFF32 = false>
 __device__ inline uint2 load_a8(const uint8_t* __restrict__ values, int32_t p, int32_t pend, int r16, int grp) {
-  if constexpr (BK >= 8) {
-    const int32_t blk = p + (8 * grp) / BK;
-    if (blk >= pend) return uint2{0, 0};
-    return *reinterpret_cast<const uint2*>(values + (int64_t)blk * 16 * BK + r16 * BK + (8 * grp) % BK);
+  if constexpr (BK >= 8 || (BK == 1 && OFF32)) {
+    // byte address of element e of block blk (OFF32: 32-bit offset from the
+    // uniform base, which the caller rebases so that it fits)
+    auto at = [&](int32_t blk, int e) -> const uint8_t* {
+      if constexpr (OFF32) return values + (uint64_t)((uint32_t)blk * (16u * BK) + (uint32_t)e);
+      else return values + (int64_t)blk * 16 * BK + e;
+    };
+    if constexpr (BK >= 8) {
+      const int32_t blk = p + (8 * grp) / BK;
+      if (blk >= pend) return uint2{0, 0};
+      return *reinterpret_cast<const uint2*>(at(blk, r16 * BK + (8 * grp) % BK));
+    } else {
+      uint32_t w[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int32_t blk = p + 8 * grp + q;
+        w[q] = blk < pend ? *at(blk, r16) : 0u;
+      }
+      return uint2{w[0] | (w[1] << 8) | (w[2] << 16) | (w[3] << 24),
+                   w[4] | (w[5] << 8) | (w[6] << 16) | (w[7] << 24)};
+    }
   } else if constexpr (BK == 4) {
     uint32_t w[2];
 #pragma unroll
@@ -352,9 +369,13 @@ __device__ inline uint2 load_a8(const uint8_t* __restrict__ values, int32_t p, i
 template <int BK, int DT>
 __device__ inline void load_a_any(const void* __restrict__ values, int64_t lo_off, int32_t pb, int32_t p,
                                   int32_t pend, int r16, int grp, uint4 (&a)[Codec<DT>::NARR]) {
-  if constexpr (DT == 2) {
-    const uint2 v = load_a8<BK>(static_cast<const uint8_t*>(values), p, pend, r16, grp);
-    a[0] = uint4{v.x, v.y, 0, 0};
+  if constexpr (DT == 2 && (BK == 1 || BK >= 8)) {
+    const uint8_t* v = static_cast<const uint8_t*>(values) + (int64_t)pb * 16 * BK;  // item's first block
+    const uint2 w = load_a8<BK, true>(v, p - pb, pend - pb, r16, grp);
+    a[0] = uint4{w.x, w.y, 0, 0};
+  } else if constexpr (DT == 2) {
+    const uint2 w = load_a8<BK>(static_cast<const uint8_t*>(values), p, pend, r16, grp);
+    a[0] = uint4{w.x, w.y, 0, 0};
   } else if constexpr (BK == 1 || BK >= 8) {
     // 32-bit offsets from the item's first block (an item spans at most
     // 65536 blocks, 64 MiB at bk = 32): measured faster for bk = 1 and

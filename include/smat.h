@@ -523,6 +523,41 @@ void preprocessing_mmaSTKernel(uint16_t* bcsrValuesA, char* metadata, uint16_t* 
                                size_t M, size_t N, size_t K, size_t nonzeroBlocks,
                                int* blockInfo, int* relativeBlockIndexMapping);
 
+/*
+ * HGEMM_FUNC_SPARSE2 (main.cu:25-28) — replaces mmaOBTKernel_large
+ * (src/cuda_hgemm/src/mma/mmaOBT_large.cu:160-170), which is a copy of
+ * mmaOBTKernel still on 16x16 blocks: same behaviour as mmaOBTKernel.
+ */
+void mmaOBTKernel_large(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdxA, uint16_t* B,
+                        uint16_t* C, size_t M, size_t N, size_t K, size_t nonzeroBlocks,
+                        int* blockInfo, int* relativeBlockIndexMapping);
+
+/*
+ * HGEMM_FUNC_SPARSE24_2 (main.cu:20-24) — replaces mmaOBTSKernel_large
+ * (src/cuda_hgemm/src/mma/mmaOBTS_large.cu:19-266), the reference's evaluated
+ * kernel (main.cu:306-308): mixed BCSR SpMM on 16x32 blocks (K a multiple of
+ * 32; blockInfo / relativeBlockIndexMapping over the k=32 grid).  blockInfo 2
+ * blocks are read from bcsrValuesA (16 x 32 row-major), blockInfo 1 blocks
+ * from the (sparseMatrixA, metadata) pair of preprocessing_mmaSTKernel_large.
+ */
+void mmaOBTSKernel_large(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdxA, char* metadata,
+                         uint16_t* sparseMatrixA, uint16_t* B, uint16_t* C, size_t M, size_t N,
+                         size_t K, size_t nonzeroBlocks, int* blockInfo,
+                         int* relativeBlockIndexMapping);
+
+/*
+ * Replaces preprocessing_mmaSTKernel_large (main.cu:64-68, mmaST_large.cu:
+ * 324-335): compresses every blockInfo==1 16x32 block.  The reference's k=32
+ * producer and consumer disagree on metadata byte offsets (mmaST_large.cu:
+ * 143-146 vs mmaOBTS_large.cu:111-115), so no reference byte layout exists;
+ * each block is stored as its two 16x16 halves (columns 0-15, 16-31) in the
+ * SMAT_24_REFERENCE_K16 layout: 256 halves of sparseMatrixA and 64 bytes of
+ * metadata per block (the sizes the reference allocates, tester.h:318-320).
+ */
+void preprocessing_mmaSTKernel_large(uint16_t* bcsrValuesA, char* metadata, uint16_t* sparseMatrixA,
+                                     size_t M, size_t N, size_t K, size_t nonzeroBlocks,
+                                     int* blockInfo, int* relativeBlockIndexMapping);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,8 +35,11 @@ __device__ inline smat_h8 expand_ref24(const uint16_t* __restrict__ sp4, uint8_t
   return __builtin_bit_cast(smat_h8, *reinterpret_cast<uint4*>(out));
 }
 
-// C[M x N] (FP16, row-major, ldc = N) = A * B with B[n*K + k].
-template <bool MIXED>
+// C[M x N] (FP16, row-major, ldc = N) = A * B with B[n*K + k].  BKW = 16:
+// 16x16 blocks, two per MFMA step; BKW = 32: 16x32 blocks (the reference's
+// k=32 "large" BCSR), one per step, whose 2:4 form is two k=16 halves (see
+// preprocessing_mmaSTKernel_large).
+template <bool MIXED, int BKW>
 __global__ __launch_bounds__(256) void k_ref_spmm(const uint16_t* __restrict__ vals,
                                                   const int* __restrict__ rowptr,
                                                   const int* __restrict__ colidx,
@@ -52,31 +55,37 @@ __global__ __launch_bounds__(256) void k_ref_spmm(const uint16_t* __restrict__ v
   const int64_t brow = task / ntiles;
   if (brow >= M / 16) return;
   const int64_t n0 = (task % ntiles) * 16 * kNsub;
-  const int64_t ncr = K / 16;
+  const int64_t ncr = K / BKW;
   const int r16 = lane & 15, grp = lane >> 4, sub = lane >> 5, half = grp & 1;
+  constexpr int PER = 32 / BKW;  // blocks per MFMA step
 
   smat_f4 acc[kNsub];
 #pragma unroll
   for (int j = 0; j < kNsub; j++) acc[j] = smat_f4{0.f, 0.f, 0.f, 0.f};
 
   const int pb = rowptr[brow], pe = rowptr[brow + 1];
-  for (int p = pb; p < pe; p += 2) {
-    const int blk = p + sub;
+  for (int p = pb; p < pe; p += PER) {
+    const int blk = BKW == 16 ? p + sub : p;
     const bool valid = blk < pe;
     smat_h8 a = smat_h8{0, 0, 0, 0, 0, 0, 0, 0};
     int64_t c0 = 0;
     if (valid) {
       const int col = colidx[blk];
-      const int64_t g = brow * ncr + col / 16;
+      const int64_t g = brow * ncr + col / BKW;
       const int64_t rel = relMap[g];
       const int info = MIXED ? blockInfo[g] : 2;
+      // k-slots of this lane: columns 8*half .. of block `blk` (BKW = 16), or
+      // columns 8*grp .. of the 16x32 block, i.e. of its 16-column half grp/2
+      const int64_t sb = BKW == 16 ? rel : rel * 2 + (grp >> 1);  // 16x16 (sub-)block of the 2:4 arrays
       if (info == 1) {
         const int q = r16 * 2 + half;
-        a = expand_ref24(sparseA + rel * 128 + q * 4, meta[rel * 32 + q]);
-      } else {
+        a = expand_ref24(sparseA + sb * 128 + q * 4, meta[sb * 32 + q]);
+      } else if (BKW == 16) {
         a = __builtin_bit_cast(smat_h8, *reinterpret_cast<const uint4*>(vals + rel * 256 + r16 * 16 + half * 8));
+      } else {
+        a = __builtin_bit_cast(smat_h8, *reinterpret_cast<const uint4*>(vals + rel * 512 + r16 * 32 + grp * 8));
       }
-      c0 = col + half * 8;
+      c0 = col + (BKW == 16 ? half * 8 : grp * 8);
     }
 #pragma unroll
     for (int j = 0; j < kNsub; j++) {
@@ -106,14 +115,33 @@ __global__ void k_gather_tags(int64_t ncells, const int* __restrict__ info, cons
   if (r >= 0 && r < nnzb) tags_by_block[r] = info[g];
 }
 
-bool check_ref_args(const void* a, const void* b, const void* c, size_t M, size_t N, size_t K, const char* who) {
+// 16x32 blocks -> pairs of 16x16 blocks (columns 0-15, 16-31), tags copied
+// to both halves: the k=32 compressor runs the k=16 one on the halves.
+__global__ void k_split32(int64_t nnzb, const uint16_t* __restrict__ vals, const int32_t* __restrict__ tags,
+                          uint16_t* __restrict__ halves, int32_t* __restrict__ tags2) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (block, row)
+  if (t >= nnzb * 16) return;
+  const int64_t b = t / 16;
+  const int r = (int)(t % 16);
+  const uint4* src = reinterpret_cast<const uint4*>(vals + b * 512 + r * 32);
+  uint4* d0 = reinterpret_cast<uint4*>(halves + (2 * b) * 256 + r * 16);
+  uint4* d1 = reinterpret_cast<uint4*>(halves + (2 * b + 1) * 256 + r * 16);
+  d0[0] = src[0];
+  d0[1] = src[1];
+  d1[0] = src[2];
+  d1[1] = src[3];
+  if (r == 0) tags2[2 * b] = tags2[2 * b + 1] = tags[b];
+}
+
+bool check_ref_args(const void* a, const void* b, const void* c, size_t M, size_t N, size_t K, const char* who,
+                    size_t kw = 16) {
   if (!a || !b || !c) {
     smat::set_error(SMAT_ERR_INVALID, "%s: null pointer", who);
     return false;
   }
-  if (M % 16 || K % 16 || N == 0) {
-    smat::set_error(SMAT_ERR_INVALID, "%s: M and K must be multiples of 16, N > 0 (M=%zu N=%zu K=%zu)", who,
-                    M, N, K);
+  if (M % 16 || K % kw || N == 0) {
+    smat::set_error(SMAT_ERR_INVALID, "%s: M must be a multiple of 16, K of %zu, N > 0 (M=%zu N=%zu K=%zu)", who,
+                    kw, M, N, K);
     return false;
   }
   return true;
@@ -135,7 +163,7 @@ void mmaOBTKernel(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdxA, uin
   }
   const int64_t ntiles = ((int64_t)N + 63) / 64;
   const int64_t tasks = (int64_t)(M / 16) * ntiles;
-  hipLaunchKernelGGL(k_ref_spmm<false>, (unsigned)((tasks + kWaves - 1) / kWaves), 64 * kWaves, 0, 0,
+  hipLaunchKernelGGL((k_ref_spmm<false, 16>), (unsigned)((tasks + kWaves - 1) / kWaves), 64 * kWaves, 0, 0,
                      bcsrValuesA, bcsrRowPtrA, bcsrColIdxA, nullptr, nullptr, B,
                      reinterpret_cast<_Float16*>(C), (int64_t)M, (int64_t)N, (int64_t)K, nullptr,
                      relativeBlockIndexMapping);
@@ -154,7 +182,7 @@ void mmaOBTSKernel(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdxA, ch
   }
   const int64_t ntiles = ((int64_t)N + 63) / 64;
   const int64_t tasks = (int64_t)(M / 16) * ntiles;
-  hipLaunchKernelGGL(k_ref_spmm<true>, (unsigned)((tasks + kWaves - 1) / kWaves), 64 * kWaves, 0, 0,
+  hipLaunchKernelGGL((k_ref_spmm<true, 16>), (unsigned)((tasks + kWaves - 1) / kWaves), 64 * kWaves, 0, 0,
                      bcsrValuesA, bcsrRowPtrA, bcsrColIdxA, reinterpret_cast<const uint8_t*>(metadata),
                      sparseMatrixA, B, reinterpret_cast<_Float16*>(C), (int64_t)M, (int64_t)N, (int64_t)K,
                      blockInfo, relativeBlockIndexMapping);
@@ -191,6 +219,86 @@ void preprocessing_mmaSTKernel(uint16_t* bcsrValuesA, char* metadata, uint16_t* 
   if (over)
     smat::set_error(SMAT_ERR_NOT_24,
                     "preprocessing_mmaSTKernel: a blockInfo==1 block holds a 1x4 group with more than two "
+                    "nonzeros; the extra values were dropped");
+}
+
+// The reference's mmaOBTKernel_large (mmaOBT_large.cu:160-170) is a copy of
+// mmaOBTKernel that still works on 16x16 blocks (MMA_K 16), so this is too.
+void mmaOBTKernel_large(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdxA, uint16_t* B, uint16_t* C,
+                        size_t M, size_t N, size_t K, size_t nonzeroBlocks, int* blockInfo,
+                        int* relativeBlockIndexMapping) {
+  mmaOBTKernel(bcsrValuesA, bcsrRowPtrA, bcsrColIdxA, B, C, M, N, K, nonzeroBlocks, blockInfo,
+               relativeBlockIndexMapping);
+}
+
+// Mixed 16x32 BCSR SpMM (mmaOBTS_large.cu:256-266): blockInfo 2 blocks from
+// bcsrValuesA (16 x 32 row-major), blockInfo 1 blocks from the compressed
+// pair written by preprocessing_mmaSTKernel_large.
+void mmaOBTSKernel_large(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdxA, char* metadata,
+                         uint16_t* sparseMatrixA, uint16_t* B, uint16_t* C, size_t M, size_t N, size_t K,
+                         size_t nonzeroBlocks, int* blockInfo, int* relativeBlockIndexMapping) {
+  (void)nonzeroBlocks;
+  if (!check_ref_args(bcsrRowPtrA, B, C, M, N, K, "mmaOBTSKernel_large", 32)) return;
+  if (!relativeBlockIndexMapping || !blockInfo || !bcsrColIdxA || !bcsrValuesA || !metadata ||
+      !sparseMatrixA) {
+    smat::set_error(SMAT_ERR_INVALID, "mmaOBTSKernel_large: null BCSR or 2:4 array");
+    return;
+  }
+  const int64_t ntiles = ((int64_t)N + 63) / 64;
+  const int64_t tasks = (int64_t)(M / 16) * ntiles;
+  hipLaunchKernelGGL((k_ref_spmm<true, 32>), (unsigned)((tasks + kWaves - 1) / kWaves), 64 * kWaves, 0, 0,
+                     bcsrValuesA, bcsrRowPtrA, bcsrColIdxA, reinterpret_cast<const uint8_t*>(metadata),
+                     sparseMatrixA, B, reinterpret_cast<_Float16*>(C), (int64_t)M, (int64_t)N, (int64_t)K,
+                     blockInfo, relativeBlockIndexMapping);
+  if (hipGetLastError() != hipSuccess) smat::set_error(SMAT_ERR_HIP, "mmaOBTSKernel_large: launch failed");
+}
+
+// k=32 compressor (replaces mmaST_large.cu:324-335).  The reference's k=32
+// producer and consumer disagree on the metadata byte offsets
+// (mmaST_large.cu:143-146 writes bytes lane, lane+1; mmaOBTS_large.cu:111-115
+// reads 2*lane, 2*lane+1), so there is no byte layout to match; here each
+// 16x32 block is two consecutive 16x16 halves (columns 0-15, 16-31) in the
+// k=16 layout: sparseMatrixA holds 256 halves and metadata 64 bytes per
+// block, the same sizes the reference allocates (tester.h:318-320).
+void preprocessing_mmaSTKernel_large(uint16_t* bcsrValuesA, char* metadata, uint16_t* sparseMatrixA, size_t M,
+                                     size_t N, size_t K, size_t nonzeroBlocks, int* blockInfo,
+                                     int* relativeBlockIndexMapping) {
+  (void)N;
+  if (!bcsrValuesA || !metadata || !sparseMatrixA || !blockInfo || !relativeBlockIndexMapping) {
+    smat::set_error(SMAT_ERR_INVALID, "preprocessing_mmaSTKernel_large: null pointer");
+    return;
+  }
+  if (M % 16 || K % 32) {
+    smat::set_error(SMAT_ERR_INVALID, "preprocessing_mmaSTKernel_large: M must be a multiple of 16 and K of 32");
+    return;
+  }
+  const int64_t nnzb = (int64_t)nonzeroBlocks;
+  const int64_t ncells = (int64_t)(M / 16) * (int64_t)(K / 32);
+  smat::DevBuf tags, tags2, halves;
+  if (tags.alloc((nnzb + 1) * 4) != SMAT_OK || tags2.alloc((2 * nnzb + 1) * 4) != SMAT_OK ||
+      halves.alloc((2 * nnzb + 1) * 512) != SMAT_OK)
+    return;
+  if (hipMemsetAsync(tags.p, 0, (nnzb + 1) * 4, 0) != hipSuccess) {
+    smat::set_error(SMAT_ERR_HIP, "preprocessing_mmaSTKernel_large: memset failed");
+    return;
+  }
+  if (ncells > 0)
+    hipLaunchKernelGGL(k_gather_tags, (unsigned)((ncells + 255) / 256), 256, 0, 0, ncells, blockInfo,
+                       relativeBlockIndexMapping, tags.as<int32_t>(), nnzb);
+  if (nnzb > 0)
+    hipLaunchKernelGGL(k_split32, (unsigned)((nnzb * 16 + 255) / 256), 256, 0, 0, nnzb, bcsrValuesA,
+                       tags.as<int32_t>(), halves.as<uint16_t>(), tags2.as<int32_t>());
+  if (hipGetLastError() != hipSuccess) {
+    smat::set_error(SMAT_ERR_HIP, "preprocessing_mmaSTKernel_large: launch failed");
+    return;
+  }
+  bool over = false;
+  if (smat::compress_ref16(2 * nnzb, halves.as<uint16_t>(), tags2.as<int32_t>(), sparseMatrixA,
+                           reinterpret_cast<uint8_t*>(metadata), 0, &over) != SMAT_OK)
+    return;
+  if (over)
+    smat::set_error(SMAT_ERR_NOT_24,
+                    "preprocessing_mmaSTKernel_large: a blockInfo==1 block holds a 1x4 group with more than two "
                     "nonzeros; the extra values were dropped");
 }
 

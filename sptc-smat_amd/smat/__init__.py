@@ -149,6 +149,9 @@ SIGNATURES = {
     "mmaOBTKernel": (None, [_p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
     "mmaOBTSKernel": (None, [_p, _p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
     "preprocessing_mmaSTKernel": (None, [_p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
+    "mmaOBTKernel_large": (None, [_p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
+    "mmaOBTSKernel_large": (None, [_p, _p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
+    "preprocessing_mmaSTKernel_large": (None, [_p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
 }
 
 _lib = None

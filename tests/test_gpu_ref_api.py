@@ -212,3 +212,89 @@ def test_dropin_reports_bad_arguments():
     smat.lib().smat_clear_error()
     smat.lib().mmaOBTKernel(None, None, None, None, None, 16, 8, 16, 0, None, None)
     assert smat.lib().smat_last_status() == smat.SMAT_ERR_INVALID
+
+
+def mixed_matrix_32(n=1024, tiles=200, seed=83):
+    """16x32 tiles (aligned to the k=32 block grid): half pruned to exact
+    2:4, half dense."""
+    rng = np.random.default_rng(seed)
+    nbr, nbc = n // 16, n // 32
+    pos = rng.choice(nbr * nbc, size=tiles, replace=False)
+    dense = np.zeros((n, n), np.float16)
+    for t, f in enumerate(pos):
+        br, bc = divmod(int(f), nbc)
+        blk = rng.uniform(-1, 1, (16, 32)).astype(np.float16)
+        blk[blk == 0] = 0.5
+        if t % 2 == 0:
+            blk = gen.prune_2of4(blk)
+        dense[br * 16:(br + 1) * 16, bc * 32:(bc + 1) * 32] = blk
+    rows, cols = np.nonzero(dense)
+    rp = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=n))]).astype(np.int32)
+    return n, n, rp, cols.astype(np.int32), dense[rows, cols].view(np.uint16)
+
+
+@pytest.mark.parametrize("N", [64, 40])
+def test_mmaOBTSKernel_large_dropin_mixed(N):
+    """k=32 drop-ins: preprocessing_mmaSTKernel_large compresses the 2:4
+    16x32 blocks, mmaOBTSKernel_large reads them from the compressed pair
+    (their dense copies are zeroed first) and the dense ones from
+    bcsrValuesA; FP16 C against the FP64 product."""
+    _need_gpu()
+    import smat
+    M, K, rp, ci, v = mixed_matrix_32(seed=93 + N)
+    A = _bcsr(M, K, rp, ci, v, bk=32, rule=smat.SMAT_TAG_EXACT24)
+    info, rel = A.grid_maps()
+    d = A.desc
+    sp = torch.zeros(d.nnzb * 256, dtype=torch.int16, device="cuda")
+    meta = torch.zeros(d.nnzb * 64, dtype=torch.uint8, device="cuda")
+    smat.lib().smat_clear_error()
+    smat.lib().preprocessing_mmaSTKernel_large(d.values, meta.data_ptr(), sp.data_ptr(), M, K, K, d.nnzb,
+                                               info.data_ptr(), rel.data_ptr())
+    torch.cuda.synchronize()
+    assert smat.lib().smat_last_status() == 0, smat.lib().smat_last_error()
+    tags = torch.empty(d.nnzb, dtype=torch.int32)
+    smat.copy_to_host(tags.data_ptr(), d.tags, d.nnzb * 4)
+    assert bool((tags == 1).any()) and bool((tags == 2).any())
+    host_vals = torch.empty(d.nnzb * 512, dtype=torch.int16)
+    smat.copy_to_host(host_vals.data_ptr(), d.values, d.nnzb * 1024)
+    hv = host_vals.view(d.nnzb, 512).clone()
+    hv[tags == 1] = 0
+    vals = hv.view(-1).cuda()
+    B = gen.dense_b(K, N, seed=N)
+    Bt = _dev(np.ascontiguousarray(B.T).view(np.int16))  # B[n*K + k]
+    C = torch.full((M, N), float("nan"), dtype=torch.float16, device="cuda")
+    smat.lib().mmaOBTSKernel_large(vals.data_ptr(), d.rowptr, d.colidx, meta.data_ptr(), sp.data_ptr(),
+                                   Bt.data_ptr(), C.data_ptr(), M, N, K, d.nnzb, info.data_ptr(), rel.data_ptr())
+    torch.cuda.synchronize()
+    assert smat.lib().smat_last_status() == 0, smat.lib().smat_last_error()
+    R = of.spmm_csr_f64(M, rp, ci, v, B)
+    Cn = C.cpu().numpy().astype(np.float64)
+    assert np.all(np.isfinite(Cn))
+    assert np.linalg.norm(Cn - R) / np.linalg.norm(R) < 1e-3
+
+
+def test_large_dropins_alias_and_arguments():
+    """mmaOBTKernel_large works on 16x16 blocks like the reference's copy of
+    mmaOBTKernel; the k=32 entry points refuse K that is not a multiple of 32."""
+    _need_gpu()
+    import smat
+    N = 64
+    M, K, A, Bt, R, Rsim = _ref_case(94, N)
+    info, rel = A.grid_maps()
+    d = A.desc
+    C = torch.full((M, N), float("nan"), dtype=torch.float16, device="cuda")
+    smat.lib().smat_clear_error()
+    smat.lib().mmaOBTKernel_large(d.values, d.rowptr, d.colidx, Bt.data_ptr(), C.data_ptr(), M, N, K, d.nnzb,
+                                  info.data_ptr(), rel.data_ptr())
+    torch.cuda.synchronize()
+    assert smat.lib().smat_last_status() == 0, smat.lib().smat_last_error()
+    Cn = C.cpu().numpy().astype(np.float64)
+    assert np.linalg.norm(Cn - R) / np.linalg.norm(R) < 1e-3
+    smat.lib().smat_clear_error()
+    smat.lib().mmaOBTSKernel_large(d.values, d.rowptr, d.colidx, d.values, d.values, Bt.data_ptr(), C.data_ptr(),
+                                   M, N, 48, d.nnzb, info.data_ptr(), rel.data_ptr())
+    assert smat.lib().smat_last_status() == smat.SMAT_ERR_INVALID
+    smat.lib().smat_clear_error()
+    smat.lib().preprocessing_mmaSTKernel_large(d.values, d.values, d.values, M, N, 48, d.nnzb, info.data_ptr(),
+                                               rel.data_ptr())
+    assert smat.lib().smat_last_status() == smat.SMAT_ERR_INVALID

@@ -103,12 +103,18 @@ def test_compress24_smfmac_layout_round_trip(bk):
 def test_compress24_rejects_non_24_blocks():
     _need_gpu()
     import smat
-    M, K, rp, ci, v = gen.band(n=256, hb=16, p=0.9, seed=83)
-    A = _bcsr(M, K, rp, ci, v)  # reference rule: some tag-1 blocks are not 2:4
+    # row 0 holds columns {0, 2, 3}: no three consecutive columns, so the
+    # reference rule (matrix.h:689-709) tags the block 2:4, yet its first
+    # 1x4 group has three nonzeros
+    M, K = 32, 32
+    rp = np.array([0, 3] + [3] * 31, np.int32)
+    ci = np.array([0, 2, 3], np.int32)
+    v = np.float16([1.0, 2.0, 3.0]).view(np.uint16)
+    A = _bcsr(M, K, rp, ci, v)
     brp, bci, bv, tags = of.csr_to_bcsr(M, K, 16, rp, ci, v)
+    assert tags.tolist() == [1]
     _, _, ov = of.compress24_k16(bv, tags)
-    if ov == 0:
-        pytest.skip("no overflowing block in this draw")
+    assert ov > 0  # the oracle overflows too
     with pytest.raises(smat.SmatError) as e:
         A.compress_24(smat.SMAT_24_REFERENCE_K16)
     assert e.value.status == smat.SMAT_ERR_NOT_24

@@ -29,37 +29,78 @@ __device__ inline int len_bucket(int64_t len) {
 }
 
 // Per row: histogram, longest row; per entry: |row - col| sum, entries near the
-// diagonal, and the sort key (block row, column).  Block-local LDS bins, then
-// one atomic per bin and block.
+// diagonal, and the sort key (block row << colbits | column).  Rows with
+// ascending columns also report their violations of the exact 2:4 rule (an
+// aligned 1x4 group with more than two nonzero values, +-0 counting as zero):
+// one key (block row << (colbits - 4) | column / 16) per violating group,
+// written to vkeys at positions reserved with one atomic per workgroup; a row
+// whose columns do not ascend sets *unsorted instead.  Block-local bins, then
+// one atomic per bin and workgroup.
+template <typename KT>
 __global__ __launch_bounds__(256) void k_row_stats(int64_t M, const int32_t* __restrict__ rp,
-                                                   const int32_t* __restrict__ ci,
-                                                   unsigned long long* __restrict__ acc,
-                                                   uint64_t* __restrict__ keys) {
+                                                   const int32_t* __restrict__ ci, const uint16_t* __restrict__ val,
+                                                   int colbits, unsigned long long* __restrict__ acc,
+                                                   KT* __restrict__ keys, KT* __restrict__ vkeys,
+                                                   unsigned long long* __restrict__ vcount,
+                                                   int32_t* __restrict__ unsorted) {
   // acc: [0..11] histogram, [12] max row, [13] sum |r-c|, [14] near diagonal
+  using Scan = hipcub::BlockScan<uint32_t, 256>;
+  __shared__ typename Scan::TempStorage scan_tmp;
   __shared__ unsigned long long hist[12], sums[2];
-  __shared__ unsigned long long mx;
+  __shared__ unsigned long long mx, vbase;
   if (threadIdx.x < 12) hist[threadIdx.x] = 0;
   if (threadIdx.x < 2) sums[threadIdx.x] = 0;
   if (threadIdx.x == 0) mx = 0;
   __syncthreads();
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t nv = 0;  // violating groups of this row
+  int32_t b = 0, e = 0;
   if (r < M) {
-    const int32_t b = rp[r], e = rp[r + 1];
+    b = rp[r];
+    e = rp[r + 1];
     const int64_t len = e - b;
     atomicAdd(&hist[len_bucket(len)], 1ull);
     atomicMax(&mx, (unsigned long long)len);
     unsigned long long s = 0, nd = 0;
+    int32_t prev = -1, g_prev = -1, cnt = 0;
+    bool sorted = true;
     for (int32_t q = b; q < e; q++) {
-      const int64_t c = ci[q];
-      const int64_t d = r > c ? r - c : c - r;
+      const int32_t c = ci[q];
+      const int64_t d = r > c ? r - (int64_t)c : (int64_t)c - r;
       s += (unsigned long long)d;
       nd += d <= 64;
-      keys[q] = ((uint64_t)(r / 16) << 32) | (uint32_t)c;
+      keys[q] = ((KT)(r / 16) << colbits) | (KT)(uint32_t)c;
+      sorted = sorted && c > prev;
+      prev = c;
+      if ((c >> 2) != g_prev) {
+        g_prev = c >> 2;
+        cnt = 0;
+      }
+      if ((val[q] & 0x7fff) != 0 && ++cnt == 3) nv++;
+    }
+    if (!sorted) {
+      atomicOr(unsorted, 1);
+      nv = 0;
     }
     atomicAdd(&sums[0], s);
     atomicAdd(&sums[1], nd);
   }
+  uint32_t off = 0, total = 0;
+  Scan(scan_tmp).ExclusiveSum(nv, off, total);
+  if (threadIdx.x == 0) vbase = total ? atomicAdd(vcount, (unsigned long long)total) : 0ull;
   __syncthreads();
+  if (nv) {  // second walk over the row: write its violation keys
+    KT* out = vkeys + vbase + off;
+    int32_t g_prev = -1, cnt = 0;
+    for (int32_t q = b; q < e; q++) {
+      const int32_t c = ci[q];
+      if ((c >> 2) != g_prev) {
+        g_prev = c >> 2;
+        cnt = 0;
+      }
+      if ((val[q] & 0x7fff) != 0 && ++cnt == 3) *out++ = ((KT)(r / 16) << (colbits - 4)) | (KT)(uint32_t)(c >> 4);
+    }
+  }
   if (threadIdx.x < 12 && hist[threadIdx.x]) atomicAdd(&acc[threadIdx.x], hist[threadIdx.x]);
   if (threadIdx.x == 0) {
     atomicMax(&acc[12], mx);
@@ -70,23 +111,37 @@ __global__ __launch_bounds__(256) void k_row_stats(int64_t M, const int32_t* __r
 
 // On keys sorted by (block row, column): a key starts a new 16 x bk block iff
 // its block row or its column / bk differs from the previous key's.
-__global__ __launch_bounds__(256) void k_block_heads(int64_t n, const uint64_t* __restrict__ keys,
+template <typename KT>
+__global__ __launch_bounds__(256) void k_block_heads(int64_t n, const KT* __restrict__ keys, int colbits,
                                                      unsigned long long* __restrict__ nnzb) {
   __shared__ unsigned long long cnt[6];
   if (threadIdx.x < 6) cnt[threadIdx.x] = 0;
   __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
-    const uint64_t k = keys[i];
-    const uint64_t p = i > 0 ? keys[i - 1] : ~0ull;
-    const bool new_row = i == 0 || (k >> 32) != (p >> 32);
-    const uint32_t c = (uint32_t)k, pc = (uint32_t)p;
+    const KT k = keys[i];
+    const KT p = i > 0 ? keys[i - 1] : k;
+    const bool new_row = i == 0 || (k >> colbits) != (p >> colbits);
+    const KT mask = ((KT)1 << colbits) - 1;
+    const KT c = k & mask, pc = p & mask;
 #pragma unroll
     for (int w = 0; w < 6; w++)
       if (new_row || (c >> w) != (pc >> w)) atomicAdd(&cnt[w], 1ull);
   }
   __syncthreads();
   if (threadIdx.x < 6 && cnt[threadIdx.x]) atomicAdd(&nnzb[threadIdx.x], cnt[threadIdx.x]);
+}
+
+// Distinct keys of a sorted array (run heads).
+template <typename KT>
+__global__ void k_count_heads(int64_t n, const KT* __restrict__ keys, unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long c;
+  if (threadIdx.x == 0) c = 0;
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && (i == 0 || keys[i] != keys[i - 1])) atomicAdd(&c, 1ull);
+  __syncthreads();
+  if (threadIdx.x == 0 && c) atomicAdd(out, c);
 }
 
 __global__ void k_count_tag1(int64_t n, const int32_t* __restrict__ tags, unsigned long long* __restrict__ out) {
@@ -238,47 +293,114 @@ __global__ void k_mlp_argmin(const float* __restrict__ scores, const uint8_t* __
 
 extern "C" {
 
+}  // extern "C"
+
+namespace {
+
+inline int bits_for(int64_t x) {  // bits needed for values 0..x (x >= 0)
+  int b = 1;
+  while (b < 63 && (x >> b) != 0) b++;
+  return b;
+}
+
+// Statistics pass with KT-wide sort keys: row statistics, the stored-block
+// counts of the six block widths (one radix sort over the key bits in use),
+// and the number of distinct 16 x 16 blocks that violate the exact 2:4 rule
+// (*violated; -1 when some row's columns do not ascend).
+template <typename KT>
+smat_status stats_pass(int64_t M, int64_t nnz, const int32_t* rowptr, const int32_t* colidx, const uint16_t* values,
+                       int colbits, int keybits, unsigned long long* a, int64_t* violated, hipStream_t s) {
+  smat::DevBuf keys, keys2, vkeys, vkeys2, tmp, tmp2, misc;
+  SMAT_TRY(keys.alloc((nnz + 1) * sizeof(KT)));
+  SMAT_TRY(keys2.alloc((nnz + 1) * sizeof(KT)));
+  SMAT_TRY(vkeys.alloc((nnz / 3 + 1) * sizeof(KT)));  // at most one violation per three entries
+  SMAT_TRY(misc.alloc(32));                            // [0] violation count (u64), [1] unsorted flag
+  SMAT_HIP_TRY(hipMemsetAsync(misc.p, 0, 32, s));
+  unsigned long long* vcount = misc.as<unsigned long long>();
+  int32_t* unsorted = reinterpret_cast<int32_t*>(misc.as<unsigned long long>() + 1);
+  hipLaunchKernelGGL(k_row_stats<KT>, grid_of(M, 256), 256, 0, s, M, rowptr, colidx, values, colbits, a,
+                     keys.as<KT>(), vkeys.as<KT>(), vcount, unsorted);
+  SMAT_LAUNCH_CHECK();
+  if (nnz > 0) {
+    size_t tb = 0;
+    SMAT_HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, keys.as<KT>(), keys2.as<KT>(), (int)nnz, 0, keybits,
+                                                   s));
+    SMAT_TRY(tmp.alloc(tb));
+    SMAT_HIP_TRY(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, keys.as<KT>(), keys2.as<KT>(), (int)nnz, 0, keybits, s));
+    hipLaunchKernelGGL(k_block_heads<KT>, grid_of(nnz, 256), 256, 0, s, nnz, keys2.as<KT>(), colbits, a + 15);
+    SMAT_LAUNCH_CHECK();
+  }
+  unsigned long long h[2] = {0, 0};
+  SMAT_HIP_TRY(hipMemcpyAsync(h, misc.p, 16, hipMemcpyDeviceToHost, s));
+  SMAT_HIP_TRY(hipStreamSynchronize(s));
+  if ((h[1] & 0xffffffffull) != 0) {
+    *violated = -1;
+    return SMAT_OK;
+  }
+  const int64_t nv = (int64_t)h[0];
+  *violated = 0;
+  if (nv == 0) return SMAT_OK;
+  SMAT_TRY(vkeys2.alloc(nv * sizeof(KT)));
+  size_t tb = 0;
+  SMAT_HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, vkeys.as<KT>(), vkeys2.as<KT>(), (int)nv, 0,
+                                                 keybits - 4, s));
+  SMAT_TRY(tmp2.alloc(tb));
+  SMAT_HIP_TRY(hipcub::DeviceRadixSort::SortKeys(tmp2.p, tb, vkeys.as<KT>(), vkeys2.as<KT>(), (int)nv, 0,
+                                                 keybits - 4, s));
+  SMAT_HIP_TRY(hipMemsetAsync(misc.p, 0, 8, s));
+  hipLaunchKernelGGL(k_count_heads<KT>, grid_of(nv, 256), 256, 0, s, nv, vkeys2.as<KT>(), vcount);
+  SMAT_LAUNCH_CHECK();
+  SMAT_HIP_TRY(hipMemcpyAsync(h, misc.p, 8, hipMemcpyDeviceToHost, s));
+  SMAT_HIP_TRY(hipStreamSynchronize(s));
+  *violated = (int64_t)h[0];
+  return SMAT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 smat_status smat_matrix_stats(int64_t M, int64_t K, int64_t nnz, const int32_t* rowptr, const int32_t* colidx,
                               const uint16_t* values, smat_matrix_stats_t* out, smat_stream stream) {
   SMAT_REQUIRE(out && rowptr && M > 0 && K > 0 && nnz >= 0, "smat_matrix_stats: bad argument");
   SMAT_REQUIRE(M % 16 == 0 && K % 32 == 0, "smat_matrix_stats: M must be a multiple of 16 and K of 32");
   SMAT_REQUIRE(nnz == 0 || (colidx && values), "smat_matrix_stats: null CSR arrays");
+  SMAT_REQUIRE(K <= (int64_t)INT32_MAX + 1, "smat_matrix_stats: K too large");
   hipStream_t s = smat::to_hip(stream);
-  smat::DevBuf acc, keys, keys2, tmp;
-  SMAT_TRY(acc.alloc(32 * sizeof(unsigned long long)));  // 15 row stats, 6 block counts, 1 2:4 count
-  SMAT_TRY(keys.alloc((nnz + 1) * 8));
-  SMAT_TRY(keys2.alloc((nnz + 1) * 8));
+  smat::DevBuf acc;
+  SMAT_TRY(acc.alloc(32 * sizeof(unsigned long long)));  // 15 row stats, 6 block counts
   unsigned long long* a = acc.as<unsigned long long>();
   hipLaunchKernelGGL(k_zero_u64, 1, 32, 0, s, a, 32);
-  hipLaunchKernelGGL(k_row_stats, grid_of(M, 256), 256, 0, s, M, rowptr, colidx, a, keys.as<uint64_t>());
-  SMAT_LAUNCH_CHECK();
-  if (nnz > 0) {
-    size_t tb = 0;
-    SMAT_HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, keys.as<uint64_t>(), keys2.as<uint64_t>(), (int)nnz, 0,
-                                                   64, s));
-    SMAT_TRY(tmp.alloc(tb));
-    SMAT_HIP_TRY(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, keys.as<uint64_t>(), keys2.as<uint64_t>(), (int)nnz, 0,
-                                                   64, s));
-    hipLaunchKernelGGL(k_block_heads, grid_of(nnz, 256), 256, 0, s, nnz, keys2.as<uint64_t>(), a + 15);
-    SMAT_LAUNCH_CHECK();
-  }
-  // 2:4 compliance of the 16 x 16 blocks under the exact rule
-  smat_bcsr A16 = nullptr;
-  SMAT_TRY(smat_bcsr_from_csr(M, K, nnz, rowptr, colidx, values, 16, SMAT_TAG_EXACT24, stream, &A16));
-  smat_bcsr_desc d{};
-  smat_status st = smat_bcsr_describe(A16, &d);
-  if (st == SMAT_OK && d.nnzb > 0) {
-    hipLaunchKernelGGL(k_count_tag1, grid_of(d.nnzb, 256), 256, 0, s, d.nnzb, d.tags, a + 21);
-    if (hipGetLastError() != hipSuccess) st = smat::set_error(SMAT_ERR_HIP, "smat_matrix_stats: launch failed");
-  }
+  // sort keys: block row above the column; 32-bit keys when both fit
+  const int colbits = std::max(5, bits_for(K - 1));
+  const int keybits = bits_for(M / 16 - 1) + colbits;
+  int64_t violated = 0;
+  if (keybits <= 32) SMAT_TRY(stats_pass<uint32_t>(M, nnz, rowptr, colidx, values, colbits, keybits, a, &violated, s));
+  else SMAT_TRY(stats_pass<uint64_t>(M, nnz, rowptr, colidx, values, colbits, keybits, a, &violated, s));
   unsigned long long h[32];
-  if (st == SMAT_OK) {
-    const hipError_t e = hipMemcpyAsync(h, a, sizeof(h), hipMemcpyDeviceToHost, s);
-    const hipError_t e2 = e == hipSuccess ? hipStreamSynchronize(s) : e;
-    if (e2 != hipSuccess) st = smat::set_error(SMAT_ERR_HIP, "smat_matrix_stats: copy: %s", hipGetErrorString(e2));
+  SMAT_HIP_TRY(hipMemcpyAsync(h, a, sizeof(h), hipMemcpyDeviceToHost, s));
+  SMAT_HIP_TRY(hipStreamSynchronize(s));
+  const int64_t nnzb16 = (int64_t)h[15 + 4];
+  int64_t blocks24 = nnzb16 - violated;
+  if (violated < 0) {
+    // rows with unsorted columns: tag the 16 x 16 BCSR under the exact rule
+    smat_bcsr A16 = nullptr;
+    SMAT_TRY(smat_bcsr_from_csr(M, K, nnz, rowptr, colidx, values, 16, SMAT_TAG_EXACT24, stream, &A16));
+    smat_bcsr_desc d{};
+    smat_status st = smat_bcsr_describe(A16, &d);
+    unsigned long long c = 0;
+    if (st == SMAT_OK && d.nnzb > 0) {
+      hipLaunchKernelGGL(k_zero_u64, 1, 32, 0, s, a, 1);
+      hipLaunchKernelGGL(k_count_tag1, grid_of(d.nnzb, 256), 256, 0, s, d.nnzb, d.tags, a);
+      hipError_t e = hipGetLastError();
+      if (e == hipSuccess) e = hipMemcpyAsync(&c, a, 8, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) st = smat::set_error(SMAT_ERR_HIP, "smat_matrix_stats: %s", hipGetErrorString(e));
+    }
+    smat_bcsr_destroy(A16);
+    if (st != SMAT_OK) return st;
+    blocks24 = (int64_t)c;
   }
-  smat_bcsr_destroy(A16);
-  if (st != SMAT_OK) return st;
   out->rows = M;
   out->cols = K;
   out->nnz = nnz;
@@ -287,8 +409,8 @@ smat_status smat_matrix_stats(int64_t M, int64_t K, int64_t nnz, const int32_t* 
   out->sum_abs_offset = (int64_t)h[13];
   out->near_diag = (int64_t)h[14];
   for (int w = 0; w < 6; w++) out->nnzb[w] = (int64_t)h[15 + w];
-  out->nnzb16 = d.nnzb;
-  out->blocks24 = (int64_t)h[21];
+  out->nnzb16 = nnzb16;
+  out->blocks24 = blocks24;
   return SMAT_OK;
 }
 

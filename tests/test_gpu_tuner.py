@@ -37,6 +37,50 @@ def test_stats_match_oracle(seed):
     assert st.as_dict() == of.matrix_stats(M, K, rp, ci, v)
 
 
+@pytest.mark.parametrize("case", ["stored_zeros", "unsorted_rows", "wide_64bit_keys"])
+def test_stats_edge_cases_match_oracle(case):
+    """Exact 2:4 block count with stored +0 / -0 values (they do not count),
+    with rows whose columns do not ascend (the 16 x 16 BCSR route), and with
+    more block-row and column bits than a 32-bit sort key holds."""
+    _need_gpu()
+    from smat import gen, tuner
+    rng = np.random.default_rng(7)
+    if case == "wide_64bit_keys":
+        M, K = 131072, 1 << 22  # 13 block-row bits + 22 column bits
+        rows = np.sort(rng.choice(M, size=4000, replace=False))
+        per = rng.integers(1, 9, size=rows.size)
+        rp = np.zeros(M + 1, np.int64)
+        rp[rows + 1] = per
+        rp = np.cumsum(rp).astype(np.int32)
+        ci = np.concatenate([np.sort(4 * rng.integers(0, K // 4 - 2) + np.arange(n) % 4 + 4 * (np.arange(n) // 4))
+                             for n in per]).astype(np.int32)
+        v = np.float16(rng.uniform(0.5, 1.0, ci.size)).view(np.uint16)
+    else:
+        M, K, rp, ci, v = gen.block_clustered(n=2048, tiles=400, seed=11, prune=False)
+        v = v.copy()
+        # Half the 16 x 16 tiles hold +0 / -0 in columns 2 and 3 of every 1x4
+        # group (2:4 by value, although all 16 columns are stored); the rest
+        # stay dense, with a few stored zeros that leave them violating.
+        rows = np.repeat(np.arange(M), np.diff(rp))
+        sel = ((rows // 16 + ci // 16) % 2 == 0) & (ci % 4 >= 2)
+        v[sel] = np.where(rng.random(int(sel.sum())) < 0.5, 0x0000, 0x8000)
+        v[~sel & (rng.random(v.size) < 0.05)] = 0x8000
+        if case == "unsorted_rows":
+            ci = ci.copy()
+            for row in range(0, M, 7):
+                a, b = rp[row], rp[row + 1]
+                if b - a > 1:
+                    perm = rng.permutation(b - a)
+                    ci[a:b] = ci[a:b][perm]
+                    v[a:b] = v[a:b][perm]
+    print(f"stats edge case {case}: rows {M} cols {K} nnz {ci.size}", flush=True)
+    st = tuner.stats(M, K, _dev(rp), _dev(ci), _dev(v.view(np.int16)), int(ci.size))
+    exp = of.matrix_stats(M, K, rp, ci, v)
+    assert st.as_dict() == exp
+    if case != "wide_64bit_keys":
+        assert 0 < exp["blocks24"] < exp["nnzb16"]  # both kinds of 16 x 16 block occur
+
+
 def test_features_formula():
     _need_gpu()
     from smat import tuner

@@ -546,6 +546,27 @@ void mmaOBTSKernel_large(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColId
                          int* relativeBlockIndexMapping);
 
 /*
+ * HGEMM_FUNC_SPLIT (main.cu:35-41) — replaces mmaOBTSKernel_large_separate
+ * (src/cuda_hgemm/src/mma/mmaOBTS_large_separate.cu:18-275, evaluated by
+ * tester.h:552-606 from main.cu:309-311): 16x32 BCSR SpMM over two block
+ * lists, the 2:4 list (bcsrRowPtrA_sparse / bcsrColIdxA_sparse, as written by
+ * smat_bcsr_filter) read from the (sparseMatrixA, metadata) pair of
+ * preprocessing_mmaSTKernel_large, and the dense list (bcsrRowPtrA_dense /
+ * bcsrColIdxA_dense) read from bcsrValuesA_sparse at each block's
+ * relativeBlockIndexMapping slot, as the reference kernel reads its first
+ * values argument.  The reference's 2:4 loop is commented out
+ * (mmaOBTS_large_separate.cu:53-154), so it returns only the dense blocks'
+ * share; this computes the whole product.  bcsrValuesA_dense, nonzeroBlocks
+ * and blockInfo are not read.
+ */
+void mmaOBTSKernel_large_separate(uint16_t* bcsrValuesA_sparse, int* bcsrRowPtrA_sparse,
+                                  int* bcsrColIdxA_sparse, uint16_t* bcsrValuesA_dense,
+                                  int* bcsrRowPtrA_dense, int* bcsrColIdxA_dense, char* metadata,
+                                  uint16_t* sparseMatrixA, uint16_t* B, uint16_t* C, size_t M, size_t N,
+                                  size_t K, size_t nonzeroBlocks, int* blockInfo,
+                                  int* relativeBlockIndexMapping);
+
+/*
  * Replaces preprocessing_mmaSTKernel_large (main.cu:64-68, mmaST_large.cu:
  * 324-335): compresses every blockInfo==1 16x32 block.  The reference's k=32
  * producer and consumer disagree on metadata byte offsets (mmaST_large.cu:

@@ -35,34 +35,25 @@ __device__ inline smat_h8 expand_ref24(const uint16_t* __restrict__ sp4, uint8_t
   return __builtin_bit_cast(smat_h8, *reinterpret_cast<uint4*>(out));
 }
 
-// C[M x N] (FP16, row-major, ldc = N) = A * B with B[n*K + k].  BKW = 16:
-// 16x16 blocks, two per MFMA step; BKW = 32: 16x32 blocks (the reference's
-// k=32 "large" BCSR), one per step, whose 2:4 form is two k=16 halves (see
-// preprocessing_mmaSTKernel_large).
-template <bool MIXED, int BKW>
-__global__ __launch_bounds__(256) void k_ref_spmm(const uint16_t* __restrict__ vals,
-                                                  const int* __restrict__ rowptr,
-                                                  const int* __restrict__ colidx,
-                                                  const uint8_t* __restrict__ meta,
-                                                  const uint16_t* __restrict__ sparseA,
-                                                  const uint16_t* __restrict__ B,
-                                                  _Float16* __restrict__ C, int64_t M, int64_t N,
-                                                  int64_t K, const int* __restrict__ blockInfo,
-                                                  const int* __restrict__ relMap) {
-  const int lane = threadIdx.x & 63;
-  const int64_t task = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
-  const int64_t ntiles = (N + 16 * kNsub - 1) / (16 * kNsub);
-  const int64_t brow = task / ntiles;
-  if (brow >= M / 16) return;
-  const int64_t n0 = (task % ntiles) * 16 * kNsub;
+// How a list's blocks are read: every block dense (bcsrValuesA), every block
+// 2:4 (the compressed arrays), or per block as blockInfo says.
+enum RefInfo { kAllDense = 0, kAll24 = 1, kPerBlock = 2 };
+
+// Accumulates the blocks of block row `brow` listed in rowptr/colidx into the
+// wave's 16 x (16*kNsub) tile.  BKW = 16: 16x16 blocks, two per MFMA step;
+// BKW = 32: 16x32 blocks (the reference's k=32 "large" BCSR), one per step,
+// whose 2:4 form is two k=16 halves (see preprocessing_mmaSTKernel_large).
+// Blocks are found through relativeBlockIndexMapping like the reference
+// kernels (mmaOBT.cu:56-61).
+template <int BKW, RefInfo INFO>
+__device__ inline void ref_list(smat_f4 (&acc)[kNsub], const uint16_t* __restrict__ vals,
+                                const int* __restrict__ rowptr, const int* __restrict__ colidx,
+                                const uint8_t* __restrict__ meta, const uint16_t* __restrict__ sparseA,
+                                const uint16_t* __restrict__ B, int64_t brow, int64_t n0, int64_t N, int64_t K,
+                                const int* __restrict__ blockInfo, const int* __restrict__ relMap, int lane) {
   const int64_t ncr = K / BKW;
   const int r16 = lane & 15, grp = lane >> 4, sub = lane >> 5, half = grp & 1;
   constexpr int PER = 32 / BKW;  // blocks per MFMA step
-
-  smat_f4 acc[kNsub];
-#pragma unroll
-  for (int j = 0; j < kNsub; j++) acc[j] = smat_f4{0.f, 0.f, 0.f, 0.f};
-
   const int pb = rowptr[brow], pe = rowptr[brow + 1];
   for (int p = pb; p < pe; p += PER) {
     const int blk = BKW == 16 ? p + sub : p;
@@ -73,7 +64,7 @@ __global__ __launch_bounds__(256) void k_ref_spmm(const uint16_t* __restrict__ v
       const int col = colidx[blk];
       const int64_t g = brow * ncr + col / BKW;
       const int64_t rel = relMap[g];
-      const int info = MIXED ? blockInfo[g] : 2;
+      const int info = INFO == kPerBlock ? blockInfo[g] : INFO == kAll24 ? 1 : 2;
       // k-slots of this lane: columns 8*half .. of block `blk` (BKW = 16), or
       // columns 8*grp .. of the 16x32 block, i.e. of its 16-column half grp/2
       const int64_t sb = BKW == 16 ? rel : rel * 2 + (grp >> 1);  // 16x16 (sub-)block of the 2:4 arrays
@@ -96,6 +87,39 @@ __global__ __launch_bounds__(256) void k_ref_spmm(const uint16_t* __restrict__ v
       acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc[j], 0, 0, 0);
     }
   }
+}
+
+// C[M x N] (FP16, row-major, ldc = N) = A * B with B[n*K + k]; one wave per
+// (block row, 64-column tile).  SPLIT: the block row's 2:4 list
+// (rowptr2/colidx2, read from the compressed arrays) and then its dense list
+// (rowptr/colidx) go into one FP32 tile; otherwise the one list rowptr/colidx
+// is read as INFO says.
+template <RefInfo INFO, int BKW, bool SPLIT>
+__global__ __launch_bounds__(256) void k_ref_spmm(const uint16_t* __restrict__ vals,
+                                                  const int* __restrict__ rowptr,
+                                                  const int* __restrict__ colidx,
+                                                  const int* __restrict__ rowptr2,
+                                                  const int* __restrict__ colidx2,
+                                                  const uint8_t* __restrict__ meta,
+                                                  const uint16_t* __restrict__ sparseA,
+                                                  const uint16_t* __restrict__ B,
+                                                  _Float16* __restrict__ C, int64_t M, int64_t N,
+                                                  int64_t K, const int* __restrict__ blockInfo,
+                                                  const int* __restrict__ relMap) {
+  const int lane = threadIdx.x & 63;
+  const int64_t task = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int64_t ntiles = (N + 16 * kNsub - 1) / (16 * kNsub);
+  const int64_t brow = task / ntiles;
+  if (brow >= M / 16) return;
+  const int64_t n0 = (task % ntiles) * 16 * kNsub;
+  const int r16 = lane & 15, grp = lane >> 4;
+
+  smat_f4 acc[kNsub];
+#pragma unroll
+  for (int j = 0; j < kNsub; j++) acc[j] = smat_f4{0.f, 0.f, 0.f, 0.f};
+  if (SPLIT)
+    ref_list<BKW, kAll24>(acc, vals, rowptr2, colidx2, meta, sparseA, B, brow, n0, N, K, blockInfo, relMap, lane);
+  ref_list<BKW, INFO>(acc, vals, rowptr, colidx, meta, sparseA, B, brow, n0, N, K, blockInfo, relMap, lane);
 #pragma unroll
   for (int j = 0; j < kNsub; j++) {
     const int64_t n = n0 + 16 * j + r16;
@@ -163,8 +187,8 @@ void mmaOBTKernel(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdxA, uin
   }
   const int64_t ntiles = ((int64_t)N + 63) / 64;
   const int64_t tasks = (int64_t)(M / 16) * ntiles;
-  hipLaunchKernelGGL((k_ref_spmm<false, 16>), (unsigned)((tasks + kWaves - 1) / kWaves), 64 * kWaves, 0, 0,
-                     bcsrValuesA, bcsrRowPtrA, bcsrColIdxA, nullptr, nullptr, B,
+  hipLaunchKernelGGL((k_ref_spmm<kAllDense, 16, false>), (unsigned)((tasks + kWaves - 1) / kWaves), 64 * kWaves,
+                     0, 0, bcsrValuesA, bcsrRowPtrA, bcsrColIdxA, nullptr, nullptr, nullptr, nullptr, B,
                      reinterpret_cast<_Float16*>(C), (int64_t)M, (int64_t)N, (int64_t)K, nullptr,
                      relativeBlockIndexMapping);
   if (hipGetLastError() != hipSuccess) smat::set_error(SMAT_ERR_HIP, "mmaOBTKernel: launch failed");
@@ -182,8 +206,8 @@ void mmaOBTSKernel(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdxA, ch
   }
   const int64_t ntiles = ((int64_t)N + 63) / 64;
   const int64_t tasks = (int64_t)(M / 16) * ntiles;
-  hipLaunchKernelGGL((k_ref_spmm<true, 16>), (unsigned)((tasks + kWaves - 1) / kWaves), 64 * kWaves, 0, 0,
-                     bcsrValuesA, bcsrRowPtrA, bcsrColIdxA, reinterpret_cast<const uint8_t*>(metadata),
+  hipLaunchKernelGGL((k_ref_spmm<kPerBlock, 16, false>), (unsigned)((tasks + kWaves - 1) / kWaves), 64 * kWaves,
+                     0, 0, bcsrValuesA, bcsrRowPtrA, bcsrColIdxA, nullptr, nullptr, reinterpret_cast<const uint8_t*>(metadata),
                      sparseMatrixA, B, reinterpret_cast<_Float16*>(C), (int64_t)M, (int64_t)N, (int64_t)K,
                      blockInfo, relativeBlockIndexMapping);
   if (hipGetLastError() != hipSuccess) smat::set_error(SMAT_ERR_HIP, "mmaOBTSKernel: launch failed");
@@ -246,11 +270,47 @@ void mmaOBTSKernel_large(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColId
   }
   const int64_t ntiles = ((int64_t)N + 63) / 64;
   const int64_t tasks = (int64_t)(M / 16) * ntiles;
-  hipLaunchKernelGGL((k_ref_spmm<true, 32>), (unsigned)((tasks + kWaves - 1) / kWaves), 64 * kWaves, 0, 0,
-                     bcsrValuesA, bcsrRowPtrA, bcsrColIdxA, reinterpret_cast<const uint8_t*>(metadata),
+  hipLaunchKernelGGL((k_ref_spmm<kPerBlock, 32, false>), (unsigned)((tasks + kWaves - 1) / kWaves), 64 * kWaves,
+                     0, 0, bcsrValuesA, bcsrRowPtrA, bcsrColIdxA, nullptr, nullptr, reinterpret_cast<const uint8_t*>(metadata),
                      sparseMatrixA, B, reinterpret_cast<_Float16*>(C), (int64_t)M, (int64_t)N, (int64_t)K,
                      blockInfo, relativeBlockIndexMapping);
   if (hipGetLastError() != hipSuccess) smat::set_error(SMAT_ERR_HIP, "mmaOBTSKernel_large: launch failed");
+}
+
+// Split 16x32 SpMM (mmaOBTS_large_separate.cu:262-275, evaluated by
+// tester.h:552-606): the 2:4 blocks come from their own BCSR list
+// (bcsrRowPtrA_sparse/bcsrColIdxA_sparse, filterBcsrBlocks) and are read
+// from the compressed pair written by preprocessing_mmaSTKernel_large; the
+// dense blocks come from the dense list and are read from bcsrValuesA_sparse
+// at their relativeBlockIndexMapping slot, as the reference kernel reads its
+// first values argument (its only caller passes the same BCSR values for
+// both).  The reference has its 2:4 loop commented out
+// (mmaOBTS_large_separate.cu:53-154) and so returns the dense blocks' share
+// only; this computes the whole product.  bcsrValuesA_dense and blockInfo
+// are not read, as in the reference.
+void mmaOBTSKernel_large_separate(uint16_t* bcsrValuesA_sparse, int* bcsrRowPtrA_sparse, int* bcsrColIdxA_sparse,
+                                  uint16_t* bcsrValuesA_dense, int* bcsrRowPtrA_dense, int* bcsrColIdxA_dense,
+                                  char* metadata, uint16_t* sparseMatrixA, uint16_t* B, uint16_t* C, size_t M,
+                                  size_t N, size_t K, size_t nonzeroBlocks, int* blockInfo,
+                                  int* relativeBlockIndexMapping) {
+  (void)bcsrValuesA_dense;
+  (void)nonzeroBlocks;
+  (void)blockInfo;
+  if (!check_ref_args(bcsrRowPtrA_dense, B, C, M, N, K, "mmaOBTSKernel_large_separate", 32)) return;
+  if (!relativeBlockIndexMapping || !bcsrRowPtrA_sparse || !bcsrColIdxA_sparse || !bcsrColIdxA_dense ||
+      !bcsrValuesA_sparse || !metadata || !sparseMatrixA) {
+    smat::set_error(SMAT_ERR_INVALID, "mmaOBTSKernel_large_separate: null BCSR or 2:4 array");
+    return;
+  }
+  const int64_t ntiles = ((int64_t)N + 63) / 64;
+  const int64_t tasks = (int64_t)(M / 16) * ntiles;
+  hipLaunchKernelGGL((k_ref_spmm<kAllDense, 32, true>), (unsigned)((tasks + kWaves - 1) / kWaves), 64 * kWaves,
+                     0, 0, bcsrValuesA_sparse, bcsrRowPtrA_dense, bcsrColIdxA_dense, bcsrRowPtrA_sparse,
+                     bcsrColIdxA_sparse, reinterpret_cast<const uint8_t*>(metadata), sparseMatrixA, B,
+                     reinterpret_cast<_Float16*>(C), (int64_t)M, (int64_t)N, (int64_t)K, nullptr,
+                     relativeBlockIndexMapping);
+  if (hipGetLastError() != hipSuccess)
+    smat::set_error(SMAT_ERR_HIP, "mmaOBTSKernel_large_separate: launch failed");
 }
 
 // k=32 compressor (replaces mmaST_large.cu:324-335).  The reference's k=32

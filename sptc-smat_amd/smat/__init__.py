@@ -151,6 +151,7 @@ SIGNATURES = {
     "preprocessing_mmaSTKernel": (None, [_p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
     "mmaOBTKernel_large": (None, [_p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
     "mmaOBTSKernel_large": (None, [_p, _p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
+    "mmaOBTSKernel_large_separate": (None, [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
     "preprocessing_mmaSTKernel_large": (None, [_p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
 }
 

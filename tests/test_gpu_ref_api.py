@@ -279,6 +279,77 @@ def test_mmaOBTSKernel_large_dropin_mixed(N):
     assert np.linalg.norm(Cn - R) / np.linalg.norm(R) < 1e-3
 
 
+@pytest.mark.parametrize("N", [64, 40])
+def test_mmaOBTSKernel_large_separate_dropin(N):
+    """Split k=32 drop-in: the 2:4 list of smat_bcsr_filter read from the
+    compressed pair of preprocessing_mmaSTKernel_large (their dense copies
+    zeroed), the dense list from bcsrValuesA_sparse; bcsrValuesA_dense is NaN
+    and must not be read.  The whole product (the reference returns only the
+    dense blocks' share) against the FP64 product."""
+    _need_gpu()
+    import smat
+    M, K, rp, ci, v = mixed_matrix_32(seed=95 + N)
+    A = _bcsr(M, K, rp, ci, v, bk=32, rule=smat.SMAT_TAG_EXACT24)
+    info, rel = A.grid_maps()
+    d = A.desc
+    sp = torch.zeros(d.nnzb * 256, dtype=torch.int16, device="cuda")
+    meta = torch.zeros(d.nnzb * 64, dtype=torch.uint8, device="cuda")
+    smat.lib().smat_clear_error()
+    smat.lib().preprocessing_mmaSTKernel_large(d.values, meta.data_ptr(), sp.data_ptr(), M, K, K, d.nnzb,
+                                               info.data_ptr(), rel.data_ptr())
+    (srp, sci, _), (drp, dci, dvals) = A.filter()
+    torch.cuda.synchronize()
+    assert smat.lib().smat_last_status() == 0, smat.lib().smat_last_error()
+    assert sci.numel() > 0 and dci.numel() > 0
+    tags = torch.empty(d.nnzb, dtype=torch.int32)
+    smat.copy_to_host(tags.data_ptr(), d.tags, d.nnzb * 4)
+    host_vals = torch.empty(d.nnzb * 512, dtype=torch.int16)
+    smat.copy_to_host(host_vals.data_ptr(), d.values, d.nnzb * 1024)
+    hv = host_vals.view(d.nnzb, 512).clone()
+    hv[tags == 1] = 0
+    vals = hv.view(-1).cuda()
+    nan_dense = torch.full_like(dvals, 0x7e00)  # FP16 NaN bits
+    B = gen.dense_b(K, N, seed=N + 1)
+    Bt = _dev(np.ascontiguousarray(B.T).view(np.int16))  # B[n*K + k]
+    C = torch.full((M, N), float("nan"), dtype=torch.float16, device="cuda")
+    smat.lib().mmaOBTSKernel_large_separate(vals.data_ptr(), srp.data_ptr(), sci.data_ptr(), nan_dense.data_ptr(),
+                                            drp.data_ptr(), dci.data_ptr(), meta.data_ptr(), sp.data_ptr(),
+                                            Bt.data_ptr(), C.data_ptr(), M, N, K, d.nnzb, info.data_ptr(),
+                                            rel.data_ptr())
+    torch.cuda.synchronize()
+    assert smat.lib().smat_last_status() == 0, smat.lib().smat_last_error()
+    R = of.spmm_csr_f64(M, rp, ci, v, B)
+    Cn = C.cpu().numpy().astype(np.float64)
+    assert np.all(np.isfinite(Cn))
+    assert np.linalg.norm(Cn - R) / np.linalg.norm(R) < 1e-3
+    # the 2:4 half contributes: without it the error would be of order one
+    rows = np.repeat(np.arange(M), np.diff(rp))
+    in24 = np.isin((rows // 16) * (K // 32) + ci // 32, _blocks_with_tag(A, 1))
+    dense_only = of.spmm_csr_f64(M, rp, ci, np.where(in24, 0, v).astype(np.uint16), B)
+    assert np.linalg.norm(dense_only - R) / np.linalg.norm(R) > 0.1
+    smat.lib().smat_clear_error()
+    smat.lib().mmaOBTSKernel_large_separate(vals.data_ptr(), srp.data_ptr(), sci.data_ptr(), nan_dense.data_ptr(),
+                                            drp.data_ptr(), dci.data_ptr(), meta.data_ptr(), sp.data_ptr(),
+                                            Bt.data_ptr(), C.data_ptr(), M, N, 48, d.nnzb, info.data_ptr(),
+                                            rel.data_ptr())
+    assert smat.lib().smat_last_status() == smat.SMAT_ERR_INVALID
+
+
+def _blocks_with_tag(A, tag):
+    """Grid cells (block row * K/bk + column block) of the BCSR blocks tagged `tag`."""
+    import smat
+    d = A.desc
+    rowptr = torch.empty(d.nbrows + 1, dtype=torch.int32)
+    colidx = torch.empty(max(d.nnzb, 1), dtype=torch.int32)
+    tags = torch.empty(max(d.nnzb, 1), dtype=torch.int32)
+    smat.copy_to_host(rowptr.data_ptr(), d.rowptr, (d.nbrows + 1) * 4)
+    smat.copy_to_host(colidx.data_ptr(), d.colidx, d.nnzb * 4)
+    smat.copy_to_host(tags.data_ptr(), d.tags, d.nnzb * 4)
+    brow = np.repeat(np.arange(d.nbrows), np.diff(rowptr.numpy()))
+    cells = brow * (d.K // d.bk) + colidx.numpy()[: d.nnzb] // d.bk
+    return cells[tags.numpy()[: d.nnzb] == tag]
+
+
 def test_large_dropins_alias_and_arguments():
     """mmaOBTKernel_large works on 16x16 blocks like the reference's copy of
     mmaOBTKernel; the k=32 entry points refuse K that is not a multiple of 32."""

@@ -13,7 +13,10 @@ one is reported beside it.  Format conversion (CSR -> BCSR on the GPU) is
 preprocessing, timed separately as `build_ms`, outside the steps.
 
 Prints ONE JSON line.  `roofline` is the HBM roofline of the tuned SpMM call
-with algorithmic bytes = stored A + block indices + row pointers + B + C.
+with algorithmic bytes = stored A + block indices + row pointers + B + C;
+`mfma` puts the same call's stored-block MFMA rate (2 * nnzb * 16 * bk * N
+flop, the zero padding of a block row's last MFMA step left out) against the
+dense MFMA peak.
 `cpu_baseline` times the reference's own CPU SpMM (CSR::multiply, compiled
 from the reference sources into oracle/_ref) on one host core, on a bounded
 sample of the same matrix (its leading principal submatrix), same unit.
@@ -30,6 +33,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "sptc-smat_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# dense MFMA peaks (TFLOP/s) by input type: FP16 / BF16 2.5 PFLOP/s, FP8 5.0
+MFMA_PEAK_TFLOPS = {"f16": 2500.0, "bf16": 2500.0, "fp8": 5000.0}
 
 # HBM bytes per SpMM call measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (calibrated with a 1 GiB copy: x2.0 / x1.0) for configurations that have a
@@ -623,6 +628,9 @@ def main():
                      "traffic_source": traffic[1] if traffic else None,
                      "kernel": "SpMM call (main kernel + split-row reduction)",
                      "alg_bytes_per_call": tuned["alg_bytes"], "event_ms_avg": tuned["event_ms"]},
+        "mfma": {"achieved": round(tuned["stored_gflops"] / 1e3, 2), "peak": MFMA_PEAK_TFLOPS[args.dtype],
+                 "unit": "TFLOP/s", "frac": round(tuned["stored_gflops"] / 1e3 / MFMA_PEAK_TFLOPS[args.dtype], 5),
+                 "flop": "stored-block MFMA work (2*nnzb*16*bk*N), not the bound: the call is HBM-bound"},
         "sweep": [{k: s[k] for k in ("bk", "n_tile", "kernel", "event_ms", "useful_gflops")} for s in sweep],
         "precision_ladder": ladder,
         "cpu_baseline": cpu,

@@ -289,14 +289,6 @@ __global__ void k_mlp_argmin(const float* __restrict__ scores, const uint8_t* __
   best[0] = bi;
 }
 
-}  // namespace
-
-extern "C" {
-
-}  // extern "C"
-
-namespace {
-
 inline int bits_for(int64_t x) {  // bits needed for values 0..x (x >= 0)
   int b = 1;
   while (b < 63 && (x >> b) != 0) b++;

@@ -6,11 +6,16 @@ a dense row-major FP16 B of 262144 x 256, FP32 accumulate and FP32 C.  A step
 is one full SpMM call (smat_spmm_execute) with A, B and C resident in HBM.
 
 The "plain" configuration is the reference's: BCSR 16x16, no reordering,
-dense-block kernel.  The "tuned" configuration is the fastest of an
-exhaustive sweep over block width (1..32) and N-tile (the tuner's fallback
-and judge); `value` is the tuned configuration's useful GFLOP/s and the plain
-one is reported beside it.  Format conversion (CSR -> BCSR on the GPU) is
-preprocessing, timed separately as `build_ms`, outside the steps.
+dense-block kernel.  The "tuned" configuration is the learned cost model's
+pick (block width, N-tile, kernel); for a matrix too large for the exact
+Keeper the pick is made on the matrix as given and on its locality row order
+(smat_locality_order), and a short probe keeps the faster.  An exhaustive
+sweep over block width (1..32) and N-tile, on the matrix as given, is
+reported beside it as the judge (the fallback when no trained weights
+exist); `value` is the tuned configuration's useful GFLOP/s and the plain one
+is reported beside it.  Format conversion (CSR -> BCSR on the GPU) and the
+row reorder are preprocessing, timed separately (`build_ms`, `decide_ms`),
+outside the steps.
 
 Prints ONE JSON line.  `roofline` is the HBM roofline of the tuned SpMM call
 with algorithmic bytes = stored A + block indices + row pointers + B + C;
@@ -41,7 +46,7 @@ MFMA_PEAK_TFLOPS = {"f16": 2500.0, "bf16": 2500.0, "fp8": 5000.0}
 # committed profile, under the automatic work schedule:
 # (workload, scale, N, bk, n_tile) -> (bytes, profile file).
 MEASURED_TRAFFIC = {
-    ("rmat", 18, 256, 2, "auto"): (2159e6, "profiles/spmm_lds_bk2_rmat18_n256_hbm_traffic.md"),
+    ("rmat", 18, 256, 2, "auto", "none"): (2159e6, "profiles/spmm_lds_bk2_rmat18_n256_hbm_traffic.md"),
 }
 
 
@@ -61,6 +66,11 @@ def parse():
     ap.add_argument("--kernel", default="dense", choices=["dense", "mixed24"])
     ap.add_argument("--dtype", default="f16", choices=["f16", "bf16"])
     ap.add_argument("--single", action="store_true", help="time only --bk/--n-tile (no sweep, no plain leg)")
+    ap.add_argument("--order", default="none", choices=["none", "locality"],
+                    help="row order of a single run: as given, or the locality order (C written back through "
+                         "the row map)")
+    ap.add_argument("--no-locality", action="store_true",
+                    help="tuned path: do not consider the locality row order")
     ap.add_argument("--no-ladder", action="store_true", help="skip the precision ladder (F16/BF16/FP8/split)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and preprocessing legs")
     ap.add_argument("--batch-tuned-only", action="store_true",
@@ -183,14 +193,18 @@ def _make16(args):
 class Run:
     """One configuration: BCSR build (timed as preprocessing) + plan + steps."""
 
-    def __init__(self, ctx, bk, n_tile, kernel="dense"):
+    def __init__(self, ctx, bk, n_tile, kernel="dense", order=None):
         import torch
         import smat
         self.bk, self.n_tile, self.kernel = bk, n_tile, kernel
+        # order: None, or (rowptr, colidx, values, rowmap) of the row-permuted
+        # CSR; C rows are written back through the row map
+        self.rowmap = order[3] if order is not None else None
+        rp, ci, v = order[:3] if order is not None else (ctx.d_rp, ctx.d_ci, ctx.d_v)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         rule = smat.SMAT_TAG_EXACT24 if kernel == "mixed24" else smat.SMAT_TAG_REFERENCE
-        self.A = smat.Bcsr(ctx.M, ctx.K, ctx.d_rp, ctx.d_ci, ctx.d_v, bk=bk, tag_rule=rule)
+        self.A = smat.Bcsr(ctx.M, ctx.K, rp, ci, v, bk=bk, tag_rule=rule)
         self.v24 = self.i24 = None
         if kernel == "mixed24":
             self.v24, self.i24 = self.A.compress_24(smat.SMAT_24_SMFMAC)
@@ -203,7 +217,10 @@ class Run:
         self.ctx = ctx
 
     def call(self):
-        self.plan(self.ctx.B, self.ctx.C, values24=self.v24, index24=self.i24)
+        if self.rowmap is None:
+            self.plan(self.ctx.B, self.ctx.C, values24=self.v24, index24=self.i24)
+        else:
+            self.plan.execute(self.ctx.B, self.ctx.C, values24=self.v24, index24=self.i24, c_rowmap=self.rowmap)
 
     def time(self, steps, warmup):
         import torch
@@ -254,25 +271,63 @@ class Ctx:
     pass
 
 
-def learned_pick(ctx, log):
-    """The learned cost model's configuration for the workload matrix
-    (features from GPU statistics, scores from the inference kernel).  None
-    when no trained weights are present."""
+def learned_pick(ctx, log, order=None):
+    """The learned cost model's configuration for the workload matrix, or for
+    its row-permuted copy `order` (features from GPU statistics, scores from
+    the inference kernel).  None when no trained weights are present."""
     import os
     from smat import tuner
     if not os.path.exists(tuner.WEIGHTS_PATH):
         log("no trained tuner weights; the sweep decides")
         return None
     t0 = time.perf_counter()
-    st = tuner.stats(ctx.M, ctx.K, ctx.d_rp, ctx.d_ci, ctx.d_v, ctx.nnz)
+    rp, ci, v = order[:3] if order is not None else (ctx.d_rp, ctx.d_ci, ctx.d_v)
+    st = tuner.stats(ctx.M, ctx.K, rp, ci, v, ctx.nnz)
     feats = tuner.features(st, ctx.N)
-    # the bench times configurations of the matrix as given (no reordering)
+    # the Keeper candidates (tau set) are not used here: the matrix is timed as
+    # given or in the locality order
     avail = tuner.available(ctx.M) * (tuner.ENC[:, -1] == 0)
     _, b = tuner.Model().scores(feats, avail.astype("uint8"))
     cfg = tuner.CONFIGS[b]
     ms = (time.perf_counter() - t0) * 1e3
     log(f"learned pick {cfg} ({ms:.1f} ms for statistics, features and inference)")
     return {"config": cfg, "decide_ms": round(ms, 2)}
+
+
+def locality_variant(ctx):
+    """The workload CSR in the locality row order (smat_locality_order +
+    smat_permute_csr_rows, on the GPU): (rowptr, colidx, values, rowmap)."""
+    import torch
+    import smat
+    perm = smat.locality_order(ctx.M, ctx.K, ctx.d_rp, ctx.d_ci)
+    rp, ci, v = smat.permute_rows(ctx.d_rp, ctx.d_ci, ctx.d_v, perm)
+    torch.cuda.synchronize()
+    return rp, ci, v, perm.to(torch.int32)
+
+
+def locality_choice(ctx, learned, log):
+    """For a matrix too large for the exact Keeper: the learned pick on its
+    locality-ordered copy against the learned pick on the matrix as given,
+    each timed over a short probe (2 warm-up and 5 timed calls, not the
+    measurement); the faster is kept.  Returns (order or None, report)."""
+    t0 = time.perf_counter()
+    order = locality_variant(ctx)
+    lo = learned_pick(ctx, log, order)
+
+    def probe(cfg, o):
+        _, bk, nt, kernel = cfg
+        r = Run(ctx, bk, nt, kernel, o).time(5, 2)
+        ms = r.event_ms
+        r.close()
+        return ms
+
+    as_given, ordered = probe(learned["config"], None), probe(lo["config"], order)
+    use = ordered < as_given
+    rep = {"config": lo["config"], "probe_event_ms": {"as_given": round(as_given, 4), "locality": round(ordered, 4)},
+           "used": use, "decide_ms": round((time.perf_counter() - t0) * 1e3, 1)}
+    log(f"locality order: pick {lo['config']}, probe {ordered:.4f} ms vs {as_given:.4f} ms as given -> "
+        f"{'used' if use else 'not used'}")
+    return (order if use else None), rep
 
 
 def _gen_batch_matrix(seed):
@@ -484,7 +539,7 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
-    from smat import gen
+    from smat import gen, tuner
 
     import smat
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)  # noqa: E731
@@ -535,13 +590,14 @@ def main():
     del A64, A32
     log(f"FP64 references in {time.time() - t1:.1f}s")
 
-    def measure(bk, nt, kernel, steps, warmup):
-        r = Run(ctx, bk, nt, kernel).time(steps, warmup)
+    def measure(bk, nt, kernel, steps, warmup, order=None):
+        r = Run(ctx, bk, nt, kernel, order).time(steps, warmup)
         err = smat.rel_fro_error(ctx.C, ctx.R_in)
         st = r.stats()
         st["rel_fro_err"] = err
         st["rel_fro_err_vs_fp32_inputs"] = smat.rel_fro_error(ctx.C, ctx.R_32)
-        log(f"bk={bk:2d} n_tile={nt or 'auto':>4} {kernel}: {st['event_ms']:.3f} ms, "
+        st["order"] = "locality" if order is not None else "none"
+        log(f"bk={bk:2d} n_tile={nt or 'auto':>4} {kernel} order {st['order']}: {st['event_ms']:.3f} ms, "
             f"{st['useful_gflops']:.1f} useful GFLOP/s, rel err {err:.2e}")
         r.close()
         if not err < 1e-5:
@@ -551,7 +607,8 @@ def main():
 
     learned = sweep_best = None
     if args.single:
-        tuned = measure(args.bk, args.n_tile, args.kernel, args.steps, args.warmup)
+        order = locality_variant(ctx) if args.order == "locality" else None
+        tuned = measure(args.bk, args.n_tile, args.kernel, args.steps, args.warmup, order)
         plain = None
         sweep = [tuned]
     else:
@@ -570,7 +627,12 @@ def main():
         learned = learned_pick(ctx, log)
         if learned is not None:
             tau, bk, nt, kernel = learned["config"]
-            tuned = measure(bk, nt, kernel, args.steps, args.warmup)
+            order = None
+            if ctx.M > tuner.REORDER_MAX_ROWS and not args.no_locality:
+                order, learned["locality"] = locality_choice(ctx, learned, log)
+                if order is not None:
+                    tau, bk, nt, kernel = learned["locality"]["config"]
+            tuned = measure(bk, nt, kernel, args.steps, args.warmup, order)
             learned["ratio_to_sweep_best"] = round(tuned["event_ms"] / sweep_best["event_ms"], 3)
         else:
             tuned = sweep_best
@@ -594,7 +656,7 @@ def main():
             cpu = {"error": str(ex)}
     # "auto" N-tiles are 64 columns for N >= 64
     nt_key = "auto" if tuned["n_tile"] == "auto" or (tuned["n_tile"] == 64 and ctx.N >= 64) else tuned["n_tile"]
-    traffic = MEASURED_TRAFFIC.get((args.workload, args.scale, ctx.N, tuned["bk"], nt_key)) \
+    traffic = MEASURED_TRAFFIC.get((args.workload, args.scale, ctx.N, tuned["bk"], nt_key, tuned["order"])) \
         if tuned["kernel"] == "dense" and args.dtype == "f16" and args.split == 0 and args.schedule == 0 else None
     out = {
         "metric": "SpMM useful GFLOP/s (2*nnz*N / t), learned-tuned vs plain FP16 BCSR",
@@ -610,7 +672,7 @@ def main():
         "dtype": f"{args.dtype} in, f32 accumulate/out",
         "data": "synthetic (seeded generator; B U[-1,1] seed 7)",
         "config": {"workload": f"{desc}, B {K}x{ctx.N} row-major, FP32 C", "rows": M, "nnz": ctx.nnz,
-                   "N": ctx.N, "tuned": {k: tuned[k] for k in ("bk", "n_tile", "kernel", "nnzb")},
+                   "N": ctx.N, "tuned": {k: tuned[k] for k in ("bk", "n_tile", "kernel", "nnzb", "order")},
                    "tuning": ("learned MLP pick (exhaustive sweep reported as judge)" if learned else
                               "exhaustive sweep over block width x N-tile") if not args.single else "fixed"},
         "learned": learned,

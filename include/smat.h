@@ -114,7 +114,7 @@ typedef struct {
   int64_t M;          /* rows, multiple of 16 */
   int64_t K;          /* columns, multiple of bk */
   int32_t bm;         /* block rows (16) */
-  int32_t bk;         /* block columns: 16 or 32 */
+  int32_t bk;         /* block columns: 1, 2, 4, 8, 16 or 32 */
   int64_t nbrows;     /* M / 16 */
   int64_t nnzb;       /* stored blocks */
   int32_t* rowptr;    /* [nbrows + 1] */
@@ -227,6 +227,19 @@ smat_status smat_blocking_info(int64_t rows, int64_t cols, int64_t nnz, const in
 smat_status smat_permute_csr_rows(int64_t rows, const int32_t* rowptr, const int32_t* colidx,
                                   const uint16_t* values, const int64_t* perm, int32_t* out_rowptr,
                                   int32_t* out_colidx, uint16_t* out_values, smat_stream stream);
+
+/*
+ * Locality row order — no counterpart in the reference; it stands in for the
+ * Jaccard Keeper (smat_reorder_grouping) on matrices too large for the
+ * Keeper's host replay.  Rows sorted by descending floor(log2 length), then
+ * by their first stored column, then by their second (a missing column
+ * counts as `cols`; the second is dropped when cols >= 2^29), ties in row
+ * order.  perm (device int64[rows]) receives the old row of each new row, as
+ * smat_permute_csr_rows takes it; the SpMM writes C back to the original
+ * rows through smat_spmm_operands.c_rowmap.
+ */
+smat_status smat_locality_order(int64_t rows, int64_t cols, const int32_t* rowptr, const int32_t* colidx,
+                                int64_t* perm, smat_stream stream);
 
 /*
  * Column pass of the reference's reorder driver (src/reorder.py:143-183):

@@ -263,6 +263,28 @@ def collect_blocking_info(rowptr64, colidx64, grouping, cols, col_block, use_ref
     return out, float(avg.value)
 
 
+def locality_order(rows, cols, rowptr, colidx):
+    """Locality row order (no reference counterpart; the library's stand-in
+    for the Keeper on matrices too large for it): rows by descending
+    floor(log2 length), then first and second stored column (`cols` when
+    missing; the second dropped when cols >= 2^29), ties in row order.
+    Returns perm (new row r = old row perm[r])."""
+    rp = np.asarray(rowptr, np.int64)
+    ci = np.asarray(colidx, np.int64)
+    lens = np.diff(rp)
+    lb = np.where(lens > 1, np.floor(np.log2(np.maximum(lens, 1))), 0).astype(np.int64)
+
+    def kth(k):
+        out = np.full(rows, cols, np.int64)
+        ok = lens > k
+        out[ok] = ci[rp[:-1][ok] + k]
+        return out
+
+    if cols < (1 << 29):
+        return np.lexsort((kth(1), kth(0), -lb))
+    return np.lexsort((kth(0), -lb))
+
+
 def read_edgelist(text: str, weighted: bool = True):
     """Edge-list reader with CSR::read_from_edgelist conventions
     (preprocess/src/general/csr.cpp:166-290, el format): rows must ascend,

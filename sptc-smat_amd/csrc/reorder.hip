@@ -434,6 +434,23 @@ __global__ void k_vbr_heads(int64_t n, const uint64_t* __restrict__ keys, int64_
   }
 }
 
+// Locality order key of each row: (63 - floor(log2 length)) above the first
+// stored column above the second one (`cols` where the row has fewer); the
+// second column is left out when three fields would not fit in 64 bits.
+__global__ void k_locality_keys(int64_t rows, int64_t cols, const int32_t* __restrict__ rowptr,
+                                const int32_t* __restrict__ colidx, int cbits, int with_second,
+                                uint64_t* __restrict__ keys, int64_t* __restrict__ ids) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const int32_t b = rowptr[r], e = rowptr[r + 1];
+  const int64_t len = e - b;
+  const uint64_t lb = len > 1 ? (uint64_t)(63 - __clzll((unsigned long long)len)) : 0ull;
+  const uint64_t c0 = len > 0 ? (uint64_t)(uint32_t)colidx[b] : (uint64_t)cols;
+  const uint64_t c1 = len > 1 ? (uint64_t)(uint32_t)colidx[b + 1] : (uint64_t)cols;
+  keys[r] = with_second ? ((63ull - lb) << (2 * cbits)) | (c0 << cbits) | c1 : ((63ull - lb) << cbits) | c0;
+  ids[r] = r;
+}
+
 __global__ void k_max_i32(int64_t n, const int32_t* __restrict__ v, unsigned int* __restrict__ out) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   int32_t m = i < n ? v[i] : 0;
@@ -671,6 +688,35 @@ smat_status smat_permute_csr_rows(int64_t rows, const int32_t* rowptr, const int
   hipLaunchKernelGGL(k_perm_gather, nblk(rows, 4), 256, 0, s, rows, perm, rowptr, colidx, values, out_rowptr,
                      out_colidx, out_values);
   SMAT_LAUNCH_CHECK();
+  SMAT_HIP_TRY(hipStreamSynchronize(s));
+  return SMAT_OK;
+}
+
+// Locality row order (no reference counterpart): rows sorted by descending
+// log2 length, then by their first and second stored columns, ties in row
+// order (a stable radix sort).  perm[r] = old row of new row r.
+smat_status smat_locality_order(int64_t rows, int64_t cols, const int32_t* rowptr, const int32_t* colidx,
+                                int64_t* perm, smat_stream stream) {
+  SMAT_REQUIRE(rows > 0 && cols > 0 && rowptr && colidx && perm, "smat_locality_order: bad argument");
+  SMAT_REQUIRE(rows <= INT32_MAX && cols <= INT32_MAX, "smat_locality_order: too many rows or columns");
+  hipStream_t s = smat::to_hip(stream);
+  int cbits = 1;
+  while (cbits < 32 && (cols >> cbits) != 0) cbits++;  // bits for 0..cols
+  const int with_second = 6 + 2 * cbits <= 64;
+  const int end_bit = 6 + (with_second ? 2 : 1) * cbits;
+  smat::DevBuf keys, keys2, ids, tmp;
+  SMAT_TRY(keys.alloc(rows * 8));
+  SMAT_TRY(keys2.alloc(rows * 8));
+  SMAT_TRY(ids.alloc(rows * 8));
+  hipLaunchKernelGGL(k_locality_keys, nblk(rows, 256), 256, 0, s, rows, cols, rowptr, colidx, cbits, with_second,
+                     keys.as<uint64_t>(), ids.as<int64_t>());
+  SMAT_LAUNCH_CHECK();
+  size_t tb = 0;
+  SMAT_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys.as<uint64_t>(), keys2.as<uint64_t>(),
+                                                  ids.as<int64_t>(), perm, (int)rows, 0, end_bit, s));
+  SMAT_TRY(tmp.alloc(tb));
+  SMAT_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, keys.as<uint64_t>(), keys2.as<uint64_t>(),
+                                                  ids.as<int64_t>(), perm, (int)rows, 0, end_bit, s));
   SMAT_HIP_TRY(hipStreamSynchronize(s));
   return SMAT_OK;
 }

@@ -199,12 +199,27 @@ __device__ inline uint4 load_a(const uint16_t* __restrict__ values, int32_t p, i
   }
 }
 
-// D layout: lane holds rows 4*grp + i, column 16*j + r16 of its tile.  With a
-// row map (A's rows reordered), row r of the tile goes to C row rowmap[r].
+// With a row map (A's rows reordered) row r of A goes to C row rowmap[r].
+// Each lane reads the C row of tile row lane & 15 when the task starts, so
+// the load's latency hides behind the task's main loop (one VGPR);
+// store_tile gathers its four rows from lanes 0..15.
+__device__ inline int32_t c_row_of_lane(const Item& it, int lane, const int32_t* __restrict__ rowmap) {
+  return rowmap && it.slot < 0 ? rowmap[(int64_t)it.brow * 16 + (lane & 15)] : 0;
+}
+
+// D layout: lane holds rows 4*grp + i, column 16*j + r16 of its tile.  crl:
+// c_row_of_lane of this lane (used when rowmap is set).  Called by the whole
+// wave.
 template <int NSUB, bool OUT_F16>
 __device__ inline void store_tile(const smat_f4 (&acc)[NSUB], const Item& it, int64_t n0, int r16, int grp,
                                   int64_t N, void* __restrict__ C, int64_t ldc, float* __restrict__ scratch,
-                                  const int32_t* __restrict__ rowmap) {
+                                  const int32_t* __restrict__ rowmap, int32_t crl) {
+  int64_t cr[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int row = 4 * grp + i;
+    cr[i] = rowmap ? (int64_t)__shfl(crl, row, 64) : (int64_t)it.brow * 16 + row;
+  }
 #pragma unroll
   for (int j = 0; j < NSUB; j++) {
     const int64_t n = n0 + 16 * j + r16;
@@ -215,8 +230,7 @@ __device__ inline void store_tile(const smat_f4 (&acc)[NSUB], const Item& it, in
       if (it.slot >= 0) {
         scratch[((int64_t)it.slot * 16 + row) * N + n] = acc[j][i];
       } else {
-        int64_t crow = (int64_t)it.brow * 16 + row;
-        if (rowmap) crow = rowmap[crow];
+        const int64_t crow = cr[i];
         if (OUT_F16)
           reinterpret_cast<_Float16*>(C)[crow * ldc + n] = (_Float16)acc[j][i];
         else
@@ -250,6 +264,7 @@ __global__ __launch_bounds__(256) void k_spmm_dense(
   const int r16 = lane & 15;   // A row / B column within the tile
   const int grp = lane >> 4;   // 0..3: k-slot group of 8
   constexpr int STEP = 32 / BK;  // blocks per MFMA step
+  const int32_t crl = c_row_of_lane(it, lane, rowmap);
 
   for (int32_t p = it.pbeg; p < it.pend; p += STEP) {
     const uint4 a = load_a<BK>(values, p, it.pend, r16, grp);
@@ -284,7 +299,7 @@ __global__ __launch_bounds__(256) void k_spmm_dense(
       acc[j] = mma<DT>(a, b, acc[j]);
     }
   }
-  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, C, ldc, scratch, rowmap);
+  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, C, ldc, scratch, rowmap, crl);
 }
 
 // Element codecs of the LDS kernel.  DT 0 = FP16, 1 = BF16 (A values and B
@@ -455,6 +470,7 @@ __device__ __forceinline__ void lds_tile(const Item& it, int64_t n0, const Opera
   const int64_t ldb = op.ldb, N = op.N;
 
   const int r16 = lane & 15, grp = lane >> 4;
+  const int32_t crl = c_row_of_lane(it, lane, op.rowmap);
   // load coordinates: instruction i covers rows i*RPL .. i*RPL+RPL-1
   const int lrow = lane / CPR, lch = lane % CPR;
   const int64_t ncol = n0 + lch * (16 / ES);
@@ -554,11 +570,16 @@ __device__ __forceinline__ void lds_tile(const Item& it, int64_t n0, const Opera
       for (int i = 0; i < 4; i++) acc[j][i] *= sc;
     }
   }
-  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, op.c, op.ldc, op.scratch, op.rowmap);
+  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, op.c, op.ldc, op.scratch, op.rowmap, crl);
 }
 
+// Register budgets of the 64-column instances: 4 waves per SIMD for FP16 /
+// BF16 (128 VGPR + AGPR), 3 for split FP16 at bk = 1.
 template <int BK, int NSUB, int DT, bool OUT_F16>
-__global__ __launch_bounds__(256) void k_spmm_lds(const Item* __restrict__ items, int64_t n_items, int32_t ntiles,
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(NSUB != 4 ? 1 : DT <= 1 ? 4 : DT == 3 && BK == 1 ? 3 : 1)))
+void k_spmm_lds(
+    const Item* __restrict__ items, int64_t n_items, int32_t ntiles,
                                                   Operands op, int32_t order) {
   constexpr int WB = lds_wave_bytes<NSUB, DT>();
   __shared__ __attribute__((aligned(16))) uint8_t lds_all[kWaves * WB];
@@ -609,6 +630,7 @@ __device__ __forceinline__ void mixed_tile(const Item& it, int64_t n0, const Ope
   const uint16_t* __restrict__ index24 = mx.index24;
 
   const int r16 = lane & 15, grp = lane >> 4;
+  const int32_t crl = c_row_of_lane(it, lane, op.rowmap);
   const int lrow = lane / CPR, lch = lane % CPR;
   const int64_t ncol = n0 + lch * 8;
   const bool col_ok = ncol < N;
@@ -695,7 +717,7 @@ __device__ __forceinline__ void mixed_tile(const Item& it, int64_t n0, const Ope
       }
     }
   }
-  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, op.c, op.ldc, op.scratch, op.rowmap);
+  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, op.c, op.ldc, op.scratch, op.rowmap, crl);
 }
 
 template <int NSUB, int DT, bool OUT_F16>

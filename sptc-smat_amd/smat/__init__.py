@@ -146,6 +146,7 @@ SIGNATURES = {
     "smat_grouping_to_permutation": (_i32, [_i64, _p, _p]),
     "smat_blocking_info": (_i32, [_i64, _i64, _i64, _p, _p, _p, _i64, _p, _p, _p]),
     "smat_permute_csr_rows": (_i32, [_i64, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "smat_locality_order": (_i32, [_i64, _i64, _p, _p, _p, _p]),
     "mmaOBTKernel": (None, [_p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
     "mmaOBTSKernel": (None, [_p, _p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
     "preprocessing_mmaSTKernel": (None, [_p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
@@ -457,6 +458,28 @@ def fixed_size_grouping(grouping, row_block: int):
     out = np.zeros_like(g)
     check(lib().smat_fixed_size_grouping(g.size, g.ctypes.data, row_block, out.ctypes.data))
     return out
+
+
+def locality_order(rows: int, cols: int, rowptr, colidx, stream=None):
+    """Locality row order of a device CSR (smat_locality_order): a device
+    int64 tensor, new row r = old row perm[r]."""
+    import torch
+    perm = torch.empty(rows, dtype=torch.int64, device="cuda")
+    check(lib().smat_locality_order(rows, cols, ptr(rowptr), ptr(colidx), ptr(perm), stream_ptr(stream)))
+    return perm
+
+
+def permute_rows(rowptr, colidx, values, perm, stream=None):
+    """Rows of a device CSR permuted on the GPU (smat_permute_csr_rows):
+    (rowptr, colidx, values) device tensors, new row r = old row perm[r]."""
+    import torch
+    rows = perm.numel()
+    out_rp = torch.empty(rows + 1, dtype=torch.int32, device="cuda")
+    out_ci = torch.empty_like(colidx)
+    out_v = torch.empty_like(values) if values is not None else None
+    check(lib().smat_permute_csr_rows(rows, ptr(rowptr), ptr(colidx), ptr(values), ptr(perm), ptr(out_rp),
+                                      ptr(out_ci), ptr(out_v), stream_ptr(stream)))
+    return out_rp, out_ci, out_v
 
 
 def grouping_to_permutation(grouping):

@@ -251,3 +251,79 @@ def test_blocking_info_errors_and_empty():
     z = torch.zeros(3, dtype=torch.int32).cuda()
     info, avg = smat.blocking_info(2, 5, z, None, torch.tensor([0, 1], dtype=torch.int64).cuda(), 2)
     assert info.tolist() == [0, 0, 0] and np.isnan(avg)
+
+
+def _rows_csr(rows, cols, seed, max_len=40, unsorted=False):
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, max_len, size=rows)
+    lens[rng.random(rows) < 0.1] = 0
+    parts = []
+    for n in lens:
+        c = rng.choice(cols, size=int(n), replace=False) if cols < 1 << 20 else \
+            np.unique(rng.integers(0, cols, size=int(n)))
+        c = np.sort(c)
+        if unsorted:
+            c = rng.permutation(c)
+        parts.append(c)
+    rp = np.concatenate([[0], np.cumsum([p.size for p in parts])]).astype(np.int32)
+    ci = np.concatenate(parts).astype(np.int32) if rp[-1] else np.zeros(0, np.int32)
+    return rp, ci
+
+
+@pytest.mark.parametrize("case", ["rmat12", "powerlaw", "random", "unsorted", "wide_cols"])
+def test_locality_order_matches_oracle(case):
+    """GPU locality order (stable radix sort of the row keys) equals the
+    oracle's lexsort, bit for bit, including empty rows, rows in file order
+    and more than 2^29 columns (second key dropped)."""
+    _need_gpu()
+    import smat
+    if case == "rmat12":
+        M, K, rp, ci, _ = gen.rmat(scale=12, seed=5)
+    elif case == "powerlaw":
+        M, K, rp, ci, _ = gen.power_law(n=8192, seed=6)[:5]
+    else:
+        M = 3000
+        K = (1 << 30) + 5 if case == "wide_cols" else 1000
+        rp, ci = _rows_csr(M, K, seed=len(case), unsorted=case == "unsorted")
+    print(f"locality order {case}: rows {M} cols {K} nnz {ci.size}", flush=True)
+    d_rp = torch.from_numpy(rp).cuda()
+    d_ci = torch.from_numpy(ci if ci.size else np.zeros(1, np.int32)).cuda()
+    perm = smat.locality_order(M, K, d_rp, d_ci).cpu().numpy()
+    assert np.array_equal(perm, of.locality_order(M, K, rp, ci))
+
+
+def test_locality_order_spmm_writes_original_rows():
+    """Reordered SpMM end to end: locality order, GPU row permutation, BCSR,
+    plan, and C written back through c_rowmap equals the FP64 product of the
+    original matrix."""
+    _need_gpu()
+    import smat
+    M, K, rp, ci, v = gen.rmat(scale=13, seed=9)
+    N = 64
+    d = [torch.from_numpy(x).cuda() for x in (rp, ci, v.view(np.int16))]
+    perm = smat.locality_order(M, K, d[0], d[1])
+    prp, pci, pv = smat.permute_rows(d[0], d[1], d[2], perm)
+    rowmap = perm.to(torch.int32)
+    B = gen.dense_b(K, N, seed=3)
+    Bd = torch.from_numpy(B.view(np.int16)).cuda()
+    R = of.spmm_csr_f64(M, rp, ci, v, B)
+    for bk in (1, 2, 16):
+        A = smat.Bcsr(M, K, prp, pci, pv, bk=bk)
+        plan = smat.SpmmPlan(A, N)
+        C = torch.full((M, N), float("nan"), device="cuda")
+        plan.execute(Bd, C, c_rowmap=rowmap)
+        torch.cuda.synchronize()
+        Cn = C.cpu().numpy().astype(np.float64)
+        assert np.linalg.norm(Cn - R) / np.linalg.norm(R) < 1e-5, bk
+        plan.close()
+        A.close()
+
+
+def test_locality_order_rejects_bad_arguments():
+    _need_gpu()
+    import smat
+    d = torch.zeros(4, dtype=torch.int32, device="cuda")
+    out = torch.empty(4, dtype=torch.int64, device="cuda")
+    assert smat.lib().smat_locality_order(0, 4, d.data_ptr(), d.data_ptr(), out.data_ptr(), None) == \
+        smat.SMAT_ERR_INVALID
+    assert smat.lib().smat_locality_order(3, 4, d.data_ptr(), None, out.data_ptr(), None) == smat.SMAT_ERR_INVALID

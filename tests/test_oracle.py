@@ -294,3 +294,46 @@ def test_fp16_conversions():
         assert of.lib().or_f2h(float(x)) == int(np.float16(x).view(np.uint16))
     for h in range(0, 0x7C00, 37):
         assert of.lib().or_h2f(h) == float(np.uint16(h).view(np.float16))
+
+
+# ---------------------------------------------------------------- locality order
+def _locality_brute(rows, cols, rp, ci, with_second):
+    """The order's definition as a plain Python sort of per-row tuples."""
+    def key(r):
+        row = list(ci[rp[r]:rp[r + 1]])
+        n = len(row)
+        lb = n.bit_length() - 1 if n > 1 else 0
+        c0 = row[0] if n > 0 else cols
+        c1 = row[1] if n > 1 else cols
+        return (-lb, c0, c1, r) if with_second else (-lb, c0, r)
+    return sorted(range(rows), key=key)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_locality_order_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    rows, cols = 200, 97
+    lens = rng.integers(0, 12, size=rows)
+    lens[rng.random(rows) < 0.1] = 0            # empty rows
+    lens[:3] = [33, 64, 65]                      # bucket edges 32 / 64
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ci = np.concatenate([np.sort(rng.choice(cols, size=min(n, cols), replace=False)) for n in lens]).astype(np.int32)
+    rp = np.concatenate([[0], np.cumsum(np.minimum(lens, cols))]).astype(np.int32)
+    perm = of.locality_order(rows, cols, rp, ci)
+    assert perm.tolist() == _locality_brute(rows, cols, rp, ci, True)
+    # a planted mistake (length buckets ascending instead of descending) is caught
+    lens_k = np.diff(rp)
+    planted = sorted(range(rows), key=lambda r: (int(lens_k[r]).bit_length() - 1 if lens_k[r] > 1 else 0,
+                                                 ci[rp[r]] if lens_k[r] else cols, r))
+    assert perm.tolist() != planted
+
+
+def test_locality_order_wide_columns_drop_second_key():
+    rows, cols = 6, 1 << 30
+    rp = np.array([0, 2, 4, 5, 6, 7, 9], np.int32)
+    ci = np.array([5, 9, 5, 7, 5, 3, 1, 5, 6], np.int32)
+    perm = of.locality_order(rows, cols, rp, ci)
+    # rows 0, 1, 5 have two entries (bucket 1) and tie on the first column:
+    # row order decides, where the second column would give 5, 1, 0
+    assert perm.tolist() == _locality_brute(rows, cols, rp, ci, False) == [0, 1, 5, 4, 3, 2]
+    assert of.locality_order(rows, 1 << 20, rp, ci).tolist() == [5, 1, 0, 4, 3, 2]

@@ -200,25 +200,26 @@ __device__ inline uint4 load_a(const uint16_t* __restrict__ values, int32_t p, i
 }
 
 // With a row map (A's rows reordered) row r of A goes to C row rowmap[r].
-// Each lane reads the C row of tile row lane & 15 when the task starts, so
-// the load's latency hides behind the task's main loop (one VGPR);
-// store_tile gathers its four rows from lanes 0..15.
+// PRE (calls with a row map, 64-column FP16 / BF16 tiles): each lane read the
+// C row of tile row lane & 15 when the task started (c_row_of_lane), so the
+// load's latency hid behind the task's main loop, and store_tile gathers its
+// four rows from lanes 0..15 with shuffles.  Otherwise rows are read from the
+// map at the end (a dependent load before the stores).
 __device__ inline int32_t c_row_of_lane(const Item& it, int lane, const int32_t* __restrict__ rowmap) {
   return rowmap && it.slot < 0 ? rowmap[(int64_t)it.brow * 16 + (lane & 15)] : 0;
 }
 
 // D layout: lane holds rows 4*grp + i, column 16*j + r16 of its tile.  crl:
-// c_row_of_lane of this lane (used when rowmap is set).  Called by the whole
-// wave.
-template <int NSUB, bool OUT_F16>
+// c_row_of_lane of this lane when PRE (then rowmap is set and the whole wave
+// calls this).
+template <int NSUB, bool OUT_F16, bool PRE = false>
 __device__ inline void store_tile(const smat_f4 (&acc)[NSUB], const Item& it, int64_t n0, int r16, int grp,
                                   int64_t N, void* __restrict__ C, int64_t ldc, float* __restrict__ scratch,
-                                  const int32_t* __restrict__ rowmap, int32_t crl) {
+                                  const int32_t* __restrict__ rowmap, int32_t crl = 0) {
   int64_t cr[4];
+  if constexpr (PRE) {
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int row = 4 * grp + i;
-    cr[i] = rowmap ? (int64_t)__shfl(crl, row, 64) : (int64_t)it.brow * 16 + row;
+    for (int i = 0; i < 4; i++) cr[i] = (int64_t)__shfl(crl, 4 * grp + i, 64);
   }
 #pragma unroll
   for (int j = 0; j < NSUB; j++) {
@@ -230,7 +231,9 @@ __device__ inline void store_tile(const smat_f4 (&acc)[NSUB], const Item& it, in
       if (it.slot >= 0) {
         scratch[((int64_t)it.slot * 16 + row) * N + n] = acc[j][i];
       } else {
-        const int64_t crow = cr[i];
+        int64_t crow = (int64_t)it.brow * 16 + row;
+        if constexpr (PRE) crow = cr[i];
+        else if (rowmap) crow = rowmap[crow];
         if (OUT_F16)
           reinterpret_cast<_Float16*>(C)[crow * ldc + n] = (_Float16)acc[j][i];
         else
@@ -264,7 +267,6 @@ __global__ __launch_bounds__(256) void k_spmm_dense(
   const int r16 = lane & 15;   // A row / B column within the tile
   const int grp = lane >> 4;   // 0..3: k-slot group of 8
   constexpr int STEP = 32 / BK;  // blocks per MFMA step
-  const int32_t crl = c_row_of_lane(it, lane, rowmap);
 
   for (int32_t p = it.pbeg; p < it.pend; p += STEP) {
     const uint4 a = load_a<BK>(values, p, it.pend, r16, grp);
@@ -299,7 +301,7 @@ __global__ __launch_bounds__(256) void k_spmm_dense(
       acc[j] = mma<DT>(a, b, acc[j]);
     }
   }
-  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, C, ldc, scratch, rowmap, crl);
+  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, C, ldc, scratch, rowmap);
 }
 
 // Element codecs of the LDS kernel.  DT 0 = FP16, 1 = BF16 (A values and B
@@ -452,7 +454,7 @@ constexpr int lds_wave_bytes() {
   return Codec<DT>::NARR * 32 * (16 * NSUB * Codec<DT>::ES + 16);
 }
 
-template <int BK, int NSUB, int DT, bool OUT_F16>
+template <int BK, int NSUB, int DT, bool OUT_F16, bool PRE = false>
 __device__ __forceinline__ void lds_tile(const Item& it, int64_t n0, const Operands& op, uint8_t* __restrict__ lds,
                                          int lane) {
   using CD = Codec<DT>;
@@ -470,7 +472,7 @@ __device__ __forceinline__ void lds_tile(const Item& it, int64_t n0, const Opera
   const int64_t ldb = op.ldb, N = op.N;
 
   const int r16 = lane & 15, grp = lane >> 4;
-  const int32_t crl = c_row_of_lane(it, lane, op.rowmap);
+  const int32_t crl = PRE ? c_row_of_lane(it, lane, op.rowmap) : 0;
   // load coordinates: instruction i covers rows i*RPL .. i*RPL+RPL-1
   const int lrow = lane / CPR, lch = lane % CPR;
   const int64_t ncol = n0 + lch * (16 / ES);
@@ -570,17 +572,12 @@ __device__ __forceinline__ void lds_tile(const Item& it, int64_t n0, const Opera
       for (int i = 0; i < 4; i++) acc[j][i] *= sc;
     }
   }
-  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, op.c, op.ldc, op.scratch, op.rowmap, crl);
+  store_tile<NSUB, OUT_F16, PRE>(acc, it, n0, r16, grp, N, op.c, op.ldc, op.scratch, op.rowmap, crl);
 }
 
-// Register budgets of the 64-column instances: 4 waves per SIMD for FP16 /
-// BF16 (128 VGPR + AGPR), 3 for split FP16 at bk = 1.
-template <int BK, int NSUB, int DT, bool OUT_F16>
-__global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(NSUB != 4 ? 1 : DT <= 1 ? 4 : DT == 3 && BK == 1 ? 3 : 1)))
-void k_spmm_lds(
-    const Item* __restrict__ items, int64_t n_items, int32_t ntiles,
-                                                  Operands op, int32_t order) {
+template <int BK, int NSUB, int DT, bool OUT_F16, bool PRE>
+__device__ __forceinline__ void lds_kernel_body(const Item* __restrict__ items, int64_t n_items, int32_t ntiles,
+                                                const Operands& op, int32_t order) {
   constexpr int WB = lds_wave_bytes<NSUB, DT>();
   __shared__ __attribute__((aligned(16))) uint8_t lds_all[kWaves * WB];
   const int lane = threadIdx.x & 63;
@@ -589,7 +586,22 @@ void k_spmm_lds(
   int nt;
   if (!wave_task(blockIdx.x, gridDim.x, wave, n_items, ntiles, order, item_id, nt)) return;
   const Item it = items[item_id];
-  lds_tile<BK, NSUB, DT, OUT_F16>(it, (int64_t)nt * 16 * NSUB, op, lds_all + wave * WB, lane);
+  lds_tile<BK, NSUB, DT, OUT_F16, PRE>(it, (int64_t)nt * 16 * NSUB, op, lds_all + wave * WB, lane);
+}
+
+template <int BK, int NSUB, int DT, bool OUT_F16>
+__global__ __launch_bounds__(256) void k_spmm_lds(const Item* __restrict__ items, int64_t n_items, int32_t ntiles,
+                                                  Operands op, int32_t order) {
+  lds_kernel_body<BK, NSUB, DT, OUT_F16, false>(items, n_items, ntiles, op, order);
+}
+
+// Calls with a row map on 64-column FP16 / BF16 tiles (NSUB = 4, DT <= 1):
+// the row map read when a task starts (PRE), the kernel held to 4 waves per
+// SIMD (128 VGPR + AGPR), which the row-map VGPR would otherwise cost.
+template <int BK, int DT, bool OUT_F16>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_spmm_lds64(
+    const Item* __restrict__ items, int64_t n_items, int32_t ntiles, Operands op, int32_t order) {
+  lds_kernel_body<BK, 4, DT, OUT_F16, true>(items, n_items, ntiles, op, order);
 }
 
 // Mixed dense / 2:4 kernel for 16x16 blocks and row-major B.  Dense blocks
@@ -630,7 +642,6 @@ __device__ __forceinline__ void mixed_tile(const Item& it, int64_t n0, const Ope
   const uint16_t* __restrict__ index24 = mx.index24;
 
   const int r16 = lane & 15, grp = lane >> 4;
-  const int32_t crl = c_row_of_lane(it, lane, op.rowmap);
   const int lrow = lane / CPR, lch = lane % CPR;
   const int64_t ncol = n0 + lch * 8;
   const bool col_ok = ncol < N;
@@ -717,7 +728,7 @@ __device__ __forceinline__ void mixed_tile(const Item& it, int64_t n0, const Ope
       }
     }
   }
-  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, op.c, op.ldc, op.scratch, op.rowmap, crl);
+  store_tile<NSUB, OUT_F16>(acc, it, n0, r16, grp, N, op.c, op.ldc, op.scratch, op.rowmap);
 }
 
 template <int NSUB, int DT, bool OUT_F16>
@@ -1116,6 +1127,13 @@ template <int BK, int NSUB, int DT, bool OUT_F16>
 void launch_lds(const smat_plan_t* P, const Operands& op, hipStream_t s) {
   const int32_t ntiles = ntiles_of(P, NSUB);
   const int32_t order = order_of(P, ntiles);
+  if constexpr (NSUB == 4 && DT <= 1) {
+    if (op.rowmap) {
+      hipLaunchKernelGGL((k_spmm_lds64<BK, DT, OUT_F16>), grid_of(P, ntiles, order), 64 * kWaves, 0, s,
+                         P->items.as<Item>(), P->n_items, ntiles, op, order);
+      return;
+    }
+  }
   hipLaunchKernelGGL((k_spmm_lds<BK, NSUB, DT, OUT_F16>), grid_of(P, ntiles, order), 64 * kWaves, 0, s,
                      P->items.as<Item>(), P->n_items, ntiles, op, order);
 }

@@ -336,9 +336,10 @@ def _gen_batch_matrix(seed):
     return tuner.make_matrix(seed)
 
 
-def batch_plan(ctx, M, K, rp, ci, v, cfg, N):
-    """Device CSR (reordered when the configuration asks), BCSR, plan and
-    operands of one matrix of the batched workload."""
+def batch_plan(ctx, M, K, rp, ci, v, cfg, N, locality=False):
+    """Device CSR (reordered when the configuration asks: the Keeper for tau,
+    the locality row order when `locality`), BCSR, plan and operands of one
+    matrix of the batched workload."""
     import numpy as np
     import torch
     import smat
@@ -350,6 +351,10 @@ def batch_plan(ctx, M, K, rp, ci, v, cfg, N):
         rowmap = torch.from_numpy(perm.astype(np.int32)).cuda()
     d_rp, d_ci = torch.from_numpy(rp).cuda(), torch.from_numpy(ci if ci.size else np.zeros(1, np.int32)).cuda()
     d_v = torch.from_numpy((v if v.size else np.zeros(1, np.uint16)).view(np.int16)).cuda()
+    if locality and tau is None:
+        perm = smat.locality_order(M, K, d_rp, d_ci)
+        d_rp, d_ci, d_v = smat.permute_rows(d_rp, d_ci, d_v, perm)
+        rowmap = perm.to(torch.int32)
     mixed = kernel == "mixed24"
     A = smat.Bcsr(M, K, d_rp, d_ci, d_v, bk=bk, tag_rule=smat.SMAT_TAG_EXACT24 if mixed else smat.SMAT_TAG_REFERENCE)
     v24 = i24 = None
@@ -406,8 +411,37 @@ def batched_workload(args, log):
         single_ms[1] += float(times[sb])
     log(f"features, model picks and exhaustive sweeps in {time.time() - t_sweep:.1f}s")
 
-    def build(cfgs):
-        return [batch_plan(None, M, K, rp, ci, v, cfg, N) for (kind, (M, K, rp, ci, v)), cfg in zip(mats, cfgs)]
+    # matrices above the Keeper's limit: the learned pick on the locality-ordered
+    # copy against the pick as given, each probed over 5 calls; the faster is kept
+    t_loc = time.time()
+    ordered = [False] * 64
+    if not args.no_locality:
+        for i, (kind, (M, K, rp, ci, v)) in enumerate(mats):
+            if M <= tuner.REORDER_MAX_ROWS or picks[i][0] is not None:
+                continue
+            d = [torch.from_numpy(x).cuda() for x in (rp, ci, v.view(np.int16))]
+            perm = smat.locality_order(M, K, d[0], d[1])
+            o = smat.permute_rows(d[0], d[1], d[2], perm)
+            _, bo = model.scores(tuner.features(tuner.stats(M, K, *o, int(ci.size)), N), tuner.available(M))
+            cand = tuner.CONFIGS[bo]
+            if cand[0] is not None:
+                continue
+            ms = []
+            for cfg, loc in ((picks[i], False), (cand, True)):
+                b = batch_plan(None, M, K, rp, ci, v, cfg, N, locality=loc)
+                ms.append(tuner.time_call(lambda: b["plan"].execute(Bs[i], Cs[i], values24=b["v24"],
+                                                                    index24=b["i24"], c_rowmap=b["rowmap"])))
+                b["plan"].close()
+                b["A"].close()
+            if ms[1] < ms[0]:
+                picks[i], ordered[i] = cand, True
+    log(f"locality order kept for {sum(ordered)} of {sum(m[1][0] > tuner.REORDER_MAX_ROWS for m in mats)} "
+        f"large matrices ({time.time() - t_loc:.1f}s)")
+
+    def build(cfgs, orders=None):
+        orders = orders or [False] * 64
+        return [batch_plan(None, M, K, rp, ci, v, cfg, N, locality=o)
+                for (kind, (M, K, rp, ci, v)), cfg, o in zip(mats, cfgs, orders)]
 
     def ops(built):
         return [(b["plan"], b["plan"].operands(Bs[i], Cs[i], values24=b["v24"], index24=b["i24"],
@@ -434,7 +468,7 @@ def batched_workload(args, log):
     legs = [("tuned", picks)] if args.batch_tuned_only else \
         [("plain", [tuner.CONFIGS[tuner.PLAIN]] * 64), ("tuned", picks), ("sweep_best", sweep_best)]
     for name, cfgs in legs:
-        built = build(cfgs)
+        built = build(cfgs, ordered if name == "tuned" else None)
         if name == "plain":  # one launch per matrix, plans with the library's default slices
             seq_plans = [smat.SpmmPlan(b["A"], N) for b in built]
             with torch.cuda.stream(s):
@@ -474,7 +508,8 @@ def batched_workload(args, log):
                       "sum_of_single_calls_ms": {"tuned": round(single_ms[0], 4),
                                                  "sweep_best": round(single_ms[1], 4)},
                       "ratio_per_matrix": [round(x, 3) for x in within],
-                      "picks": [str(c) for c in picks]},
+                      "picks": [str(c) for c in picks],
+                      "locality_ordered": [i for i, o in enumerate(ordered) if o]},
             "kinds": [m[0] for m in mats], "rows": [int(m[1][0]) for m in mats],
             "nnz": [int(m[1][3].size) for m in mats]}
 

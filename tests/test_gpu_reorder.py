@@ -292,29 +292,55 @@ def test_locality_order_matches_oracle(case):
     assert np.array_equal(perm, of.locality_order(M, K, rp, ci))
 
 
-def test_locality_order_spmm_writes_original_rows():
+@pytest.mark.parametrize("variant", ["f16", "split_rows", "f16_out", "bf16"])
+def test_locality_order_spmm_writes_original_rows(variant):
     """Reordered SpMM end to end: locality order, GPU row permutation, BCSR,
-    plan, and C written back through c_rowmap equals the FP64 product of the
-    original matrix."""
+    plan, and C written back through c_rowmap (the row-map LDS kernel for
+    64-column 16-bit tiles) equals the FP64 product of the original matrix;
+    also with every block row cut into 2-step slices (partial tiles reduced
+    through the map), FP16 C, and BF16 inputs."""
     _need_gpu()
     import smat
     M, K, rp, ci, v = gen.rmat(scale=13, seed=9)
     N = 64
-    d = [torch.from_numpy(x).cuda() for x in (rp, ci, v.view(np.int16))]
+    vals = v
+    if variant == "bf16":
+        vals = gen.f32_to_bf16_bits(v.view(np.float16).astype(np.float32))
+    d = [torch.from_numpy(x).cuda() for x in (rp, ci, vals.view(np.int16))]
     perm = smat.locality_order(M, K, d[0], d[1])
     prp, pci, pv = smat.permute_rows(d[0], d[1], d[2], perm)
     rowmap = perm.to(torch.int32)
     B = gen.dense_b(K, N, seed=3)
+    if variant == "bf16":
+        B = gen.f32_to_bf16_bits(B.view(np.float16).astype(np.float32)).reshape(K, N)
+        A64 = gen.bf16_bits_to_f32(vals).astype(np.float64)
+        B64 = gen.bf16_bits_to_f32(B).reshape(K, N).astype(np.float64)
+        import scipy.sparse as sp
+        R = sp.csr_matrix((A64, ci, rp), shape=(M, K)) @ B64
+    else:
+        R = of.spmm_csr_f64(M, rp, ci, v, B)
     Bd = torch.from_numpy(B.view(np.int16)).cuda()
-    R = of.spmm_csr_f64(M, rp, ci, v, B)
+    print(f"locality SpMM {variant}", flush=True)
     for bk in (1, 2, 16):
         A = smat.Bcsr(M, K, prp, pci, pv, bk=bk)
-        plan = smat.SpmmPlan(A, N)
-        C = torch.full((M, N), float("nan"), device="cuda")
+        kw = {}
+        if variant == "split_rows":
+            kw["max_blocks_per_item"] = 2 * (32 // bk)
+        if variant == "f16_out":
+            kw["out_dtype"] = smat.SMAT_F16
+        if variant == "bf16":
+            kw["in_dtype"] = smat.SMAT_BF16
+        plan = smat.SpmmPlan(A, N, **kw)
+        if variant == "split_rows":
+            assert plan.info().n_split_rows > 0
+        dt = torch.float16 if variant == "f16_out" else torch.float32
+        C = torch.full((M, N), float("nan"), dtype=dt, device="cuda")
         plan.execute(Bd, C, c_rowmap=rowmap)
         torch.cuda.synchronize()
-        Cn = C.cpu().numpy().astype(np.float64)
-        assert np.linalg.norm(Cn - R) / np.linalg.norm(R) < 1e-5, bk
+        Cn = C.float().cpu().numpy().astype(np.float64)
+        tol = 1e-3 if variant == "f16_out" else 1e-5
+        assert np.all(np.isfinite(Cn)), (variant, bk)
+        assert np.linalg.norm(Cn - R) / np.linalg.norm(R) < tol, (variant, bk)
         plan.close()
         A.close()
 

@@ -28,8 +28,12 @@ __device__ inline int len_bucket(int64_t len) {
   return b > 11 ? 11 : b;
 }
 
-// Per row: histogram, longest row; per entry: |row - col| sum, entries near the
-// diagonal, and the sort key (block row << colbits | column).  Rows with
+// Rows longer than this get a wave each for their per-entry statistics.
+constexpr int32_t kLongRow = 256;
+
+// Per row: histogram, longest row; per entry (rows up to kLongRow entries):
+// |row - col| sum, entries near the diagonal, and the sort key
+// (block row << colbits | column).  Rows with
 // ascending columns also report their violations of the exact 2:4 rule (an
 // aligned 1x4 group with more than two nonzero values, +-0 counting as zero):
 // one key (block row << (colbits - 4) | column / 16) per violating group,
@@ -61,6 +65,7 @@ __global__ __launch_bounds__(256) void k_row_stats(int64_t M, const int32_t* __r
     const int64_t len = e - b;
     atomicAdd(&hist[len_bucket(len)], 1ull);
     atomicMax(&mx, (unsigned long long)len);
+    if (len > kLongRow) e = b;  // entries of long rows: k_row_stats_long
     unsigned long long s = 0, nd = 0;
     int32_t prev = -1, g_prev = -1, cnt = 0;
     bool sorted = true;
@@ -106,6 +111,69 @@ __global__ __launch_bounds__(256) void k_row_stats(int64_t M, const int32_t* __r
     atomicMax(&acc[12], mx);
     atomicAdd(&acc[13], sums[0]);
     atomicAdd(&acc[14], sums[1]);
+  }
+}
+
+// Per-entry statistics of rows longer than kLongRow, one wave per row (a
+// workgroup of 4 waves takes 4 rows; short rows exit): the same sums, sort
+// keys and strict-ascending check as k_row_stats.  In a row whose columns
+// ascend, an entry is the third nonzero value of its aligned 1x4 group
+// exactly when two of the (at most three) entries before it are nonzero
+// values of the same group, so each lane finds its violations by looking
+// back three entries; slots in vkeys are reserved per 64-entry chunk.  Rows
+// that do not ascend only set *unsorted (their keys then go unused), and
+// no write passes vcap.
+template <typename KT>
+__global__ __launch_bounds__(256) void k_row_stats_long(int64_t M, const int32_t* __restrict__ rp,
+                                                        const int32_t* __restrict__ ci,
+                                                        const uint16_t* __restrict__ val, int colbits,
+                                                        unsigned long long* __restrict__ acc,
+                                                        KT* __restrict__ keys, KT* __restrict__ vkeys, int64_t vcap,
+                                                        unsigned long long* __restrict__ vcount,
+                                                        int32_t* __restrict__ unsorted) {
+  const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= M) return;
+  const int32_t b = rp[r], e = rp[r + 1];
+  if (e - b <= kLongRow) return;
+  unsigned long long s = 0, nd = 0;
+  bool sorted = true;
+  for (int32_t q0 = b; q0 < e; q0 += 64) {
+    const int32_t q = q0 + lane;
+    bool viol = false;
+    int32_t c = 0;
+    if (q < e) {
+      c = ci[q];
+      const int64_t d = r > c ? r - (int64_t)c : (int64_t)c - r;
+      s += (unsigned long long)d;
+      nd += d <= 64;
+      keys[q] = ((KT)(r / 16) << colbits) | (KT)(uint32_t)c;
+      if (q > b) sorted = sorted && c > ci[q - 1];
+      if ((val[q] & 0x7fff) != 0) {
+        int before = 0;
+        for (int k = 1; k <= 3 && q - k >= b; k++)
+          before += (ci[q - k] >> 2) == (c >> 2) && (val[q - k] & 0x7fff) != 0;
+        viol = before == 2;
+      }
+    }
+    const unsigned long long mask = __ballot(viol);
+    if (mask) {
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(vcount, (unsigned long long)__popcll(mask));
+      base = __shfl(base, 0, 64);
+      const unsigned long long below = lane ? mask & (~0ull >> (64 - lane)) : 0ull;
+      const int64_t pos = (int64_t)(base + (unsigned long long)__popcll(below));
+      if (viol && pos < vcap) vkeys[pos] = ((KT)(r / 16) << (colbits - 4)) | (KT)(uint32_t)(c >> 4);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    s += __shfl_down(s, o, 64);
+    nd += __shfl_down(nd, o, 64);
+  }
+  if (__any(!sorted) && lane == 0) atomicOr(unsorted, 1);
+  if (lane == 0) {
+    atomicAdd(&acc[13], s);
+    atomicAdd(&acc[14], nd);
   }
 }
 
@@ -312,6 +380,8 @@ smat_status stats_pass(int64_t M, int64_t nnz, const int32_t* rowptr, const int3
   int32_t* unsorted = reinterpret_cast<int32_t*>(misc.as<unsigned long long>() + 1);
   hipLaunchKernelGGL(k_row_stats<KT>, grid_of(M, 256), 256, 0, s, M, rowptr, colidx, values, colbits, a,
                      keys.as<KT>(), vkeys.as<KT>(), vcount, unsorted);
+  hipLaunchKernelGGL(k_row_stats_long<KT>, grid_of(M, 4), 256, 0, s, M, rowptr, colidx, values, colbits, a,
+                     keys.as<KT>(), vkeys.as<KT>(), nnz / 3 + 1, vcount, unsorted);
   SMAT_LAUNCH_CHECK();
   if (nnz > 0) {
     size_t tb = 0;

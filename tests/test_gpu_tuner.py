@@ -37,7 +37,8 @@ def test_stats_match_oracle(seed):
     assert st.as_dict() == of.matrix_stats(M, K, rp, ci, v)
 
 
-@pytest.mark.parametrize("case", ["stored_zeros", "unsorted_rows", "wide_64bit_keys"])
+@pytest.mark.parametrize("case", ["stored_zeros", "unsorted_rows", "wide_64bit_keys", "long_rows",
+                                  "long_rows_unsorted"])
 def test_stats_edge_cases_match_oracle(case):
     """Exact 2:4 block count with stored +0 / -0 values (they do not count),
     with rows whose columns do not ascend (the 16 x 16 BCSR route), and with
@@ -45,7 +46,32 @@ def test_stats_edge_cases_match_oracle(case):
     _need_gpu()
     from smat import gen, tuner
     rng = np.random.default_rng(7)
-    if case == "wide_64bit_keys":
+    if case.startswith("long_rows"):
+        # rows of 257-3000 entries (one wave each) with dense column runs
+        # (full 1x4 groups), stored +0 / -0 values, and short rows between
+        M, K = 2048, 8192
+        parts, vparts = [], []
+        for r in range(M):
+            if r % 20 == 0:
+                run0 = int(rng.integers(0, K - 512))
+                cols = np.union1d(np.arange(run0, run0 + int(rng.integers(64, 512))),
+                                  rng.choice(K, size=int(rng.integers(200, 2500)), replace=False))
+            else:
+                cols = np.sort(rng.choice(K, size=int(rng.integers(0, 12)), replace=False))
+            vals = np.float16(rng.uniform(0.5, 1.0, cols.size)).view(np.uint16)
+            z = rng.random(cols.size)
+            vals[z < 0.25] = 0x0000
+            vals[(z >= 0.25) & (z < 0.4)] = 0x8000
+            if case == "long_rows_unsorted" and r % 40 == 0:
+                p_ = rng.permutation(cols.size)
+                cols, vals = cols[p_], vals[p_]
+            parts.append(cols)
+            vparts.append(vals)
+        rp = np.concatenate([[0], np.cumsum([c.size for c in parts])]).astype(np.int32)
+        ci = np.concatenate(parts).astype(np.int32)
+        v = np.concatenate(vparts).astype(np.uint16)
+        assert np.diff(rp).max() > 256
+    elif case == "wide_64bit_keys":
         M, K = 131072, 1 << 22  # 13 block-row bits + 22 column bits
         rows = np.sort(rng.choice(M, size=4000, replace=False))
         per = rng.integers(1, 9, size=rows.size)

@@ -16,8 +16,12 @@
  *     hold the load-balanced work decomposition.
  *
  * All pointers are device pointers unless the parameter name ends in `_host`.
- * Nothing here allocates except *_create / smat_bcsr_from_csr, and the
- * matching *_destroy frees.  Calls on distinct objects and streams may run
+ * Objects are allocated by *_create / smat_bcsr_from_csr and freed by the
+ * matching *_destroy.  The SpMM execute calls allocate nothing (a plan owns
+ * its scratch).  The preprocessing and statistics calls (conversion, filter,
+ * compression, reorder, locality order, blocking info, matrix statistics,
+ * the reference drop-in packers) allocate temporary device memory and free
+ * it before they return.  Calls on distinct objects and streams may run
  * concurrently; one object must not be used from two threads at once.
  */
 #ifndef SMAT_H_

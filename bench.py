@@ -307,28 +307,17 @@ def locality_variant(ctx):
 
 
 def locality_choice(ctx, learned, log):
-    """For a matrix too large for the exact Keeper: the learned pick on its
-    locality-ordered copy against the learned pick on the matrix as given,
-    each timed over a short probe (2 warm-up and 5 timed calls, not the
-    measurement); the faster is kept.  Returns (order or None, report)."""
+    """For a matrix too large for the exact Keeper: smat.tuner.probe_locality
+    (the learned pick on the matrix as given against the pick on its
+    locality-ordered copy, a short probe each, not the measurement).
+    Returns (order or None, report)."""
+    from smat import tuner
     t0 = time.perf_counter()
-    order = locality_variant(ctx)
-    lo = learned_pick(ctx, log, order)
-
-    def probe(cfg, o):
-        _, bk, nt, kernel = cfg
-        r = Run(ctx, bk, nt, kernel, o).time(5, 2)
-        ms = r.event_ms
-        r.close()
-        return ms
-
-    as_given, ordered = probe(learned["config"], None), probe(lo["config"], order)
-    use = ordered < as_given
-    rep = {"config": lo["config"], "probe_event_ms": {"as_given": round(as_given, 4), "locality": round(ordered, 4)},
-           "used": use, "decide_ms": round((time.perf_counter() - t0) * 1e3, 1)}
-    log(f"locality order: pick {lo['config']}, probe {ordered:.4f} ms vs {as_given:.4f} ms as given -> "
-        f"{'used' if use else 'not used'}")
-    return (order if use else None), rep
+    use, _, order, rep = tuner.probe_locality(tuner.Model(), ctx.M, ctx.K, ctx.d_rp, ctx.d_ci, ctx.d_v, ctx.nnz,
+                                              ctx.N, ctx.B, ctx.C, pick=learned["config"])
+    rep["decide_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+    log(f"locality order: pick {rep['config']}, probe {rep['probe_event_ms']} -> {'used' if use else 'not used'}")
+    return order, rep
 
 
 def _gen_batch_matrix(seed):
@@ -411,8 +400,7 @@ def batched_workload(args, log):
         single_ms[1] += float(times[sb])
     log(f"features, model picks and exhaustive sweeps in {time.time() - t_sweep:.1f}s")
 
-    # matrices above the Keeper's limit: the learned pick on the locality-ordered
-    # copy against the pick as given, each probed over 5 calls; the faster is kept
+    # matrices above the Keeper's limit: smat.tuner.probe_locality per matrix
     t_loc = time.time()
     ordered = [False] * 64
     if not args.no_locality:
@@ -420,21 +408,9 @@ def batched_workload(args, log):
             if M <= tuner.REORDER_MAX_ROWS or picks[i][0] is not None:
                 continue
             d = [torch.from_numpy(x).cuda() for x in (rp, ci, v.view(np.int16))]
-            perm = smat.locality_order(M, K, d[0], d[1])
-            o = smat.permute_rows(d[0], d[1], d[2], perm)
-            _, bo = model.scores(tuner.features(tuner.stats(M, K, *o, int(ci.size)), N), tuner.available(M))
-            cand = tuner.CONFIGS[bo]
-            if cand[0] is not None:
-                continue
-            ms = []
-            for cfg, loc in ((picks[i], False), (cand, True)):
-                b = batch_plan(None, M, K, rp, ci, v, cfg, N, locality=loc)
-                ms.append(tuner.time_call(lambda: b["plan"].execute(Bs[i], Cs[i], values24=b["v24"],
-                                                                    index24=b["i24"], c_rowmap=b["rowmap"])))
-                b["plan"].close()
-                b["A"].close()
-            if ms[1] < ms[0]:
-                picks[i], ordered[i] = cand, True
+            use, cfg, _, _ = tuner.probe_locality(model, M, K, *d, int(ci.size), N, Bs[i], Cs[i], pick=picks[i])
+            if use:
+                picks[i], ordered[i] = cfg, True
     log(f"locality order kept for {sum(ordered)} of {sum(m[1][0] > tuner.REORDER_MAX_ROWS for m in mats)} "
         f"large matrices ({time.time() - t_loc:.1f}s)")
 

@@ -348,6 +348,46 @@ class Model:
         return CONFIGS[self.scores(feats, available(rows))[1]]
 
 
+def probe_locality(model: Model, M, K, d_rp, d_ci, d_v, nnz, N, B, C, pick=None, reps=5):
+    """Locality row order for a matrix above the exact Keeper's row limit
+    (device CSR, B and C tensors): the model's pick on the matrix as given and
+    on its locality-ordered copy (smat_locality_order + smat_permute_csr_rows),
+    each timed over `reps` calls after 2 warm-up calls; the order is kept when
+    it is faster.  Both picks skip the Keeper candidates (tau set).  Returns
+    (ordered, pick, order, report): order is (rowptr, colidx, values, rowmap)
+    of the ordered copy when kept, else None."""
+    from . import Bcsr, SpmmPlan, SMAT_KERNEL_DENSE, SMAT_KERNEL_MIXED24, SMAT_TAG_EXACT24, SMAT_TAG_REFERENCE, \
+        SMAT_24_SMFMAC, locality_order, permute_rows
+    import torch
+    avail = available(M) * (ENC[:, -1] == 0)
+    if pick is None:
+        pick = CONFIGS[model.scores(features(stats(M, K, d_rp, d_ci, d_v, nnz), N), avail)[1]]
+    perm = locality_order(M, K, d_rp, d_ci)
+    o_rp, o_ci, o_v = permute_rows(d_rp, d_ci, d_v, perm)
+    rowmap = perm.to(torch.int32)
+    o_pick = CONFIGS[model.scores(features(stats(M, K, o_rp, o_ci, o_v, nnz), N), avail)[1]]
+
+    def probe(cfg, rp, ci, v, rm):
+        _, bk, nt, kernel = cfg
+        mixed = kernel == "mixed24"
+        A = Bcsr(M, K, rp, ci, v, bk=bk, tag_rule=SMAT_TAG_EXACT24 if mixed else SMAT_TAG_REFERENCE)
+        v24 = i24 = None
+        if mixed:
+            v24, i24 = A.compress_24(SMAT_24_SMFMAC)
+        plan = SpmmPlan(A, N, n_tile=nt, kernel=SMAT_KERNEL_MIXED24 if mixed else SMAT_KERNEL_DENSE)
+        ms = time_call(lambda: plan.execute(B, C, values24=v24, index24=i24, c_rowmap=rm), reps=reps)
+        plan.close()
+        A.close()
+        return ms
+
+    as_given = probe(pick, d_rp, d_ci, d_v, None)
+    ordered_ms = probe(o_pick, o_rp, o_ci, o_v, rowmap)
+    use = ordered_ms < as_given
+    report = {"config": o_pick, "probe_event_ms": {"as_given": round(as_given, 4), "locality": round(ordered_ms, 4)},
+              "used": bool(use)}
+    return use, (o_pick if use else pick), ((o_rp, o_ci, o_v, rowmap) if use else None), report
+
+
 def evaluate(model: Model, feats, times, rows):
     """Per held-out matrix: time of the model's pick / sweep-best time, and the
     speedups of the pick over the plain configuration and over the single

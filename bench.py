@@ -408,7 +408,9 @@ def batched_workload(args, log):
             if M <= tuner.REORDER_MAX_ROWS or picks[i][0] is not None:
                 continue
             d = [torch.from_numpy(x).cuda() for x in (rp, ci, v.view(np.int16))]
-            use, cfg, _, _ = tuner.probe_locality(model, M, K, *d, int(ci.size), N, Bs[i], Cs[i], pick=picks[i])
+            # no margin here: over 64 matrices a probe's noise averages out, its gains add up
+            use, cfg, _, _ = tuner.probe_locality(model, M, K, *d, int(ci.size), N, Bs[i], Cs[i], pick=picks[i],
+                                                  margin=0.0)
             if use:
                 picks[i], ordered[i] = cfg, True
     log(f"locality order kept for {sum(ordered)} of {sum(m[1][0] > tuner.REORDER_MAX_ROWS for m in mats)} "

@@ -348,12 +348,13 @@ class Model:
         return CONFIGS[self.scores(feats, available(rows))[1]]
 
 
-def probe_locality(model: Model, M, K, d_rp, d_ci, d_v, nnz, N, B, C, pick=None, reps=5):
+def probe_locality(model: Model, M, K, d_rp, d_ci, d_v, nnz, N, B, C, pick=None, reps=10, margin=0.03):
     """Locality row order for a matrix above the exact Keeper's row limit
     (device CSR, B and C tensors): the model's pick on the matrix as given and
     on its locality-ordered copy (smat_locality_order + smat_permute_csr_rows),
     each timed over `reps` calls after 2 warm-up calls; the order is kept when
-    it is faster.  Both picks skip the Keeper candidates (tau set).  Returns
+    it is faster by more than `margin` (a probe of a 0.1 ms call moves by a
+    few per cent between runs).  Both picks skip the Keeper candidates (tau set).  Returns
     (ordered, pick, order, report): order is (rowptr, colidx, values, rowmap)
     of the ordered copy when kept, else None."""
     from . import Bcsr, SpmmPlan, SMAT_KERNEL_DENSE, SMAT_KERNEL_MIXED24, SMAT_TAG_EXACT24, SMAT_TAG_REFERENCE, \
@@ -382,7 +383,7 @@ def probe_locality(model: Model, M, K, d_rp, d_ci, d_v, nnz, N, B, C, pick=None,
 
     as_given = probe(pick, d_rp, d_ci, d_v, None)
     ordered_ms = probe(o_pick, o_rp, o_ci, o_v, rowmap)
-    use = ordered_ms < as_given
+    use = ordered_ms < as_given * (1.0 - margin)
     report = {"config": o_pick, "probe_event_ms": {"as_given": round(as_given, 4), "locality": round(ordered_ms, 4)},
               "used": bool(use)}
     return use, (o_pick if use else pick), ((o_rp, o_ci, o_v, rowmap) if use else None), report

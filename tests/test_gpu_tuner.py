@@ -107,6 +107,41 @@ def test_stats_edge_cases_match_oracle(case):
         assert 0 < exp["blocks24"] < exp["nnzb16"]  # both kinds of 16 x 16 block occur
 
 
+@pytest.mark.parametrize("margin", [0.0, 0.5])
+def test_probe_locality_contract(margin):
+    """smat.tuner.probe_locality on a matrix above the Keeper's limit: the
+    report names both picks and probe times; the returned pick (on the ordered
+    copy when kept) gives the FP64 product through the row map; a 50 % margin
+    never keeps the order."""
+    _need_gpu()
+    import smat
+    from smat import tuner
+    M, K, rp, ci, v = gen.rmat(scale=15, seed=21)
+    assert M > tuner.REORDER_MAX_ROWS
+    N = 64
+    d = [_dev(x) for x in (rp, ci, v.view(np.int16))]
+    B = gen.dense_b(K, N, seed=5)
+    Bd = _dev(B.view(np.int16))
+    C = torch.empty((M, N), dtype=torch.float32, device="cuda")
+    use, pick, order, rep = tuner.probe_locality(tuner.Model(), M, K, *d, int(ci.size), N, Bd, C, margin=margin)
+    assert set(rep) >= {"config", "probe_event_ms", "used"} and rep["used"] == use
+    assert pick[0] is None  # no Keeper candidate above its row limit
+    if margin >= 0.5:
+        assert not use and order is None
+    rp2, ci2, v2, rowmap = order if use else (d[0], d[1], d[2], None)
+    _, bk, nt, kernel = pick
+    A = smat.Bcsr(M, K, rp2, ci2, v2, bk=bk)
+    plan = smat.SpmmPlan(A, N, n_tile=nt)
+    C.fill_(float("nan"))
+    plan.execute(Bd, C, c_rowmap=rowmap)
+    torch.cuda.synchronize()
+    R = of.spmm_csr_f64(M, rp, ci, v, B)
+    Cn = C.cpu().numpy().astype(np.float64)
+    assert np.linalg.norm(Cn - R) / np.linalg.norm(R) < 1e-5
+    plan.close()
+    A.close()
+
+
 def test_features_formula():
     _need_gpu()
     from smat import tuner

@@ -131,7 +131,7 @@ def cpu_baseline(M, rp, ci, vals_u16, N, target_s):
 
     r = min(M, 8192)
     t, nnz = run(r)
-    while t < target_s / 4 and r < M:  # grow the sample to about target_s
+    while t < target_s / 2 and r < M:  # grow the sample (doubling r: ~4x the time) to ~target_s
         r = min(M, r * 2)
         t, nnz = run(r)
     gflops = 2.0 * nnz * N / t / 1e9

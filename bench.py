@@ -565,7 +565,8 @@ def main():
             "metric": "SpMM useful GFLOP/s (2*nnz*N / t), learned-tuned batch vs plain FP16 BCSR",
             "value": tuned["useful_gflops"], "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": tuned["wall_ms"], "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "f16 in, f32 accumulate/out", "data": "synthetic (seeded generators)",
+            "vs_baseline": None, "dtype": "f16", "accumulate": "f32", "c_dtype": "f32",
+            "data": "synthetic (seeded generators)",
             "config": {"workload": f"64 seeded matrices (n 2k-200k log-uniform, nnz 1e4-5e6), N={args.n}, "
                                    "one hipGraph-captured batch launch", "N": args.n,
                        "nnz_total": sum(r["nnz"])},
@@ -683,7 +684,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": f"{args.dtype} in, f32 accumulate/out",
+        "dtype": args.dtype,
+        "accumulate": "f32",
+        "c_dtype": "f32",
         "data": "synthetic (seeded generator; B U[-1,1] seed 7)",
         "config": {"workload": f"{desc}, B {K}x{ctx.N} row-major, FP32 C", "rows": M, "nnz": ctx.nnz,
                    "N": ctx.N, "tuned": {k: tuned[k] for k in ("bk", "n_tile", "kernel", "nnzb", "order")},

@@ -371,7 +371,9 @@ def batched_workload(args, log):
     from smat import gen, tuner
     N = args.n
     t0 = time.time()
-    with mp.get_context("fork").Pool(12) as pool:
+    # spawned workers (numpy generators only): a process that has touched the GPU, or
+    # runs under a profiler that initialised it, never forks
+    with mp.get_context("spawn").Pool(12) as pool:
         mats = pool.map(_gen_batch_matrix, range(64))
     log(f"generated 64 matrices in {time.time() - t0:.1f}s, nnz total {sum(m[1][3].size for m in mats)}")
     model = tuner.Model()

@@ -25,9 +25,15 @@
 //   or_compress24_k16          round trip on exact-2:4 blocks, hand-built bytes
 //   or_spmm_*_f64              dense brute-force GEMM, all-ones closed form,
 //                              identity / permutation / linearity properties
-//   or_spmm_refsim_fp16acc     parity unpinned against NVIDIA tensor cores (no
-//                              reference test holds their outputs); pinned only
-//                              against FP64 within FP16-accumulation error
+//   or_spmm_refsim_fp16acc     hand-derived FP16-accumulation cases (one rounding
+//                              per 16-wide k-step, blocks in BCSR order, a lost
+//                              term under cancellation) that an FP32 accumulator,
+//                              per-product rounding or reversed block order
+//                              each fail; within FP16-accumulation error of
+//                              FP64.  How a tensor core sums the 16 products
+//                              inside one k-step is not published and no
+//                              reference test holds its outputs: that part
+//                              stays unpinned
 //   or_csr_multiply_f32        dense brute force (FP32)
 //   or_e4m3_encode/decode      all 254 finite codes round-trip; every midpoint
 //                              between neighbouring codes ties to the even code;

@@ -431,11 +431,13 @@ def collect(first, count, out, N=128, workers=8, log=print, n_list=None, large=F
     import multiprocessing as mp
 
     import torch
-    torch.zeros(1).cuda()
     rec = {"seed": [], "kind": [], "rows": [], "nnz": [], "N": [], "feats": [], "times": [], "stats": []}
     t0 = time.time()
-    ctx = mp.get_context("fork")
+    # the workers only run numpy generators; they are spawned, so no process
+    # that has touched the GPU forks
+    ctx = mp.get_context("spawn")
     with ctx.Pool(workers) as pool:
+        torch.zeros(1).cuda()
         for k, (seed, kind, M, K, rp, ci, v) in enumerate(pool.imap(_gen_worker_large if large else _gen_worker,
                                                                                    range(first, first + count))):
             n_cols = n_list[seed % len(n_list)] if n_list else N

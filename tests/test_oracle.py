@@ -281,6 +281,76 @@ def test_refsim_fp16acc_close_to_f64():
     assert np.max(np.abs(C - R)) / np.max(np.abs(R)) < 1e-2
 
 
+def _refsim_case(bk):
+    """One 16-row block row, three blocks of 16 x bk (columns 0, bk, 2 bk), B
+    all ones, N = 2.  Rows hold closed-form cases of FP16 accumulation:
+      row 0: 2048, then +1, then +1 in three k-steps: each 2049 ties to the
+             even 2048, so C = 2048 (an FP32 accumulator ends at 2050);
+      row 1: 2048, 1, 1 inside one k-step: summed before the one rounding,
+             2050 is exact in FP16, so C = 2050 (rounding after every
+             product gives 2048);
+      row 2: 1, 1, then 2048 in block order: 1, 2, 2050, so C = 2050 (the
+             blocks visited in reverse give 2048);
+      row 3: 2048, +1, -2048: the 1 is lost, C = 0 (FP32 accumulation: 1).
+    Returns (rowptr, colidx, values, B, K, expected C[:4, 0])."""
+    K = 3 * bk
+    A = np.zeros((16, K), np.float32)
+    ks = min(bk, 16)  # the second k-step's first column (bk = 32 has two per block)
+    A[0, 0], A[0, bk], A[0, 2 * bk] = 2048, 1, 1
+    A[1, 0], A[1, 1], A[1, 2] = 2048, 1, 1
+    A[2, 0], A[2, bk], A[2, 2 * bk] = 1, 1, 2048
+    A[3, 0], A[3, bk], A[3, 2 * bk] = 2048, 1, -2048
+    if bk == 32:  # k-steps inside one 16 x 32 block: the same rounding between them
+        A[4, 0], A[4, ks], A[4, 2 * bk] = 2048, 1, 1
+    rowptr = np.array([0, 3], np.int32)
+    colidx = np.array([0, bk, 2 * bk], np.int32)
+    vals = np.concatenate([A[:, b * bk:(b + 1) * bk].reshape(-1) for b in range(3)]).astype(np.float16)
+    B = np.full((K, 2), 0x3C00, np.uint16)
+    exp = [2048.0, 2050.0, 2050.0, 0.0] + ([2048.0] if bk == 32 else [])
+    return rowptr, colidx, vals.view(np.uint16), B, K, np.array(exp)
+
+
+def _refsim_mutant(kind, bk, rowptr, colidx, vals, B):
+    """Plausible mistakes in the emulation (tiny cases only): an FP32
+    accumulator, rounding after every product, blocks in reverse order."""
+    v = vals.view(np.float16).astype(np.float32).reshape(-1, 16, bk)
+    Bf = B.view(np.float16).astype(np.float32)
+    out = np.zeros(16, np.float64)
+    blocks = list(range(rowptr[0], rowptr[1]))
+    if kind == "reverse":
+        blocks = blocks[::-1]
+    for r in range(16):
+        acc = np.float32(0) if kind == "fp32acc" else np.float16(0)
+        for p in blocks:
+            for k0 in range(0, bk, 16):
+                if kind == "per_product":
+                    for c in range(k0, min(k0 + 16, bk)):
+                        acc = np.float16(np.float32(acc) + v[p, r, c] * Bf[colidx[p] + c, 0])
+                    continue
+                s = np.float32(0)
+                for c in range(k0, min(k0 + 16, bk)):
+                    s = np.float32(s + v[p, r, c] * Bf[colidx[p] + c, 0])
+                acc = np.float32(acc + s) if kind == "fp32acc" else np.float16(np.float32(acc) + s)
+        out[r] = float(np.float16(acc))
+    return out
+
+
+@pytest.mark.parametrize("bk", [16, 32])
+def test_refsim_fp16acc_closed_forms(bk):
+    """or_spmm_refsim_fp16acc against hand-derived FP16-accumulation results
+    (HMMA16816 with an FP16 accumulator rounds once per 16-wide k-step,
+    ptx.h:20-25; blocks in BCSR order, mmaOBT.cu:87-147); each planted mistake
+    misses at least one of them."""
+    rowptr, colidx, vals, B, K, exp = _refsim_case(bk)
+    C = of.spmm_refsim_fp16acc(bk, rowptr, colidx, vals, B).astype(np.float64)
+    n = exp.size
+    assert np.array_equal(C[:n, 0], exp) and np.array_equal(C[:n, 1], exp)
+    assert np.all(C[n:] == 0)
+    for kind in ("fp32acc", "per_product", "reverse"):
+        got = _refsim_mutant(kind, bk, rowptr, colidx, vals, B)
+        assert not np.array_equal(got[:n], exp), kind
+
+
 def test_csr_multiply_golden(golden_dir):
     d = np.load(os.path.join(golden_dir, "csr_multiply_golden.npz"))
     n, bc = d["n"]

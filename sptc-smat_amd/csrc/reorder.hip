@@ -22,12 +22,16 @@
 // The host replays the order-dependent part on the compacted list: the
 // top-up candidate set with the reference's exact truncation
 // (`advance(best.end(), k)` followed by `erase(it, end())`, whose effect
-// depends on the shape of libstdc++'s red-black tree — the same operation on
-// the same library reproduces it), merge_rows on the pattern, the top-up and
-// the group numbering.  That part is pointer chasing through a set of
-// thousands of elements per seed, latency-bound on a GPU lane; measured set
-// sizes are in the design notes.
+// depends on the shape of libstdc++'s red-black tree — the same tree
+// algorithm of the same library reproduces it), merge_rows on the pattern,
+// the top-up and the group numbering.  That part is pointer chasing through a
+// set of thousands of elements per seed, latency-bound on a GPU lane.  Two
+// exact shortcuts keep it cheap: the truncation is computed from the shape
+// at the set's largest node in O(1) (truncate_from_end), and a candidate
+// above the current maximum whose insertion the truncation would undo
+// without any rebalancing is skipped (about 80 % of them on R-MAT graphs).
 #include <algorithm>
+#include <iterator>
 #include <numeric>
 #include <set>
 #include <utility>
@@ -365,6 +369,43 @@ struct PoolAlloc {
   bool operator!=(const PoolAlloc<U>& o) const { return pool != o.pool; }
 };
 
+// The Keeper's top-up set is a libstdc++ std::set, and its truncation
+// (blocking.cpp:505-507: it = end(); advance(it, keep); erase(it, end()))
+// walks forward from end(), which depends on the red-black tree's shape.
+// These helpers read that shape through the iterator's node (libstdc++'s
+// _Rb_tree_node_base: _M_color, _M_left), the library the reference is built
+// with and whose tree the replay reproduces.
+template <class It>
+inline bool node_is_red(It it) {
+  return it._M_node->_M_color == std::_S_red;
+}
+
+// Same effect as the reference's truncation, in O(1) instead of keep steps
+// (half of them climbs from the largest node to the root).  ++end() is the
+// leftmost node of the rightmost node R's subtree: R itself, or R's left
+// child L, which is then a red leaf (R has no right child).  The walk from
+// end() therefore cycles end, R (period 2) or end, L, R (period 3), so
+// advance(end(), keep) stops at R for odd keep or end() for even keep without
+// L, and at L / R / end() for keep % 3 == 1 / 2 / 0 with L.  erase(it, end())
+// removes the elements from there on, smaller first (when `it` is begin()
+// the range erase clears the set instead, with the same, empty, result).
+template <class Set>
+inline void truncate_from_end(Set& best, size_t keep) {
+  auto R = std::prev(best.end());
+  if (R._M_node->_M_left != nullptr) {
+    const size_t m = keep % 3;
+    if (m == 1) {
+      auto L = std::prev(R);
+      best.erase(L);
+      best.erase(R);
+    } else if (m == 2) {
+      best.erase(R);
+    }
+  } else if (keep & 1) {
+    best.erase(R);
+  }
+}
+
 // ---- VBR blocking statistics (BlockingEngine::CollectBlockingInfo) --------
 // A block row is the set of rows sharing one grouping value g; its height
 // h[g] comes from a histogram over the values (which lie in [0, 2 rows) for
@@ -610,17 +651,20 @@ smat_status smat_reorder_grouping(int64_t rows, const int32_t* rowptr, const int
       for (int64_t q = 0; q < ncmp && oj[q] < jm; q++) {
         comparisons++;
         const std::pair<float, int64_t> key(od[q], (int64_t)oj[q]);
-        // A key above the current maximum lands as the right child of the
-        // rightmost node whether or not the end() hint is given, so the hint
-        // only skips the descent: the tree, and the truncation below, stay
-        // exactly those of a plain insert.
-        if (!best.empty() && *best.rbegin() < key) best.insert(best.end(), key);
-        else best.insert(key);
-        if (best.size() > keep) {
-          auto it = best.end();  // blocking.cpp:505-507
-          std::advance(it, keep);
-          best.erase(it, best.end());
+        if (!best.empty() && *best.rbegin() < key) {
+          // A key above the current maximum becomes the red right child of
+          // the rightmost node R.  When R is black no rebalancing follows;
+          // with keep odd and the set over keep the truncation below then
+          // erases exactly that new leaf (it has no left child), again
+          // without rebalancing: the tree ends as it started, so the pair is
+          // skipped.  Otherwise the key is inserted with the end() hint,
+          // which only skips the descent (the node lands in the same place).
+          if ((keep & 1) && best.size() + 1 > keep && !node_is_red(std::prev(best.end()))) continue;
+          best.insert(best.end(), key);
+        } else {
+          best.insert(key);
         }
+        if (best.size() > keep) truncate_from_end(best, keep);  // blocking.cpp:505-507
       }
       j0 = jend;
       b = 1 - b;

@@ -11,7 +11,8 @@ runs them.
 Per tau the file keeps the counters [comparison_counter, merge_counter], the
 blocking info of the fixed-size grouping [VBR_nzcount, VBR_nzblocks_count,
 VBR_longest_row, round(1e6 * VBR_average_height)], the Keeper grouping
-(int32) and the fixed-size permutation (int32).  The matrix itself is
+(int32) and the SHA-256 of the fixed-size permutation (int64 bytes; the
+permutation itself is 262144 incompressible indices).  The matrix itself is
 re-generated from its seed by the tests.  One reference run takes about 13
 minutes on one core, so the four taus run as four processes.
 
@@ -19,6 +20,7 @@ Run:  python tests/golden/make_golden_rmat18.py   (this container only)
 """
 from __future__ import annotations
 
+import hashlib
 import multiprocessing as mp
 import os
 import sys
@@ -51,7 +53,8 @@ def one(tau):
     perm = of.get_permutation(fixed, use_ref=True)
     info, avg = of.collect_blocking_info(rp, ci, fixed, K, COL_BLOCK, use_ref=True)
     print(f"tau {tau}: {time.time() - t0:.0f} s, counters {cnt.tolist()}, info {info.tolist()}", flush=True)
-    return tau, cnt, np.append(info, np.int64(round(avg * 1e6))), g.astype(np.int32), perm.astype(np.int32), t
+    digest = np.frombuffer(hashlib.sha256(perm.astype(np.int64).tobytes()).digest(), np.uint8)
+    return tau, cnt, np.append(info, np.int64(round(avg * 1e6))), g.astype(np.int32), digest, t
 
 
 def main():
@@ -65,7 +68,7 @@ def main():
         arrays[key + "__counters"] = cnt
         arrays[key + "__info"] = info
         arrays[key + "__grouping"] = g
-        arrays[key + "__perm"] = perm
+        arrays[key + "__perm_sha256"] = perm
         arrays[key + "__ref_seconds"] = np.array([t], np.float64)
     np.savez_compressed(os.path.join(HERE, "keeper_rmat18_golden.npz"), **arrays)
     print("written", os.path.join(HERE, "keeper_rmat18_golden.npz"))

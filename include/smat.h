@@ -437,6 +437,30 @@ smat_status smat_batch_info(smat_batch batch, int64_t* tasks, int64_t* split_row
                             int32_t* captured);
 smat_status smat_batch_destroy(smat_batch batch);
 
+/*
+ * SpMM that allocates nothing.  No counterpart in the reference (its
+ * launchers take caller-allocated arrays and keep no work decomposition); it
+ * is for callers that cannot allocate device memory: serving loops and
+ * harnesses that forbid hipMalloc.  One call does what smat_spmm_plan_create
+ * (A, N, opts) followed by smat_spmm_execute_ex(ops) and
+ * smat_spmm_plan_destroy do, with the same work decomposition and therefore
+ * the same C bits, but keeps every device array of the plan (the planner's
+ * counters and scan space, work items, split-row records, partial tiles of
+ * split block rows) in a static device scratch of SMAT_NOALLOC_SCRATCH_BYTES
+ * that each GPU holds from the library's first use on it until the process
+ * ends.  A call that needs more returns SMAT_ERR_INVALID before any SpMM
+ * kernel is launched.  The scratch keeps the last call's contents; nothing
+ * in it has to start at zero (every array is written before it is read).
+ * The call reads plan sizes back, so it synchronizes `stream` once, and then
+ * returns with the SpMM kernels enqueued on `stream`.  Calls on one GPU share
+ * its scratch: calls that overlap in time (different streams or threads
+ * without synchronization between them) race; calls in order on one stream
+ * do not.
+ */
+#define SMAT_NOALLOC_SCRATCH_BYTES (64ull << 20)
+smat_status smat_spmm_noalloc(const smat_bcsr_desc* A, int64_t N, const smat_spmm_opts* opts,
+                              const smat_spmm_operands* ops, smat_stream stream);
+
 /* Number of work items and split rows of a plan (for reports and tests). */
 smat_status smat_spmm_plan_info(smat_plan plan, int64_t* n_items, int64_t* n_split_rows,
                                 int64_t* scratch_bytes);

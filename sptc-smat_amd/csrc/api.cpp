@@ -20,9 +20,28 @@ smat_status set_error(smat_status st, const char* fmt, ...) {
   return st;
 }
 
+namespace {
+thread_local Arena* g_arena = nullptr;
+}  // namespace
+
+ArenaScope::ArenaScope(Arena* a) : prev_(g_arena) { g_arena = a; }
+ArenaScope::~ArenaScope() { g_arena = prev_; }
+
 smat_status DevBuf::alloc(size_t n) {
   release();
   if (n == 0) n = 256;  // keep a valid, aligned pointer for empty arrays
+  if (g_arena) {
+    Arena& a = *g_arena;
+    const size_t off = (a.used + 255) & ~(size_t)255;
+    if (off > a.cap || n > a.cap - off)
+      return set_error(SMAT_ERR_INVALID, "the call needs more than the %zu bytes of its static device scratch",
+                       a.cap);
+    p = a.base + off;
+    a.used = off + n;
+    bytes = n;
+    owned = false;
+    return SMAT_OK;
+  }
   hipError_t e = hipMalloc(&p, n);
   if (e != hipSuccess) {
     p = nullptr;
@@ -30,13 +49,15 @@ smat_status DevBuf::alloc(size_t n) {
     return set_error(SMAT_ERR_ALLOC, "hipMalloc(%zu) failed: %s", n, hipGetErrorString(e));
   }
   bytes = n;
+  owned = true;
   return SMAT_OK;
 }
 
 void DevBuf::release() {
-  if (p) (void)hipFree(p);
+  if (p && owned) (void)hipFree(p);
   p = nullptr;
   bytes = 0;
+  owned = false;
 }
 
 smat_status HostBuf::alloc(size_t n) {

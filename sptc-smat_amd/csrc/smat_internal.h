@@ -16,10 +16,33 @@ smat_status set_error(smat_status st, const char* fmt, ...);
 
 inline hipStream_t to_hip(smat_stream s) { return reinterpret_cast<hipStream_t>(s); }
 
-// Device buffer owned by library objects.
+// A device region that DevBuf::alloc carves from (bump allocation, 256-byte
+// aligned) instead of calling hipMalloc, while an ArenaScope on the calling
+// thread has it installed.  Used by the call that allocates nothing.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0;
+  size_t used = 0;
+};
+
+// Installs `a` for DevBuf::alloc on this thread for the scope's lifetime.
+class ArenaScope {
+ public:
+  explicit ArenaScope(Arena* a);
+  ~ArenaScope();
+  ArenaScope(const ArenaScope&) = delete;
+  ArenaScope& operator=(const ArenaScope&) = delete;
+
+ private:
+  Arena* prev_;
+};
+
+// Device buffer owned by library objects (or, under an ArenaScope, a view
+// of the arena that release() leaves alone).
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool owned = false;
   smat_status alloc(size_t n);
   void release();
   ~DevBuf() { release(); }

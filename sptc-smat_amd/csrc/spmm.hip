@@ -41,6 +41,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "smat_internal.h"
@@ -1354,14 +1355,12 @@ smat_status build_mixed(smat_plan_t* P, int32_t S, hipStream_t s) {
   return build_items(P, drp.as<int32_t>(), 2, srp.as<int32_t>(), 4, S, s);
 }
 
-}  // namespace
-
-extern "C" {
-
-smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat_spmm_opts* opts,
-                                  smat_stream stream, smat_plan* out) {
-  SMAT_REQUIRE(A && out, "smat_spmm_plan_create: null argument");
-  *out = nullptr;
+// Validates the options and builds the work decomposition of an SpMM of A
+// (N columns) into P.  Its device arrays come from DevBuf::alloc: owned
+// allocations, or the static scratch under an ArenaScope.
+smat_status plan_init(smat_plan_t* P, const smat_bcsr_desc* A, int64_t N, const smat_spmm_opts* opts,
+                      hipStream_t s) {
+  SMAT_REQUIRE(A, "smat_spmm_plan_create: null argument");
   SMAT_REQUIRE(A->bk == 1 || A->bk == 2 || A->bk == 4 || A->bk == 8 || A->bk == 16 || A->bk == 32,
                "smat_spmm_plan_create: bk must be 1, 2, 4, 8, 16 or 32");
   SMAT_REQUIRE(N > 0, "smat_spmm_plan_create: N must be positive");
@@ -1398,7 +1397,6 @@ smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat
                "smat_spmm_plan_create: unknown b_staging %d", o.b_staging);
   SMAT_REQUIRE(o.schedule >= SMAT_SCHED_AUTO && o.schedule <= SMAT_SCHED_TILE_AFFINITY,
                "smat_spmm_plan_create: unknown schedule %d", o.schedule);
-  hipStream_t s = smat::to_hip(stream);
   int nt = o.n_tile ? o.n_tile : (N >= 64 ? 64 : (N > 16 ? 32 : 16));
   if (o.in_dtype == SMAT_FP8_E4M3) {
     // FP8 tiles are at least 32 columns wide; by default 128 (a whole
@@ -1418,7 +1416,6 @@ smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat
                                                        : SMAT_SCHED_LONGEST_FIRST;
   }
 
-  smat_plan_t* P = new smat_plan_t();
   P->A = *A;
   P->N = N;
   P->opts = o;
@@ -1456,11 +1453,20 @@ smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat
   // an item spans at most 65536 blocks (the LDS kernels address an item's A
   // values with 32-bit offsets from its first block)
   if (S > 65536) S = 65536;
-  smat_status st;
-  if (P->mixed)
-    st = build_mixed(P, (S + 1) / 2, s);  // S in MFMA steps (a dense pair or a 2:4 quad each)
-  else
-    st = build_items(P, A->rowptr, step, nullptr, 1, (S + step - 1) / step, s);
+  if (P->mixed) return build_mixed(P, (S + 1) / 2, s);  // S in MFMA steps (a dense pair or a 2:4 quad each)
+  return build_items(P, A->rowptr, step, nullptr, 1, (S + step - 1) / step, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+smat_status smat_spmm_plan_create(const smat_bcsr_desc* A, int64_t N, const smat_spmm_opts* opts,
+                                  smat_stream stream, smat_plan* out) {
+  SMAT_REQUIRE(A && out, "smat_spmm_plan_create: null argument");
+  *out = nullptr;
+  smat_plan_t* P = new smat_plan_t();
+  const smat_status st = plan_init(P, A, N, opts, smat::to_hip(stream));
   if (st != SMAT_OK) {
     delete P;
     return st;
@@ -1579,6 +1585,59 @@ smat_status smat_spmm_plan_info(smat_plan P, int64_t* n_items, int64_t* n_split_
   if (n_split_rows) *n_split_rows = P->n_red;
   if (scratch_bytes) *scratch_bytes = (int64_t)P->scratch.bytes;
   return SMAT_OK;
+}
+
+}  // extern "C"
+
+// ---- SpMM without allocation ------------------------------------------------
+// Static device scratch of smat_spmm_noalloc: the planner's counters and scan
+// space, the work items, the split-row records and the partial tiles of
+// split block rows.  Each GPU holds its own copy for the life of the
+// process.  Host code never uses the array's name as an address (on the host
+// it is only a stand-in): it uses the per-device address hipGetSymbolAddress
+// returns (noalloc_base).
+static __device__ __align__(256) char scratch_noalloc[SMAT_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_base[kMaxDevices] = {};
+
+// Address of the current device's copy of scratch_noalloc (looked up once per device).
+smat_status noalloc_base(void** out) {
+  int dev = 0;
+  SMAT_HIP_TRY(hipGetDevice(&dev));
+  SMAT_REQUIRE(dev >= 0 && dev < kMaxDevices, "smat_spmm_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_base[dev]) {
+    void* p = nullptr;
+    SMAT_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch_noalloc)));
+    SMAT_REQUIRE(p != nullptr, "smat_spmm_noalloc: no device address for the static scratch");
+    g_noalloc_base[dev] = p;
+  }
+  *out = g_noalloc_base[dev];
+  return SMAT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+smat_status smat_spmm_noalloc(const smat_bcsr_desc* A, int64_t N, const smat_spmm_opts* opts,
+                              const smat_spmm_operands* ops, smat_stream stream) {
+  SMAT_REQUIRE(A && ops, "smat_spmm_noalloc: null argument");
+  void* base = nullptr;
+  SMAT_TRY(noalloc_base(&base));
+  const hipStream_t s = smat::to_hip(stream);
+  // every device array of the plan is carved from the scratch, from its start
+  smat::Arena arena{static_cast<char*>(base), (size_t)SMAT_NOALLOC_SCRATCH_BYTES, 0};
+  smat::ArenaScope scope(&arena);
+  smat_plan_t P;
+  SMAT_TRY(plan_init(&P, A, N, opts, s));
+  Operands op{};
+  SMAT_TRY(prepare_operands(&P, ops, &op));
+  return launch_call(&P, ops, op, s);
 }
 
 }  // extern "C"

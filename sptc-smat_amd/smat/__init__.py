@@ -140,6 +140,8 @@ SIGNATURES = {
     "smat_pack_a": (_i32, [ctypes.POINTER(BcsrDesc), _p, _i32, _p, _p, _p]),
     "smat_pack_b": (_i32, [_p, _i64, _i64, _i64, _i32, _p, _p, _p]),
     "smat_error_sums": (_i32, [_p, _i64, _p, _i64, _i64, _i64, _p, _p, _p]),
+    "smat_spmm_noalloc": (_i32, [ctypes.POINTER(BcsrDesc), _i64, ctypes.POINTER(SpmmOpts),
+                                 ctypes.POINTER(SpmmOperands), _p]),
     "smat_spmm_plan_info": (_i32, [_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "smat_reorder_grouping": (_i32, [_i64, _p, _p, _p, _p, ctypes.c_float, _i64, _i64, _p, _p, _p]),
     "smat_fixed_size_grouping": (_i32, [_i64, _p, _i64, _p]),
@@ -373,6 +375,24 @@ class SpmmPlan:
             self.close()
         except Exception:
             pass
+
+
+SMAT_NOALLOC_SCRATCH_BYTES = 64 << 20
+
+
+def spmm_noalloc(A: Bcsr | BcsrDesc, N: int, B, C, *, kernel: int = SMAT_KERNEL_AUTO, out_dtype: int = SMAT_F32,
+                 b_layout: int = SMAT_B_ROW_MAJOR, n_tile: int = 0, max_blocks_per_item: int = 0,
+                 b_staging: int = 0, in_dtype: int = SMAT_F16, schedule: int = 0, a_values=None, a_scale=None,
+                 b_scale=None, ldb: int | None = None, ldc: int | None = None, values24=None, index24=None,
+                 c_rowmap=None, stream=None):
+    """smat_spmm_noalloc: plan + execute in one call, with the plan's device
+    arrays in the library's static scratch (no device allocation)."""
+    desc = A.desc if isinstance(A, Bcsr) else A
+    o = SpmmOpts(kernel, out_dtype, b_layout, n_tile, max_blocks_per_item, b_staging, in_dtype, schedule)
+    ops = SpmmOperands(ptr(a_values), ptr(a_scale), ptr(B), ptr(b_scale), N if ldb is None else ldb, ptr(C),
+                       C.shape[-1] if ldc is None else ldc, ptr(values24), ptr(index24), ptr(c_rowmap))
+    check(lib().smat_spmm_noalloc(ctypes.byref(desc), N, ctypes.byref(o), ctypes.byref(ops), stream_ptr(stream)))
+    return C
 
 
 def reorder_grouping(rowptr, colidx, tau: float, col_block: int, max_rows: int, stream=None):

@@ -107,6 +107,22 @@ smat_status smat_read_mtx(const char* path, int32_t k, smat_csr_host* out);
  */
 smat_status smat_read_edgelist(const char* path, const char* delimiter, int32_t pattern_only,
                                smat_csr_host* out);
+/* Formats of CSR::read_from_edgelist (preprocess/include/definitions.h MatrixFormat). */
+#define SMAT_FMT_EL 0
+#define SMAT_FMT_MTX 1
+/*
+ * CSR::read_from_edgelist with every option (csr.cpp:166-290, matrices.h:
+ * 61-78): mat_fmt SMAT_FMT_MTX also skips the size line after the comments
+ * and swaps the two index fields (indices kept as in the file); symmetrize
+ * != 0 mirrors every entry when none lies below the diagonal (pattern-only
+ * input; the reference throws for weighted input, and indexes past its rows
+ * when a column has no row: both are SMAT_ERR_INVALID here).
+ */
+smat_status smat_read_edgelist_ex(const char* path, const char* delimiter, int32_t pattern_only,
+                                  int32_t mat_fmt, int32_t symmetrize, smat_csr_host* out);
+/* Same, on text in memory (the reference's CSR(std::ifstream&) constructor). */
+smat_status smat_parse_edgelist(const char* text, int64_t len, const char* delimiter, int32_t pattern_only,
+                                int32_t mat_fmt, int32_t symmetrize, smat_csr_host* out);
 void smat_csr_host_free(smat_csr_host* csr);
 
 /* ------------------------------------------------------------------------ */

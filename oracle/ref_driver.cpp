@@ -199,4 +199,34 @@ intT ref_read_edgelist(const char* path, int pattern_only, intT* rows, intT* col
   }
 }
 
+// CSR::read_from_edgelist (csr.cpp:166-290) with its format and symmetrize
+// options.  Returns nnz as ref_read_edgelist does, -1 when it throws.
+intT ref_read_edgelist_opts(const char* path, int pattern_only, int mat_fmt, int symmetrize, intT* rows, intT* cols,
+                            intT* rowptr, intT* colidx, float* vals, intT cap_rows, intT cap_nnz) {
+  try {
+    std::ifstream empty;
+    CSR m(empty, " ", pattern_only != 0, el);  // rows = 0: nothing to free before the re-read
+    std::ifstream in(path);
+    m.read_from_edgelist(in, " ", pattern_only != 0, mat_fmt ? mtx : el, symmetrize != 0);
+    *rows = m.rows;
+    *cols = m.cols;
+    intT nnz = 0;
+    for (intT i = 0; i < m.rows; i++) nnz += m.nzcount[i];
+    if (m.rows + 1 <= cap_rows && nnz <= cap_nnz) {
+      rowptr[0] = 0;
+      intT at = 0;
+      for (intT i = 0; i < m.rows; i++) {
+        for (intT q = 0; q < m.nzcount[i]; q++, at++) {
+          colidx[at] = m.ja[i][q];
+          vals[at] = pattern_only ? 1.f : m.ma[i][q];
+        }
+        rowptr[i + 1] = at;
+      }
+    }
+    return nnz;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
 }  // extern "C"

@@ -5,7 +5,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <fstream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -178,23 +180,32 @@ smat_status smat_read_mtx(const char* path, int32_t k, smat_csr_host* out) {
   return SMAT_OK;
 }
 
-// CSR::read_from_edgelist (preprocess/src/general/csr.cpp:166-290), edge-list
-// format: leading '#' / '%' lines skipped; each line "row<d>col[<d>value]"
-// parsed like stoi / stof (leading integer or float prefix of each field);
+}  // extern "C"
+
+namespace {
+
+// CSR::read_from_edgelist (preprocess/src/general/csr.cpp:166-290):
+// leading '#' / '%' lines skipped, then (mat_fmt SMAT_FMT_MTX) one more line,
+// the size line; each line "a<d>b[<d>value]" parsed like stoi / stof (leading
+// integer or float prefix of each field); in MTX mode a and b are swapped
+// (the row is the second field, as the reference does, indices unchanged);
 // rows must not descend; rows = last row + 1, cols = largest column + 1;
 // columns in file order.  A weighted line with two fields takes its column
 // field as the value too (the reference's parser reads the remainder again).
-smat_status smat_read_edgelist(const char* path, const char* delimiter, int32_t pattern_only, smat_csr_host* out) {
-  SMAT_REQUIRE(path && out, "smat_read_edgelist: bad argument");
-  std::memset(out, 0, sizeof(*out));
+// symmetrize mirrors every entry (i, j) into row j when no entry lies below
+// the diagonal (inserted at std::lower_bound of row j, as the reference),
+// pattern-only input only; an entry whose column has no row is refused (the
+// reference indexes past its row list there).
+smat_status read_edgelist_stream(std::istream& in, const char* delimiter, int32_t pattern_only, int32_t mat_fmt,
+                                 int32_t symmetrize, smat_csr_host* out) {
+  SMAT_REQUIRE(mat_fmt == SMAT_FMT_EL || mat_fmt == SMAT_FMT_MTX, "smat_read_edgelist: unknown format %d", mat_fmt);
   const std::string d = delimiter && *delimiter ? delimiter : " ";
-  std::ifstream in(path);
-  SMAT_REQUIRE(in.good(), "smat_read_edgelist: cannot open %s", path);
   while (in.peek() == '#' || in.peek() == '%') in.ignore(2048, '\n');
-  std::vector<int64_t> rowlen;
-  std::vector<int32_t> cols;
-  std::vector<float> vals;
-  int64_t max_col = 0, last = -1;
+  if (mat_fmt == SMAT_FMT_MTX) in.ignore(2048, '\n');
+  std::vector<std::vector<int32_t>> pos;
+  std::vector<std::vector<float>> val;
+  int64_t max_col = 0;
+  bool triangular = true;
   std::string line;
   auto to_int = [](const std::string& s, int64_t& v) {
     char* end = nullptr;
@@ -221,8 +232,13 @@ smat_status smat_read_edgelist(const char* path, const char* delimiter, int32_t 
     p = rest.find(d);
     const std::string b = rest.substr(0, p);
     int64_t r = 0, c = 0;
-    if (!to_int(a, r) || !to_int(b, c) || r < 0 || c < 0 || c >= INT32_MAX)
+    if (!to_int(a, r) || !to_int(b, c))
       return smat::set_error(SMAT_ERR_INVALID, "smat_read_edgelist: bad line %lld", (long long)lineno);
+    if (mat_fmt == SMAT_FMT_MTX) std::swap(r, c);
+    if (r < 0 || c < 0 || c >= INT32_MAX || r >= INT32_MAX)
+      return smat::set_error(SMAT_ERR_INVALID, "smat_read_edgelist: bad line %lld", (long long)lineno);
+    if (c < r) triangular = false;
+    if (c > max_col) max_col = c;
     float v = 1.f;
     if (!pattern_only) {
       const std::string third = p == std::string::npos ? rest : rest.substr(p + d.size());
@@ -230,32 +246,78 @@ smat_status smat_read_edgelist(const char* path, const char* delimiter, int32_t 
       if (!to_float(field, v))
         return smat::set_error(SMAT_ERR_INVALID, "smat_read_edgelist: bad value on line %lld", (long long)lineno);
     }
-    if (r < last)
+    if (r < (int64_t)pos.size() - 1)
       return smat::set_error(SMAT_ERR_INVALID, "smat_read_edgelist: rows must be in ascending order (line %lld)",
                              (long long)lineno);
-    while (last < r) {
-      rowlen.push_back(0);
-      last++;
+    while ((int64_t)pos.size() <= r) {
+      pos.emplace_back();
+      val.emplace_back();
     }
-    rowlen[(size_t)r]++;
-    cols.push_back((int32_t)c);
-    vals.push_back(v);
-    if (c > max_col) max_col = c;
+    pos[(size_t)r].push_back((int32_t)c);
+    val[(size_t)r].push_back(v);
   }
-  const int64_t rows = (int64_t)rowlen.size(), nnz = (int64_t)cols.size();
+  if (symmetrize && triangular) {
+    for (size_t i = 0; i < pos.size(); i++) {
+      for (size_t nz = 0; nz < pos[i].size(); nz++) {
+        const int32_t j = pos[i][nz];
+        if ((size_t)j >= pos.size())
+          return smat::set_error(SMAT_ERR_INVALID,
+                                 "smat_read_edgelist: symmetrize: column %d has no row (the matrix has %zu rows)", j,
+                                 pos.size());
+        auto& row = pos[(size_t)j];
+        auto it = std::lower_bound(row.begin(), row.end(), (int32_t)i);
+        if (it == row.end() || *it != (int32_t)i) {
+          SMAT_REQUIRE(pattern_only, "smat_read_edgelist: symmetrize is only implemented for pattern-only input");
+          val[(size_t)j].insert(val[(size_t)j].begin() + (it - row.begin()), 1.f);
+          row.insert(it, (int32_t)i);
+        }
+      }
+    }
+  }
+  const int64_t rows = (int64_t)pos.size();
+  int64_t nnz = 0;
+  for (const auto& r : pos) nnz += (int64_t)r.size();
   SMAT_REQUIRE(nnz < INT32_MAX && rows < INT32_MAX, "smat_read_edgelist: too large");
   SMAT_TRY(alloc_out(out, rows, nnz));
   out->rows = rows;
   out->cols = max_col + 1;  // the reference starts its maximum at 0
   out->nnz = nnz;
   out->rowptr[0] = 0;
-  for (int64_t r = 0; r < rows; r++) out->rowptr[r + 1] = out->rowptr[r] + (int32_t)rowlen[(size_t)r];
-  for (int64_t q = 0; q < nnz; q++) {
-    out->colidx[q] = cols[(size_t)q];
-    out->values_f32[q] = vals[(size_t)q];
-    out->values_f16[q] = double_to_half((double)vals[(size_t)q]);
+  int64_t q = 0;
+  for (int64_t r = 0; r < rows; r++) {
+    for (size_t k = 0; k < pos[(size_t)r].size(); k++, q++) {
+      out->colidx[q] = pos[(size_t)r][k];
+      out->values_f32[q] = val[(size_t)r][k];
+      out->values_f16[q] = double_to_half((double)val[(size_t)r][k]);
+    }
+    out->rowptr[r + 1] = (int32_t)q;
   }
   return SMAT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+smat_status smat_read_edgelist_ex(const char* path, const char* delimiter, int32_t pattern_only, int32_t mat_fmt,
+                                  int32_t symmetrize, smat_csr_host* out) {
+  SMAT_REQUIRE(path && out, "smat_read_edgelist: bad argument");
+  std::memset(out, 0, sizeof(*out));
+  std::ifstream in(path);
+  SMAT_REQUIRE(in.good(), "smat_read_edgelist: cannot open %s", path);
+  return read_edgelist_stream(in, delimiter, pattern_only, mat_fmt, symmetrize, out);
+}
+
+smat_status smat_read_edgelist(const char* path, const char* delimiter, int32_t pattern_only, smat_csr_host* out) {
+  return smat_read_edgelist_ex(path, delimiter, pattern_only, SMAT_FMT_EL, 0, out);
+}
+
+smat_status smat_parse_edgelist(const char* text, int64_t len, const char* delimiter, int32_t pattern_only,
+                                int32_t mat_fmt, int32_t symmetrize, smat_csr_host* out) {
+  SMAT_REQUIRE(out && (text || len == 0) && len >= 0, "smat_parse_edgelist: bad argument");
+  std::memset(out, 0, sizeof(*out));
+  std::istringstream in(std::string(text ? text : "", (size_t)len));
+  return read_edgelist_stream(in, delimiter, pattern_only, mat_fmt, symmetrize, out);
 }
 
 }  // extern "C"

@@ -109,6 +109,8 @@ SIGNATURES = {
     "smat_permute_dense_rows": (_i32, [_i64, _i64, _i32, _p, _i64, _p, _p, _i64, _p]),
     "smat_read_mtx": (_i32, [ctypes.c_char_p, _i32, ctypes.POINTER(CsrHost)]),
     "smat_read_edgelist": (_i32, [ctypes.c_char_p, ctypes.c_char_p, _i32, ctypes.POINTER(CsrHost)]),
+    "smat_read_edgelist_ex": (_i32, [ctypes.c_char_p, ctypes.c_char_p, _i32, _i32, _i32, ctypes.POINTER(CsrHost)]),
+    "smat_parse_edgelist": (_i32, [ctypes.c_char_p, _i64, ctypes.c_char_p, _i32, _i32, _i32, ctypes.POINTER(CsrHost)]),
     "smat_csr_host_free": (None, [ctypes.POINTER(CsrHost)]),
     "smat_batch_create": (_i32, [_i32, _p, _p, _p, ctypes.POINTER(_p)]),
     "smat_batch_create_ex": (_i32, [_i32, _p, _p, _i32, _p, ctypes.POINTER(_p)]),
@@ -627,8 +629,24 @@ def read_mtx(path: str, k: int = 16):
     return _csr_from_host(c)
 
 
-def read_edgelist(path: str, delimiter: str = " ", pattern_only: bool = False):
-    """Edge-list file -> dict of host CSR arrays (smat_read_edgelist)."""
+SMAT_FMT_EL, SMAT_FMT_MTX = 0, 1
+
+
+def read_edgelist(path: str, delimiter: str = " ", pattern_only: bool = False, mat_fmt: int = SMAT_FMT_EL,
+                  symmetrize: bool = False):
+    """Edge-list (or, with mat_fmt SMAT_FMT_MTX, the reference's MTX-as-edge-list)
+    file -> dict of host CSR arrays (smat_read_edgelist_ex)."""
     c = CsrHost()
-    check(lib().smat_read_edgelist(path.encode(), delimiter.encode(), int(pattern_only), ctypes.byref(c)))
+    check(lib().smat_read_edgelist_ex(path.encode(), delimiter.encode(), int(pattern_only), mat_fmt, int(symmetrize),
+                                      ctypes.byref(c)))
+    return _csr_from_host(c)
+
+
+def parse_edgelist(text: str, delimiter: str = " ", pattern_only: bool = False, mat_fmt: int = SMAT_FMT_EL,
+                   symmetrize: bool = False):
+    """The same reader on text in memory (smat_parse_edgelist)."""
+    c = CsrHost()
+    b = text.encode()
+    check(lib().smat_parse_edgelist(b, len(b), delimiter.encode(), int(pattern_only), mat_fmt, int(symmetrize),
+                                    ctypes.byref(c)))
     return _csr_from_host(c)

@@ -138,3 +138,98 @@ def test_edgelist_kat_matrix():
     smat = _smat()
     c = smat.read_edgelist(os.path.join(GOLDEN, "TEST_matrix_weighted.el"))
     assert (c["rows"], c["cols"], c["nnz"]) == (9, 9, 13)
+
+
+def _ref_read_opts(path, pattern_only, mat_fmt, symmetrize):
+    rows, cols = ctypes.c_long(), ctypes.c_long()
+    f = of.ref().ref_read_edgelist_opts
+    f.restype = ctypes.c_long
+    f.argtypes = [ctypes.c_char_p] + [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_long)] * 2 + \
+        [ctypes.c_void_p] * 3 + [ctypes.c_long] * 2
+    cap = 1_000_000
+    rp = np.zeros(cap, np.int64)
+    ci = np.zeros(cap, np.int64)
+    v = np.zeros(cap, np.float32)
+    nnz = f(path.encode(), int(pattern_only), int(mat_fmt), int(symmetrize), ctypes.byref(rows), ctypes.byref(cols),
+            rp.ctypes.data, ci.ctypes.data, v.ctypes.data, cap, cap)
+    if nnz < 0:
+        return None
+    return rows.value, cols.value, rp[:rows.value + 1], ci[:nnz], v[:nnz]
+
+
+def _mtx_text(rng, n, m, nnz, weighted):
+    """A Matrix Market coordinate file as SuiteSparse writes them: 1-based, sorted by column."""
+    cells = rng.choice(n * m, size=nnz, replace=False)
+    r, c = cells // m + 1, cells % m + 1
+    order = np.lexsort((r, c))
+    lines = ["%%MatrixMarket matrix coordinate " + ("real" if weighted else "pattern") + " general",
+             "% written for a test", f"{n} {m} {nnz}"]
+    for i in order:
+        lines.append(f"{r[i]} {c[i]} {rng.uniform(-4, 4):.6g}" if weighted else f"{r[i]} {c[i]}")
+    return "\n".join(lines) + "\n"
+
+
+def _upper_el(rng, n, nnz):
+    """Pattern edge list with every entry on or above the diagonal, the last row holding (n-1, n-1)."""
+    cells = set()
+    while len(cells) < nnz:
+        i, j = sorted(rng.integers(0, n, 2).tolist())
+        cells.add((i, j))
+    cells.add((n - 1, n - 1))
+    return "".join(f"{i} {j}\n" for i, j in sorted(cells))
+
+
+CASES_OPTS = [("mtx_weighted", 1, 0, False), ("mtx_pattern", 1, 0, True), ("el_sym", 0, 1, True),
+              ("mtx_sym", 1, 1, True), ("el_sym_weighted", 0, 1, False), ("el_nonsym_sym", 0, 1, True)]
+
+
+@pytest.mark.parametrize("name,mat_fmt,symmetrize,pattern_only", CASES_OPTS)
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_edgelist_options_match_compiled_reference(tmp_path, name, mat_fmt, symmetrize, pattern_only, seed):
+    """read_from_edgelist's MTX mode (size line skipped, index fields swapped)
+    and symmetrize (csr.cpp:184, 199-202, 241-260) against the compiled
+    reference on generated files; where the reference rejects a file
+    (descending rows, symmetrize of weighted input), so does the reader."""
+    if not of.ref_available():
+        pytest.skip("reference build (oracle/_ref) not available")
+    smat = _smat()
+    rng = np.random.default_rng(seed)
+    if name.startswith("mtx_sym"):
+        # lower triangle in MTX: after the field swap every entry is on or above the diagonal
+        cells = sorted({tuple(sorted(rng.integers(1, 40, 2).tolist(), reverse=True)) for _ in range(120)} | {(40, 40)},
+                       key=lambda t: (t[1], t[0]))
+        text = "%%MatrixMarket matrix coordinate pattern symmetric\n40 40 %d\n" % len(cells) + \
+            "".join(f"{i} {j}\n" for i, j in cells)
+    elif name.startswith("mtx"):
+        text = _mtx_text(rng, 50, 37, 200, weighted=not pattern_only)
+    elif name == "el_nonsym_sym":
+        text = "0 1\n0 3\n1 0\n2 2\n3 1\n"
+    elif name == "el_sym_weighted":
+        text = "".join(f"{line} 1.5\n" for line in _upper_el(rng, 30, 60).splitlines())
+    else:
+        text = _upper_el(rng, 30, 60)
+    p = _write(tmp_path, "f.txt", text)
+    ref = _ref_read_opts(p, pattern_only, mat_fmt, symmetrize)
+    if ref is None:
+        with pytest.raises(smat.SmatError):
+            smat.read_edgelist(p, pattern_only=pattern_only, mat_fmt=mat_fmt, symmetrize=bool(symmetrize))
+        return
+    rows, cols, rp, ci, v = ref
+    for c in (smat.read_edgelist(p, pattern_only=pattern_only, mat_fmt=mat_fmt, symmetrize=bool(symmetrize)),
+              smat.parse_edgelist(text, pattern_only=pattern_only, mat_fmt=mat_fmt, symmetrize=bool(symmetrize))):
+        assert (c["rows"], c["cols"]) == (rows, cols)
+        assert np.array_equal(c["rowptr"], rp) and np.array_equal(c["colidx"], ci)
+        assert np.array_equal(c["values_f32"], v)
+    if symmetrize and name in ("el_sym", "mtx_sym"):  # the mirrored pattern is symmetric
+        dense = np.zeros((rows, rows), bool)
+        dense[np.repeat(np.arange(rows), np.diff(rp)), ci] = True
+        assert np.array_equal(dense, dense.T)
+
+
+def test_symmetrize_column_without_row_is_refused():
+    """An entry whose column lies past the last row: the reference indexes
+    past its row list (csr.cpp:248-249); the reader refuses the file."""
+    smat = _smat()
+    with pytest.raises(smat.SmatError) as e:
+        smat.parse_edgelist("0 1\n0 5\n1 2\n", pattern_only=True, symmetrize=True)
+    assert "no row" in str(e.value)

@@ -42,13 +42,20 @@ HBM_PEAK_GBS = 8000.0
 MFMA_PEAK_TFLOPS = {"f16": 2500.0, "bf16": 2500.0, "fp8": 5000.0}
 
 # HBM bytes per SpMM call measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-# (calibrated with a 1 GiB copy: x2.0 / x1.0) for configurations that have a
-# committed profile, under the automatic work schedule:
-# (workload, scale, N, bk, n_tile) -> (bytes, profile file).
-MEASURED_TRAFFIC = {
-    ("rmat", 18, 256, 2, "auto", "none"): (2159e6, "profiles/spmm_lds_bk2_rmat18_n256_hbm_traffic.md"),
-    ("rmat", 18, 256, 2, "auto", "locality"): (1933e6, "profiles/spmm_lds_bk2_locality_rmat18_n256_hbm_traffic.md"),
-}
+# (each scaled by what the counter reads for a copy of known size) for
+# configurations that have a committed profile, under the automatic work
+# schedule: (workload, scale, N, bk, n_tile, order) -> (bytes, profile file,
+# id of the kernel source the profile was taken with).  A figure is reported
+# only while the kernel source is the one it was measured on.
+MEASURED_TRAFFIC = {}
+KERNEL_SOURCE = os.path.join(ROOT, "sptc-smat_amd", "csrc", "spmm.hip")
+
+
+def kernel_source_id() -> str:
+    """First 12 hex digits of the SHA-256 of the SpMM kernel source."""
+    import hashlib
+    with open(KERNEL_SOURCE, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:12]
 
 
 def parse():
@@ -74,6 +81,9 @@ def parse():
                     help="tuned path: do not consider the locality row order")
     ap.add_argument("--no-ladder", action="store_true", help="skip the precision ladder (F16/BF16/FP8/split)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and preprocessing legs")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other workload configurations (band, R-MAT N=64, 2:4, power-law, batch, tau sweep)")
+    ap.add_argument("--no-tau-sweep", action="store_true", help="skip the Keeper tau sweep at R-MAT scale 18")
     ap.add_argument("--batch-tuned-only", action="store_true",
                     help="batch64: time only the learned picks (no sweeps, no plain / sweep-best legs)")
     ap.add_argument("--pre-scale", type=int, default=14, help="R-MAT scale of the reorder comparison")
@@ -214,7 +224,7 @@ class Run:
         self.plan = smat.SpmmPlan(self.A, ctx.N, n_tile=n_tile, b_staging=ctx.args.stage,
                                   max_blocks_per_item=ctx.args.split, schedule=ctx.args.schedule,
                                   kernel=smat.SMAT_KERNEL_MIXED24 if kernel == "mixed24" else smat.SMAT_KERNEL_DENSE,
-                                  in_dtype=smat.SMAT_BF16 if ctx.args.dtype == "bf16" else smat.SMAT_F16)
+                                  in_dtype=smat.SMAT_BF16 if ctx.dtype == "bf16" else smat.SMAT_F16)
         self.ctx = ctx
 
     def call(self):
@@ -355,7 +365,7 @@ def batch_plan(ctx, M, K, rp, ci, v, cfg, N, locality=False):
     return {"A": A, "plan": plan, "v24": v24, "i24": i24, "rowmap": rowmap, "keep": (d_rp, d_ci, d_v)}
 
 
-def batched_workload(args, log):
+def batched_workload(args, log, legs_mode=None):
     """64 seeded matrices of uneven size (n log-uniform 2k-200k, nnz 1e4-5e6,
     band / R-MAT / block-clustered / power-law), each times its own B (N
     columns).  Plain: 16x16 dense blocks, no reordering, one launch per matrix
@@ -377,6 +387,9 @@ def batched_workload(args, log):
         mats = pool.map(_gen_batch_matrix, range(64))
     log(f"generated 64 matrices in {time.time() - t0:.1f}s, nnz total {sum(m[1][3].size for m in mats)}")
     model = tuner.Model()
+    # "full": sweeps of every matrix plus the plain, tuned and sweep-best legs;
+    # "tuned": the learned picks only; "tuned_plain": learned picks and plain
+    legs_mode = legs_mode or ("tuned" if args.batch_tuned_only else "full")
     useful = sum(2.0 * m[1][3].size * N for m in mats)
     Bs, Cs, R = [], [], []
     picks, sweep_best, within, single_ms = [], [], [], [0.0, 0.0]
@@ -392,7 +405,7 @@ def batched_workload(args, log):
         feats = tuner.features(st, N)
         _, b = model.scores(feats, tuner.available(M))
         picks.append(tuner.CONFIGS[b])
-        if args.batch_tuned_only:
+        if legs_mode != "full":
             continue
         times, _ = tuner.sweep(M, K, rp, ci, v, N, B=Bs[-1])
         sb = int(np.argmin(times))
@@ -445,8 +458,9 @@ def batched_workload(args, log):
 
     res = {}
     s = torch.cuda.Stream()
-    legs = [("tuned", picks)] if args.batch_tuned_only else \
-        [("plain", [tuner.CONFIGS[tuner.PLAIN]] * 64), ("tuned", picks), ("sweep_best", sweep_best)]
+    plain_leg = ("plain", [tuner.CONFIGS[tuner.PLAIN]] * 64)
+    legs = {"tuned": [("tuned", picks)], "tuned_plain": [plain_leg, ("tuned", picks)],
+            "full": [plain_leg, ("tuned", picks), ("sweep_best", sweep_best)]}[legs_mode]
     for name, cfgs in legs:
         built = build(cfgs, ordered if name == "tuned" else None)
         if name == "plain":  # one launch per matrix, plans with the library's default slices
@@ -544,6 +558,289 @@ def precision_ladder(ctx, v32, Bf, bk, steps, warmup, log):
     return out
 
 
+def build_ctx(args, workload, N, dtype, log, data=None):
+    """A workload on the device: CSR, B (U[-1, 1], numpy seed 7) and C, and the
+    FP64 products of the rounded inputs (R_in) and of the FP32 inputs (R_32).
+    `data` reuses an already generated ((M, K, rp, ci, v, v32), desc)."""
+    import numpy as np
+    import scipy.sparse as sp
+    import torch
+    from smat import gen
+    t0 = time.time()
+    if data is None:
+        a = argparse.Namespace(**vars(args))
+        a.workload = workload
+        data = make_workload(a)
+    (M, K, rp, ci, v, v32), desc = data
+    if dtype == "bf16":
+        v = gen.f32_to_bf16_bits(v.view(np.float16).astype(np.float32))
+    ctx = Ctx()
+    ctx.args, ctx.M, ctx.K, ctx.N, ctx.nnz, ctx.dtype = args, M, K, N, int(ci.size), dtype
+    ctx.rp, ctx.ci, ctx.v, ctx.v32, ctx.desc, ctx.data = rp, ci, v, v32, desc, data
+    log(f"{desc}: {M} rows, nnz {ctx.nnz}, N {N}, {dtype}, ready in {time.time() - t0:.1f}s")
+    dev = torch.device("cuda:0")
+    ctx.d_rp = torch.from_numpy(rp).to(dev)
+    ctx.d_ci = torch.from_numpy(ci).to(dev)
+    ctx.d_v = torch.from_numpy(v.view(np.int16)).to(dev)
+    rng = np.random.default_rng(7)
+    Bf = rng.uniform(-1.0, 1.0, size=(K, N)).astype(np.float32)
+    ctx.Bf = Bf
+    Bbits = gen.f32_to_bf16_bits(Bf).reshape(K, N) if dtype == "bf16" else Bf.astype(np.float16).view(np.uint16)
+    ctx.Bbits = Bbits
+    ctx.B_host = (gen.bf16_bits_to_f32(Bbits).reshape(K, N) if dtype == "bf16"
+                  else Bbits.view(np.float16)).astype(np.float64)
+    ctx.vals_f64 = (gen.bf16_bits_to_f32(v) if dtype == "bf16" else v.view(np.float16)).astype(np.float64)
+    ctx.B = torch.from_numpy(np.ascontiguousarray(Bbits).view(np.int16)).to(dev)
+    ctx.C = torch.empty((M, N), dtype=torch.float32, device=dev)
+    # FP64 references on the host (scipy): the product of the rounded inputs
+    # the kernels consume, and of the FP32 inputs (the precision ladder's truth).
+    t1 = time.time()
+    A64 = sp.csr_matrix((ctx.vals_f64, ci, rp), shape=(M, K))
+    ctx.R_in = torch.from_numpy(np.ascontiguousarray(A64 @ ctx.B_host)).to(dev)
+    A32 = sp.csr_matrix((v32.astype(np.float64), ci, rp), shape=(M, K))
+    ctx.R_32 = torch.from_numpy(np.ascontiguousarray(A32 @ Bf.astype(np.float64))).to(dev)
+    del A64, A32
+    log(f"FP64 references in {time.time() - t1:.1f}s")
+    return ctx
+
+
+def measure_config(ctx, bk, nt, kernel, steps, warmup, log, order=None, order_name="locality"):
+    """Times one configuration (Run) and checks C against the FP64 product;
+    `order` is a row-permuted copy of the matrix (named `order_name`)."""
+    import smat
+    r = Run(ctx, bk, nt, kernel, order).time(steps, warmup)
+    err = smat.rel_fro_error(ctx.C, ctx.R_in)
+    st = r.stats()
+    st["rel_fro_err"] = err
+    st["rel_fro_err_vs_fp32_inputs"] = smat.rel_fro_error(ctx.C, ctx.R_32)
+    st["order"] = order_name if order is not None else "none"
+    log(f"bk={bk:2d} n_tile={nt or 'auto':>4} {kernel} order {st['order']}: {st['event_ms']:.3f} ms, "
+        f"{st['useful_gflops']:.1f} useful GFLOP/s, rel err {err:.2e}")
+    r.close()
+    if not err < 1e-5:
+        print(json.dumps({"error": f"SpMM result check failed for bk={bk}: rel error {err}"}))
+        sys.exit(3)
+    return st
+
+
+def compact(st):
+    """The figures of one timed configuration that the JSON line keeps."""
+    out = {k: st[k] for k in ("bk", "n_tile", "kernel", "order", "nnzb", "event_ms", "useful_gflops", "alg_bytes",
+                              "achieved_gbs", "rel_fro_err") if k in st}
+    out["roofline_frac"] = round(st["achieved_gbs"] / HBM_PEAK_GBS, 4)
+    return out
+
+
+def learned_config(ctx, log, steps, warmup, locality=True):
+    """The plain 16x16 call and the learned pick (with the locality probe on
+    matrices above the Keeper's row limit) of one workload."""
+    from smat import tuner
+    plain = measure_config(ctx, 16, 0, "dense", steps, warmup, log)
+    learned = learned_pick(ctx, log)
+    if learned is None:
+        return {"pick": None, "plain": compact(plain)}
+    tau, bk, nt, kernel = learned["config"]
+    order, probe = None, None
+    if locality and ctx.M > tuner.REORDER_MAX_ROWS:
+        order, probe = locality_choice(ctx, learned, log)
+        if order is not None:
+            tau, bk, nt, kernel = probe["config"]
+    tuned = measure_config(ctx, bk, nt, kernel, steps, warmup, log, order)
+    return {"pick": [tau, bk, nt, kernel], "locality_probe": probe, "tuned": compact(tuned), "plain": compact(plain),
+            "speedup_vs_plain": round(tuned["useful_gflops"] / plain["useful_gflops"], 3),
+            "_order": order, "_tuned": tuned}
+
+
+def dropin_leg(ctx, log, steps):
+    """The reference's own launcher, mmaOBTKernel (16x16 BCSR, B[n*K + k],
+    FP16 C), on the workload, timed per call as a reference caller sees it
+    (wall clock: the call checks its arrays and plans on the host)."""
+    import numpy as np
+    import torch
+    import smat
+    A = smat.Bcsr(ctx.M, ctx.K, ctx.d_rp, ctx.d_ci, ctx.d_v, bk=16)
+    info, rel = A.grid_maps()
+    Bt = torch.from_numpy(np.ascontiguousarray(ctx.Bbits.T).view(np.int16)).cuda()
+    C16 = torch.empty((ctx.M, ctx.N), dtype=torch.float16, device="cuda")
+    d = A.desc
+    lib = smat.lib()
+    call = lambda: lib.mmaOBTKernel(d.values, d.rowptr, d.colidx, Bt.data_ptr(), C16.data_ptr(), ctx.M,  # noqa: E731
+                                    ctx.N, ctx.K, d.nnzb, info.data_ptr(), rel.data_ptr())
+    lib.smat_clear_error()
+    for _ in range(2):
+        call()
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    for _ in range(steps):
+        call()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - w0) * 1e3 / steps
+    err = smat.rel_fro_error(C16.float(), ctx.R_in)
+    st = int(lib.smat_last_status())
+    out = {"launcher": "mmaOBTKernel (main.cu:25-28 signature), 16x16 BCSR, B[n*K+k], FP16 C", "wall_ms": round(ms, 4),
+           "useful_gflops": round(2.0 * ctx.nnz * ctx.N / (ms * 1e-3) / 1e9, 1), "rel_fro_err": err, "status": st}
+    log(f"drop-in mmaOBTKernel: {out}")
+    del info, rel, Bt, C16
+    A.close()
+    return out
+
+
+def tau_sweep_leg(ctx, tuned_cfg, log, steps, warmup):
+    """The Jaccard Keeper reorder (IterativeBlockingKeeper, 16-column blocks,
+    16-row groups) at tau 0.1 / 0.3 / 0.5 / 0.7, the four runs on four host
+    threads with a stream each; per tau the grouping is checked against the
+    reference's own output (tests/golden/keeper_rmat18_golden.npz, written
+    in the build container by the compiled reference), the reordered matrix is
+    built (fixed-size groups, get_permutation order) and the plain 16x16 call
+    and the tuned block width are timed on it, C written back through the row
+    map.  The reference's threshold search picks the tau with the fewest
+    nonzero blocks (ties to the later tau, reorder_matrix.cpp:17-40)."""
+    import threading
+    import numpy as np
+    import torch
+    import smat
+    taus = (0.1, 0.3, 0.5, 0.7)
+    rows = ctx.M
+    rp, ci = ctx.rp[: rows + 1], ctx.ci
+    res, errs = {}, []
+
+    def work(tau, stream):
+        try:
+            t0 = time.perf_counter()
+            g, cnt = smat.reorder_grouping(rp, ci, np.float32(tau), 16, 16, stream=stream)
+            res[tau] = (g, cnt, time.perf_counter() - t0)
+        except Exception as ex:  # reported below
+            errs.append(f"tau {tau}: {ex}")
+
+    streams = [torch.cuda.Stream() for _ in taus]
+    t0 = time.perf_counter()
+    threads = [threading.Thread(target=work, args=(t, s)) for t, s in zip(taus, streams)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    wall = time.perf_counter() - t0
+    if errs:
+        return {"error": "; ".join(errs)}
+    gold = None
+    gpath = os.path.join(ROOT, "tests", "golden", "keeper_rmat18_golden.npz")
+    if os.path.exists(gpath) and ctx.M == 1 << 18:
+        gold = np.load(gpath)
+    _, bk, nt, kernel = tuned_cfg
+    out = {"workload": "R-MAT scale 18, Keeper 16x16 groups, four taus concurrently", "wall_s_all_taus": round(wall, 2),
+           "taus": {}}
+    best_tau, best = -1.0, ctx.nnz
+    for tau in taus:
+        g, cnt, sec = res[tau]
+        perm = smat.grouping_to_permutation(smat.fixed_size_grouping(g, 16))
+        d_perm = torch.from_numpy(perm).cuda()
+        order = smat.permute_rows(ctx.d_rp, ctx.d_ci, ctx.d_v, d_perm) + (d_perm.to(torch.int32),)
+        plain = measure_config(ctx, 16, 0, "dense", steps, warmup, log, order, f"keeper tau {tau}")
+        tuned = measure_config(ctx, bk, nt, "dense", steps, warmup, log, order, f"keeper tau {tau}")
+        rec = {"keeper_s": round(sec, 2), "comparisons": int(cnt[0]), "merges": int(cnt[1]),
+               "nnzb16": plain["nnzb"], "plain": compact(plain), "tuned": compact(tuned)}
+        if gold is not None:
+            key = f"t{tau}"
+            rec["identical_to_reference"] = bool(np.array_equal(g, gold[key + "__grouping"].astype(np.int64)) and
+                                                 np.array_equal(cnt, gold[key + "__counters"]))
+            rec["reference_cpu_s"] = round(float(gold[key + "__ref_seconds"][0]), 1)
+        out["taus"][str(tau)] = rec
+        if plain["nnzb"] <= best:
+            best_tau, best = tau, plain["nnzb"]
+        log(f"tau {tau}: {rec}")
+    out["threshold_search"] = {"tau": best_tau, "nnzb16": best}
+    if gold is not None:
+        out["reference_cpu_note"] = ("reference_cpu_s: the reference's IterativeBlockingKeeper compiled from its "
+                                     "sources, one core of the build container, when the goldens were written")
+    return out
+
+
+def config_legs(args, log, head_ctx, head_cfg):
+    """Every other workload configuration, each with its learned pick, the
+    plain call, the roofline fraction of the tuned call and its error:
+    band 4096 (N 128), R-MAT s18 at N 64, block-clustered 2:4 (N 256, FP16 and
+    BF16, 2:4 SMFMAC against dense MFMA), power-law (N 128, the four precision
+    modes), 64 matrices in one graph (N 128), the Keeper tau sweep at R-MAT
+    s18 (N 256) and the reference's launcher at R-MAT s18 (N 256)."""
+    import gc
+    import torch
+    out = {}
+    steps, warmup = max(5, args.steps // 2), 2
+
+    def leg(name, fn):
+        t0 = time.time()
+        try:
+            out[name] = fn()
+        except Exception as ex:  # a report beside the headline, never the measurement
+            out[name] = {"error": repr(ex)}
+        if isinstance(out[name], dict):
+            out[name]["leg_s"] = round(time.time() - t0, 1)
+            out[name].pop("_order", None)
+            out[name].pop("_tuned", None)
+        gc.collect()
+        torch.cuda.empty_cache()
+
+    def c1():
+        ctx = build_ctx(args, "band", 128, "f16", log)
+        r = learned_config(ctx, log, steps, warmup)
+        r["workload"] = ctx.desc + ", N 128"
+        r["dropin"] = dropin_leg(ctx, log, steps)
+        return r
+
+    def c2_64():
+        ctx = build_ctx(args, "rmat", 64, "f16", log, data=head_ctx.data)
+        r = learned_config(ctx, log, steps, warmup)
+        r["workload"] = ctx.desc + ", N 64"
+        return r
+
+    def c3():
+        data = None
+        r = {}
+        for dt in ("f16", "bf16"):
+            ctx = build_ctx(args, "clustered", 256, dt, log, data=data)
+            data = ctx.data
+            rr = learned_config(ctx, log, steps, warmup)
+            sm = measure_config(ctx, 16, 0, "mixed24", steps, warmup, log)
+            dn = measure_config(ctx, 16, 0, "dense", steps, warmup, log)
+            rr["smfmac_24"] = compact(sm)
+            rr["dense_mfma_16"] = compact(dn)
+            rr["workload"] = ctx.desc + f", N 256, {dt}"
+            r[dt] = rr
+            del ctx
+        return r
+
+    def c4():
+        ctx = build_ctx(args, "powerlaw", 128, "f16", log)
+        r = learned_config(ctx, log, steps, warmup)
+        r["workload"] = ctx.desc + ", N 128"
+        bk = r["tuned"]["bk"] if "tuned" in r else 1
+        r["precision_ladder"] = precision_ladder(ctx, ctx.v32, ctx.Bf, bk, steps, warmup, log)
+        for m in r["precision_ladder"]:
+            m["roofline_frac"] = round(m["achieved_gbs"] / HBM_PEAK_GBS, 4)
+        return r
+
+    def c5():
+        a = argparse.Namespace(**vars(args))
+        a.n, a.steps, a.warmup = 128, steps, warmup
+        r = batched_workload(a, log, legs_mode="tuned_plain")
+        res = r["results"]
+        return {"workload": "64 seeded matrices (n 2k-200k log-uniform), N 128, one hipGraph-captured batch",
+                "nnz_total": sum(r["nnz"]), "results": res,
+                "speedup_vs_plain_sequential": round(res["tuned_batched"]["useful_gflops"] /
+                                                     res["plain_sequential"]["useful_gflops"], 3),
+                "locality_ordered": r["model"]["locality_ordered"]}
+
+    leg("band4096_n128", c1)
+    leg("rmat18_n64", c2_64)
+    leg("clustered24_n256", c3)
+    leg("powerlaw_n128", c4)
+    leg("batch64_n128", c5)
+    if not args.no_tau_sweep:
+        leg("rmat18_n256_tau_sweep", lambda: tau_sweep_leg(head_ctx, head_cfg, log, steps, warmup))
+    leg("rmat18_n256_dropin", lambda: dropin_leg(head_ctx, log, steps))
+    return out
+
+
 def main():
     args = parse()
     if args.calib_copy_mib:
@@ -576,51 +873,10 @@ def main():
             "batched": res, "model": r["model"], "matrices": {"kinds": r["kinds"], "rows": r["rows"],
                                                               "nnz": r["nnz"]}}))
         return
-    t0 = time.time()
-    (M, K, rp, ci, v, v32), desc = make_workload(args)
-    if args.dtype == "bf16":
-        v = gen.f32_to_bf16_bits(v.view(np.float16).astype(np.float32))
-    ctx = Ctx()
-    ctx.args, ctx.M, ctx.K, ctx.N, ctx.nnz = args, M, K, args.n, int(ci.size)
-    log(f"{desc}: {M} rows, nnz {ctx.nnz}, generated in {time.time() - t0:.1f}s")
-    dev = torch.device("cuda:0")
-    ctx.d_rp = torch.from_numpy(rp).to(dev)
-    ctx.d_ci = torch.from_numpy(ci).to(dev)
-    ctx.d_v = torch.from_numpy(v.view(np.int16)).to(dev)
-    rng = np.random.default_rng(7)
-    Bf = rng.uniform(-1.0, 1.0, size=(K, ctx.N)).astype(np.float32)
-    Bbits = gen.f32_to_bf16_bits(Bf).reshape(K, ctx.N) if args.dtype == "bf16" else Bf.astype(np.float16).view(np.uint16)
-    ctx.B_host = (gen.bf16_bits_to_f32(Bbits).reshape(K, ctx.N) if args.dtype == "bf16"
-                  else Bbits.view(np.float16)).astype(np.float64)
-    ctx.vals_f64 = (gen.bf16_bits_to_f32(v) if args.dtype == "bf16" else v.view(np.float16)).astype(np.float64)
-    ctx.B = torch.from_numpy(np.ascontiguousarray(Bbits).view(np.int16)).to(dev)
-    ctx.C = torch.empty((M, ctx.N), dtype=torch.float32, device=dev)
-
-    # FP64 references on the host (scipy): the product of the rounded inputs
-    # the kernels consume, and of the FP32 inputs (the precision ladder's truth).
-    import scipy.sparse as sp
-    t1 = time.time()
-    A64 = sp.csr_matrix((ctx.vals_f64, ci, rp), shape=(M, K))
-    ctx.R_in = torch.from_numpy(np.ascontiguousarray(A64 @ ctx.B_host)).to(dev)
-    A32 = sp.csr_matrix((v32.astype(np.float64), ci, rp), shape=(M, K))
-    ctx.R_32 = torch.from_numpy(np.ascontiguousarray(A32 @ Bf.astype(np.float64))).to(dev)
-    del A64, A32
-    log(f"FP64 references in {time.time() - t1:.1f}s")
-
-    def measure(bk, nt, kernel, steps, warmup, order=None):
-        r = Run(ctx, bk, nt, kernel, order).time(steps, warmup)
-        err = smat.rel_fro_error(ctx.C, ctx.R_in)
-        st = r.stats()
-        st["rel_fro_err"] = err
-        st["rel_fro_err_vs_fp32_inputs"] = smat.rel_fro_error(ctx.C, ctx.R_32)
-        st["order"] = "locality" if order is not None else "none"
-        log(f"bk={bk:2d} n_tile={nt or 'auto':>4} {kernel} order {st['order']}: {st['event_ms']:.3f} ms, "
-            f"{st['useful_gflops']:.1f} useful GFLOP/s, rel err {err:.2e}")
-        r.close()
-        if not err < 1e-5:
-            print(json.dumps({"error": f"SpMM result check failed for bk={bk}: rel error {err}"}))
-            sys.exit(3)
-        return st
+    ctx = build_ctx(args, args.workload, args.n, args.dtype, log)
+    M, K, rp, ci, v, v32, Bf, desc = ctx.M, ctx.K, ctx.rp, ctx.ci, ctx.v, ctx.v32, ctx.Bf, ctx.desc
+    measure = lambda bk, nt, kernel, steps, warmup, order=None: measure_config(  # noqa: E731
+        ctx, bk, nt, kernel, steps, warmup, log, order)
 
     learned = sweep_best = None
     if args.single:
@@ -671,10 +927,16 @@ def main():
             cpu = cpu_baseline(M, rp, ci, v16, ctx.N, args.cpu_seconds)
         except Exception as ex:  # the baseline is a report, never the measurement
             cpu = {"error": str(ex)}
+    configs = None
+    if not args.no_configs and not args.single and args.workload == "rmat":
+        configs = config_legs(args, log, ctx, [None, tuned["bk"], tuned["n_tile"] if tuned["n_tile"] != "auto" else 0,
+                                               tuned["kernel"]])
     # "auto" N-tiles are 64 columns for N >= 64
     nt_key = "auto" if tuned["n_tile"] == "auto" or (tuned["n_tile"] == 64 and ctx.N >= 64) else tuned["n_tile"]
     traffic = MEASURED_TRAFFIC.get((args.workload, args.scale, ctx.N, tuned["bk"], nt_key, tuned["order"])) \
         if tuned["kernel"] == "dense" and args.dtype == "f16" and args.split == 0 and args.schedule == 0 else None
+    if traffic and traffic[2] != kernel_source_id():
+        traffic = None  # measured on another build of the kernels
     out = {
         "metric": "SpMM useful GFLOP/s (2*nnz*N / t), learned-tuned vs plain FP16 BCSR",
         "value": tuned["useful_gflops"],
@@ -707,6 +969,7 @@ def main():
                      "frac": round(tuned["achieved_gbs"] / HBM_PEAK_GBS, 4),
                      "traffic": traffic[0] if traffic else None,
                      "traffic_source": traffic[1] if traffic else None,
+                     "kernel_source_id": kernel_source_id(),
                      "kernel": "SpMM call (main kernel + split-row reduction)",
                      "alg_bytes_per_call": tuned["alg_bytes"], "event_ms_avg": tuned["event_ms"]},
         "mfma": {"achieved": round(tuned["stored_gflops"] / 1e3, 2), "peak": MFMA_PEAK_TFLOPS[args.dtype],
@@ -716,6 +979,7 @@ def main():
         "precision_ladder": ladder,
         "cpu_baseline": cpu,
         "preprocess": pre,
+        "configs": configs,
     }
     print(json.dumps(out))
 

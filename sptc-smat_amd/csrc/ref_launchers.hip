@@ -4,8 +4,20 @@
 //  * accumulation is FP32 (the reference's HMMA accumulates in FP16,
 //    src/cuda_hgemm/src/common/ptx.h:20-41);
 //  * a failure is recorded for smat_last_error() instead of exit();
-//  * no plan: one wave per (block row, 64-column tile), blocks read through
-//    relativeBlockIndexMapping like the reference kernels (mmaOBT.cu:56-61).
+//  * the SpMM launchers first check on the GPU that their arrays are the
+//    BCSR csrToBcsr writes (relativeBlockIndexMapping points every block at
+//    its own position, and for the mixed ones every blockInfo 1 block's
+//    compressed pair holds exactly its values in bcsrValuesA); then they run
+//    the planned dense-block kernel of smat_spmm_execute on the reference's
+//    B layout, planned per call in the library's static scratch
+//    (smat_spmm_noalloc; one host sync per call for the check and two for
+//    the plan), with B first transposed into a library-owned row-major copy
+//    (grown on demand, kept per device until the process ends).  Otherwise (or when the plan does not fit
+//    the scratch) they run k_ref_spmm: one wave per (block row, 64-column
+//    tile), blocks read through relativeBlockIndexMapping and blockInfo like
+//    the reference kernels (mmaOBT.cu:56-61).
+#include <mutex>
+
 #include "smat_internal.h"
 
 namespace smat {
@@ -157,6 +169,179 @@ __global__ void k_split32(int64_t nnzb, const uint16_t* __restrict__ vals, const
   if (r == 0) tags2[2 * b] = tags2[2 * b + 1] = tags[b];
 }
 
+// Fast-path check, one thread per block p: rowptr bounds, the block's
+// column start (a multiple of BKW inside K), relativeBlockIndexMapping of its
+// grid cell == p and, with MIXED, a blockInfo 1 block's compressed 2:4 pair
+// (reference k=16 layout, two halves per 16x32 block) expanding to its values
+// in `vals` (zeros of either sign count as equal).  Any mismatch sets *flag.
+template <int BKW, bool MIXED>
+__global__ void k_check_planned(int64_t nbrows, int64_t nnzb, int64_t K, const int* __restrict__ rowptr,
+                                const int* __restrict__ colidx, const uint16_t* __restrict__ vals,
+                                const int* __restrict__ blockInfo, const int* __restrict__ relMap,
+                                const uint8_t* __restrict__ meta, const uint16_t* __restrict__ sparseA,
+                                int* __restrict__ flag) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p == 0 && (rowptr[0] != 0 || rowptr[nbrows] != nnzb)) atomicOr(flag, 1);
+  if (p >= nnzb) return;
+  // block row of p: the last r with rowptr[r] <= p
+  int64_t lo = 0, hi = nbrows;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (rowptr[mid] <= p) lo = mid;
+    else hi = mid;
+  }
+  const int col = colidx[p];
+  const int64_t ncr = K / BKW;
+  if (col < 0 || col % BKW != 0 || col / BKW >= ncr) {
+    atomicOr(flag, 1);
+    return;
+  }
+  const int64_t g = lo * ncr + col / BKW;
+  if (relMap[g] != p) {
+    atomicOr(flag, 1);
+    return;
+  }
+  if (!MIXED || blockInfo[g] != 1) return;
+  for (int h = 0; h < BKW / 16; h++) {
+    const int64_t sb = BKW == 16 ? p : 2 * p + h;  // 16x16 (half-)block of the compressed arrays
+    for (int q = 0; q < 32; q++) {
+      const smat_h8 e = expand_ref24(sparseA + sb * 128 + q * 4, meta[sb * 32 + q]);
+      const uint4 eb = __builtin_bit_cast(uint4, e);
+      const uint16_t* ev = reinterpret_cast<const uint16_t*>(&eb);
+      const int r = q / 2, c0 = (q % 2) * 8;
+      for (int j = 0; j < 8; j++) {
+        const uint16_t d = vals[p * 16 * BKW + r * BKW + h * 16 + c0 + j];
+        if (d != ev[j] && ((d | ev[j]) & 0x7fff) != 0) {
+          atomicOr(flag, 1);
+          return;
+        }
+      }
+    }
+  }
+}
+
+// B in the reference's layout (element (k, n) at n*K + k) -> row-major K x N
+// (k*N + n), 64 x 64 tiles through LDS: 16-byte reads along k, 2-byte
+// writes along n coalesced per wave row.
+__global__ __launch_bounds__(256) void k_transpose_b(const uint16_t* __restrict__ bt, uint16_t* __restrict__ b,
+                                                     int64_t K, int64_t N) {
+  __shared__ uint16_t tile[64][65];
+  const int64_t k0 = (int64_t)blockIdx.x * 64, n0 = (int64_t)blockIdx.y * 64;
+  const int t = threadIdx.x;
+  // read: 64 rows n of 64 halves (k), 8 halves per thread
+  for (int i = t; i < 64 * 8; i += 256) {
+    const int nn = i / 8, kk = (i % 8) * 8;
+    const int64_t n = n0 + nn, k = k0 + kk;
+    if (n < N && k + 8 <= K) {
+      const uint4 v = *reinterpret_cast<const uint4*>(bt + n * K + k);
+      const uint16_t* e = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+      for (int j = 0; j < 8; j++) tile[kk + j][nn] = e[j];
+    } else if (n < N) {
+      for (int j = 0; j < 8 && k + j < K; j++) tile[kk + j][nn] = bt[n * K + k + j];
+    }
+  }
+  __syncthreads();
+  for (int i = t; i < 64 * 64; i += 256) {
+    const int kk = i / 64, nn = i % 64;
+    const int64_t k = k0 + kk, n = n0 + nn;
+    if (k < K && n < N) b[k * N + n] = tile[kk][nn];
+  }
+}
+
+// Library-owned row-major copy of B for the launchers, per device, grown on
+// demand and kept until the process ends.
+constexpr int kMaxDev = 64;
+std::mutex g_bstage_mu;
+void* g_bstage[kMaxDev] = {};
+size_t g_bstage_bytes[kMaxDev] = {};
+
+uint16_t* staged_b(size_t bytes) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_bstage_mu);
+  if (g_bstage_bytes[dev] < bytes) {
+    if (g_bstage[dev]) (void)hipFree(g_bstage[dev]);
+    g_bstage[dev] = nullptr;
+    g_bstage_bytes[dev] = 0;
+    if (hipMalloc(&g_bstage[dev], bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      g_bstage[dev] = nullptr;
+      return nullptr;
+    }
+    g_bstage_bytes[dev] = bytes;
+  }
+  return static_cast<uint16_t*>(g_bstage[dev]);
+}
+
+__global__ void k_zero_flag(int* flag) {
+  if (threadIdx.x == 0) *flag = 0;
+}
+
+// The planned path of the SpMM launchers (see the file comment); false when
+// it does not apply and the caller runs k_ref_spmm.
+template <int BKW, bool MIXED>
+bool planned_spmm(const uint16_t* vals, const int* rowptr, const int* colidx, const uint8_t* meta,
+                  const uint16_t* sparseA, const uint16_t* B, uint16_t* C, size_t M, size_t N, size_t K,
+                  size_t nnzb, const int* blockInfo, const int* relMap) {
+  if (nnzb == 0 || nnzb >= (size_t)INT32_MAX) return false;
+  void* base = nullptr;
+  if (smat::noalloc_scratch(&base) != SMAT_OK) {
+    smat_clear_error();
+    return false;
+  }
+  int* flag = static_cast<int*>(base);
+  const int64_t nbrows = (int64_t)M / 16;
+  hipLaunchKernelGGL(k_zero_flag, 1, 64, 0, 0, flag);
+  hipLaunchKernelGGL((k_check_planned<BKW, MIXED>), (unsigned)((nnzb + 255) / 256), 256, 0, 0, nbrows,
+                     (int64_t)nnzb, (int64_t)K, rowptr, colidx, vals, blockInfo, relMap, meta, sparseA, flag);
+  int h = 1;
+  if (hipGetLastError() != hipSuccess || hipMemcpy(&h, flag, 4, hipMemcpyDeviceToHost) != hipSuccess || h != 0) {
+    (void)hipGetLastError();
+    return false;
+  }
+  smat_bcsr_desc d{};
+  d.M = (int64_t)M;
+  d.K = (int64_t)K;
+  d.bm = 16;
+  d.bk = BKW;
+  d.nbrows = nbrows;
+  d.nnzb = (int64_t)nnzb;
+  d.rowptr = const_cast<int32_t*>(rowptr);
+  d.colidx = const_cast<int32_t*>(colidx);
+  d.values = const_cast<uint16_t*>(vals);
+  smat_spmm_opts o{};
+  o.kernel = SMAT_KERNEL_DENSE;
+  o.out_dtype = SMAT_F16;
+  o.in_dtype = SMAT_F16;
+  smat_spmm_operands ops{};
+  ops.c = C;
+  ops.ldc = (int64_t)N;
+  // B[n*K + k] gathers touch 32 bytes of a 128-byte line per block row
+  // segment; a row-major copy (one transpose, 2 x K x N x 2 bytes of traffic)
+  // lets the kernel stage whole B lines through LDS
+  uint16_t* Brm = (reinterpret_cast<uintptr_t>(B) & 15) == 0 ? staged_b(K * N * sizeof(uint16_t)) : nullptr;
+  if (Brm) {
+    hipLaunchKernelGGL(k_transpose_b, dim3((unsigned)((K + 63) / 64), (unsigned)((N + 63) / 64)), 256, 0, 0, B, Brm,
+                       (int64_t)K, (int64_t)N);
+    o.b_layout = SMAT_B_ROW_MAJOR;
+    ops.b = Brm;
+    ops.ldb = (int64_t)N;
+  } else {
+    o.b_layout = SMAT_B_REFERENCE;
+    ops.b = B;
+    ops.ldb = (int64_t)K;
+  }
+  if (smat_spmm_noalloc(&d, (int64_t)N, &o, &ops, nullptr) != SMAT_OK) {
+    smat_clear_error();
+    return false;
+  }
+  return true;
+}
+
 bool check_ref_args(const void* a, const void* b, const void* c, size_t M, size_t N, size_t K, const char* who,
                     size_t kw = 16) {
   if (!a || !b || !c) {
@@ -178,13 +363,15 @@ extern "C" {
 void mmaOBTKernel(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdxA, uint16_t* B, uint16_t* C,
                   size_t M, size_t N, size_t K, size_t nonzeroBlocks, int* blockInfo,
                   int* relativeBlockIndexMapping) {
-  (void)nonzeroBlocks;
   (void)blockInfo;
   if (!check_ref_args(bcsrRowPtrA, B, C, M, N, K, "mmaOBTKernel")) return;
   if (!relativeBlockIndexMapping || !bcsrColIdxA || !bcsrValuesA) {
     smat::set_error(SMAT_ERR_INVALID, "mmaOBTKernel: null BCSR array");
     return;
   }
+  if (planned_spmm<16, false>(bcsrValuesA, bcsrRowPtrA, bcsrColIdxA, nullptr, nullptr, B, C, M, N, K, nonzeroBlocks,
+                              nullptr, relativeBlockIndexMapping))
+    return;
   const int64_t ntiles = ((int64_t)N + 63) / 64;
   const int64_t tasks = (int64_t)(M / 16) * ntiles;
   hipLaunchKernelGGL((k_ref_spmm<kAllDense, 16, false>), (unsigned)((tasks + kWaves - 1) / kWaves), 64 * kWaves,
@@ -197,13 +384,15 @@ void mmaOBTKernel(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdxA, uin
 void mmaOBTSKernel(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdxA, char* metadata,
                    uint16_t* sparseMatrixA, uint16_t* B, uint16_t* C, size_t M, size_t N, size_t K,
                    size_t nonzeroBlocks, int* blockInfo, int* relativeBlockIndexMapping) {
-  (void)nonzeroBlocks;
   if (!check_ref_args(bcsrRowPtrA, B, C, M, N, K, "mmaOBTSKernel")) return;
   if (!relativeBlockIndexMapping || !blockInfo || !bcsrColIdxA || !bcsrValuesA || !metadata ||
       !sparseMatrixA) {
     smat::set_error(SMAT_ERR_INVALID, "mmaOBTSKernel: null BCSR or 2:4 array");
     return;
   }
+  if (planned_spmm<16, true>(bcsrValuesA, bcsrRowPtrA, bcsrColIdxA, reinterpret_cast<const uint8_t*>(metadata),
+                             sparseMatrixA, B, C, M, N, K, nonzeroBlocks, blockInfo, relativeBlockIndexMapping))
+    return;
   const int64_t ntiles = ((int64_t)N + 63) / 64;
   const int64_t tasks = (int64_t)(M / 16) * ntiles;
   hipLaunchKernelGGL((k_ref_spmm<kPerBlock, 16, false>), (unsigned)((tasks + kWaves - 1) / kWaves), 64 * kWaves,
@@ -261,13 +450,15 @@ void mmaOBTKernel_large(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdx
 void mmaOBTSKernel_large(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdxA, char* metadata,
                          uint16_t* sparseMatrixA, uint16_t* B, uint16_t* C, size_t M, size_t N, size_t K,
                          size_t nonzeroBlocks, int* blockInfo, int* relativeBlockIndexMapping) {
-  (void)nonzeroBlocks;
   if (!check_ref_args(bcsrRowPtrA, B, C, M, N, K, "mmaOBTSKernel_large", 32)) return;
   if (!relativeBlockIndexMapping || !blockInfo || !bcsrColIdxA || !bcsrValuesA || !metadata ||
       !sparseMatrixA) {
     smat::set_error(SMAT_ERR_INVALID, "mmaOBTSKernel_large: null BCSR or 2:4 array");
     return;
   }
+  if (planned_spmm<32, true>(bcsrValuesA, bcsrRowPtrA, bcsrColIdxA, reinterpret_cast<const uint8_t*>(metadata),
+                             sparseMatrixA, B, C, M, N, K, nonzeroBlocks, blockInfo, relativeBlockIndexMapping))
+    return;
   const int64_t ntiles = ((int64_t)N + 63) / 64;
   const int64_t tasks = (int64_t)(M / 16) * ntiles;
   hipLaunchKernelGGL((k_ref_spmm<kPerBlock, 32, false>), (unsigned)((tasks + kWaves - 1) / kWaves), 64 * kWaves,

@@ -65,6 +65,11 @@ struct HostBuf {
   T* dev() const { return static_cast<T*>(d); }
 };
 
+// Device address of the current GPU's static scratch of smat_spmm_noalloc
+// (SMAT_NOALLOC_SCRATCH_BYTES).  Users of it run before smat_spmm_noalloc on
+// the same stream and are done with it (host-synchronized) when it starts.
+smat_status noalloc_scratch(void** base);
+
 }  // namespace smat
 
 #define SMAT_HIP_TRY(expr)                                                      \

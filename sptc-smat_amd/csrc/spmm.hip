@@ -1622,6 +1622,10 @@ smat_status noalloc_base(void** out) {
 
 }  // namespace
 
+namespace smat {
+smat_status noalloc_scratch(void** base) { return noalloc_base(base); }
+}  // namespace smat
+
 extern "C" {
 
 smat_status smat_spmm_noalloc(const smat_bcsr_desc* A, int64_t N, const smat_spmm_opts* opts,

@@ -375,3 +375,54 @@ def test_large_dropins_alias_and_arguments():
     smat.lib().preprocessing_mmaSTKernel_large(d.values, d.values, d.values, M, N, 48, d.nnzb, info.data_ptr(),
                                                rel.data_ptr())
     assert smat.lib().smat_last_status() == smat.SMAT_ERR_INVALID
+
+
+def test_dropin_planned_and_reference_paths():
+    """The SpMM launchers run the planned kernel when their arrays are the
+    BCSR csrToBcsr writes, and the per-block reference path otherwise: a grid
+    map that stores the blocks at permuted slots (values moved with them)
+    gives the same product through the reference path; the mixed launchers
+    with intact dense copies of their 2:4 blocks take the planned path."""
+    _need_gpu()
+    import smat
+    N = 72
+    M, K, A, Bt, R, Rsim = _ref_case(93, N)
+    info, rel = A.grid_maps()
+    d = A.desc
+    nb = int(d.nnzb)
+    # blocks stored at permuted slots: relMap[g(p)] = sigma(p), values at sigma(p)
+    sigma = np.random.default_rng(3).permutation(nb).astype(np.int32)
+    hv = torch.empty(nb * 256, dtype=torch.int16)
+    smat.copy_to_host(hv.data_ptr(), d.values, nb * 512)
+    moved = torch.empty_like(hv).view(nb, 256)
+    moved[torch.from_numpy(sigma).long()] = hv.view(nb, 256)
+    vals2 = moved.view(-1).cuda()
+    relh = rel.cpu().numpy()
+    rel2 = np.where(relh >= 0, sigma[np.maximum(relh, 0)], -1).astype(np.int32)
+    rel2d = torch.from_numpy(rel2).cuda()
+    for vals, relmap, what in ((vals2, rel2d, "reference path"), (None, rel, "planned path")):
+        C = torch.full((M, N), float("nan"), dtype=torch.float16, device="cuda")
+        smat.lib().smat_clear_error()
+        smat.lib().mmaOBTKernel(vals.data_ptr() if vals is not None else d.values, d.rowptr, d.colidx,
+                                Bt.data_ptr(), C.data_ptr(), M, N, K, nb, info.data_ptr(), relmap.data_ptr())
+        torch.cuda.synchronize()
+        assert smat.lib().smat_last_status() == 0, (what, smat.lib().smat_last_error())
+        Cn = C.cpu().numpy().astype(np.float64)
+        assert np.linalg.norm(Cn - R) / np.linalg.norm(R) < 1e-3, what
+        assert np.max(np.abs(Cn - Rsim)) / np.max(np.abs(Rsim)) < 1e-2, what
+    # the mixed launcher with intact dense copies (planned path)
+    sp = torch.zeros(nb * 128, dtype=torch.int16, device="cuda")
+    meta = torch.zeros(nb * 32, dtype=torch.uint8, device="cuda")
+    _, _, rp, ci, v = mixed_matrix(seed=93)
+    A24 = _bcsr(M, K, rp, ci, v, rule=smat.SMAT_TAG_EXACT24)
+    info24, rel24 = A24.grid_maps()
+    d24 = A24.desc
+    smat.lib().preprocessing_mmaSTKernel(d24.values, meta.data_ptr(), sp.data_ptr(), M, K, K, d24.nnzb,
+                                         info24.data_ptr(), rel24.data_ptr())
+    C = torch.full((M, N), float("nan"), dtype=torch.float16, device="cuda")
+    smat.lib().mmaOBTSKernel(d24.values, d24.rowptr, d24.colidx, meta.data_ptr(), sp.data_ptr(), Bt.data_ptr(),
+                             C.data_ptr(), M, N, K, d24.nnzb, info24.data_ptr(), rel24.data_ptr())
+    torch.cuda.synchronize()
+    assert smat.lib().smat_last_status() == 0, smat.lib().smat_last_error()
+    Cn = C.cpu().numpy().astype(np.float64)
+    assert np.linalg.norm(Cn - R) / np.linalg.norm(R) < 1e-3

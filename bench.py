@@ -47,7 +47,21 @@ MFMA_PEAK_TFLOPS = {"f16": 2500.0, "bf16": 2500.0, "fp8": 5000.0}
 # schedule: (workload, scale, N, bk, n_tile, order) -> (bytes, profile file,
 # id of the kernel source the profile was taken with).  A figure is reported
 # only while the kernel source is the one it was measured on.
-MEASURED_TRAFFIC = {}
+MEASURED_TRAFFIC = {
+    ("rmat", 18, 256, 2, "auto", "locality"): (1930e6, "profiles/spmm_lds_bk2_locality_rmat18_n256_hbm_traffic.md",
+                                                 "be11342f66f3"),
+}
+# the same for the other workloads' calls:
+# (workload, N, bk, n_tile, order, kernel, dtype) -> (bytes, profile file, kernel source id)
+CONFIG_TRAFFIC = {
+    ("powerlaw", 128, 1, 64, "locality", "dense", "f16"): (2476e6, "profiles/spmm_lds_bk1_locality_powerlaw_n128.md",
+                                                           "be11342f66f3"),
+    ("clustered", 256, 16, "auto", "none", "mixed24", "f16"): (163.7e6,
+                                                              "profiles/spmm_mixed_vs_dense_clustered24_n256.md",
+                                                              "be11342f66f3"),
+    ("clustered", 256, 16, "auto", "none", "dense", "f16"): (166.1e6, "profiles/spmm_mixed_vs_dense_clustered24_n256.md",
+                                                            "be11342f66f3"),
+}
 KERNEL_SOURCE = os.path.join(ROOT, "sptc-smat_amd", "csrc", "spmm.hip")
 
 
@@ -576,6 +590,7 @@ def build_ctx(args, workload, N, dtype, log, data=None):
         v = gen.f32_to_bf16_bits(v.view(np.float16).astype(np.float32))
     ctx = Ctx()
     ctx.args, ctx.M, ctx.K, ctx.N, ctx.nnz, ctx.dtype = args, M, K, N, int(ci.size), dtype
+    ctx.workload = workload
     ctx.rp, ctx.ci, ctx.v, ctx.v32, ctx.desc, ctx.data = rp, ci, v, v32, desc, data
     log(f"{desc}: {M} rows, nnz {ctx.nnz}, N {N}, {dtype}, ready in {time.time() - t0:.1f}s")
     dev = torch.device("cuda:0")
@@ -623,11 +638,18 @@ def measure_config(ctx, bk, nt, kernel, steps, warmup, log, order=None, order_na
     return st
 
 
-def compact(st):
-    """The figures of one timed configuration that the JSON line keeps."""
+def compact(st, ctx=None):
+    """The figures of one timed configuration that the JSON line keeps, with
+    the bytes a committed rocprof profile measured for the same call (while
+    the kernel source is the one profiled)."""
     out = {k: st[k] for k in ("bk", "n_tile", "kernel", "order", "nnzb", "event_ms", "useful_gflops", "alg_bytes",
                               "achieved_gbs", "rel_fro_err") if k in st}
     out["roofline_frac"] = round(st["achieved_gbs"] / HBM_PEAK_GBS, 4)
+    if ctx is not None:
+        t = CONFIG_TRAFFIC.get((ctx.workload, ctx.N, st["bk"], st["n_tile"], st["order"], st["kernel"], ctx.dtype))
+        if t and t[2] == kernel_source_id():
+            out["traffic"], out["traffic_source"] = t[0], t[1]
+            out["traffic_ratio"] = round(t[0] / st["alg_bytes"], 2)
     return out
 
 
@@ -646,7 +668,8 @@ def learned_config(ctx, log, steps, warmup, locality=True):
         if order is not None:
             tau, bk, nt, kernel = probe["config"]
     tuned = measure_config(ctx, bk, nt, kernel, steps, warmup, log, order)
-    return {"pick": [tau, bk, nt, kernel], "locality_probe": probe, "tuned": compact(tuned), "plain": compact(plain),
+    return {"pick": [tau, bk, nt, kernel], "locality_probe": probe, "tuned": compact(tuned, ctx),
+            "plain": compact(plain, ctx),
             "speedup_vs_plain": round(tuned["useful_gflops"] / plain["useful_gflops"], 3),
             "_order": order, "_tuned": tuned}
 
@@ -802,8 +825,8 @@ def config_legs(args, log, head_ctx, head_cfg):
             rr = learned_config(ctx, log, steps, warmup)
             sm = measure_config(ctx, 16, 0, "mixed24", steps, warmup, log)
             dn = measure_config(ctx, 16, 0, "dense", steps, warmup, log)
-            rr["smfmac_24"] = compact(sm)
-            rr["dense_mfma_16"] = compact(dn)
+            rr["smfmac_24"] = compact(sm, ctx)
+            rr["dense_mfma_16"] = compact(dn, ctx)
             rr["workload"] = ctx.desc + f", N 256, {dt}"
             r[dt] = rr
             del ctx

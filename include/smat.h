@@ -303,6 +303,74 @@ smat_status smat_compress_24(const smat_bcsr_desc* A, int32_t layout, uint16_t* 
                              void* metadata, smat_stream stream);
 
 /* ------------------------------------------------------------------------ */
+/* Merged tiles: 64 x 4bk tiles of 4 x 4 sub-blocks of 16 x bk              */
+/* ------------------------------------------------------------------------ */
+/*
+ * Replaces SparseMatrix::bcsrBlocking (src/cuda_hgemm/src/common/matrix.h:
+ * 794-1137, BLOCK = 4; bk = 16 gives the 64 x 64 tiles of mmaOBT_tiled.cu,
+ * bk = 32 the 64 x 128 tiles of mmaOBTS_tiled_large.cu).  As in the
+ * reference, sub-block sb = 4*sr + sc of tile t (sr its 16-row band, sc its
+ * bk-column band) is stored 16 x bk row-major at (16*t + sb)*16*bk, colidx
+ * holds the tile's first column, and info holds one entry per sub-block
+ * (0 empty, 1 every aligned 1x4 group of every row has <= 2 nonzeros, 2
+ * otherwise).  Unlike bcsrBlocking, tiles are sorted by (tile row, tile
+ * column), rowptr has an entry for every tile row, and info is computed
+ * from each sub-block's own values (matrix.h:1042-1050 reads other
+ * sub-blocks' rows).
+ */
+typedef struct smat_tiles_t* smat_tiles;
+
+typedef struct {
+  int64_t M;        /* rows of A (a multiple of 16) */
+  int64_t K;        /* columns of A (a multiple of bk) */
+  int32_t bk;       /* sub-block width: 16 or 32 */
+  int32_t tw;       /* tile width, 4 * bk */
+  int64_t ntrows;   /* tile rows, ceil(M / 64) */
+  int64_t ntcols;   /* tile columns, ceil(K / tw) */
+  int64_t ntiles;   /* stored tiles */
+  int32_t* rowptr;  /* [ntrows + 1] */
+  int32_t* colidx;  /* [ntiles] first column of the tile, a multiple of tw */
+  uint16_t* values; /* [ntiles * 16 * 16 * bk] FP16 */
+  int32_t* info;    /* [ntiles * 16] sub-block types */
+} smat_tiles_desc;
+
+/* Groups the blocks of a BCSR with 16 x bk blocks (bk 16 or 32) into tiles
+ * on the GPU (sort of block keys by tile, O(nnzb log nnzb)). */
+smat_status smat_tiles_from_bcsr(const smat_bcsr_desc* A, smat_stream stream, smat_tiles* out);
+smat_status smat_tiles_describe(smat_tiles T, smat_tiles_desc* out);
+smat_status smat_tiles_destroy(smat_tiles T);
+
+/* mergedBlockInfo / mergedRelativeBlockIndexMapping (matrix.h:919-927,
+ * 1083-1089): [ntrows * ntcols] int32 each, caller-allocated: 1 / the stored
+ * tile index where a tile exists, 0 / -1 elsewhere. */
+smat_status smat_tiles_grid_maps(const smat_tiles_desc* T, int32_t* tile_info_grid,
+                                 int32_t* relative_tile_index, smat_stream stream);
+
+/* Compresses every info-1 sub-block into the SMAT_24_SMFMAC layout, indexed
+ * by sub-block (16*t + sb): values24 [ntiles*16 * 16 * bk/2] halves,
+ * index24 [ntiles*16 * 16 * bk/16] uint16. */
+smat_status smat_tiles_compress_24(const smat_tiles_desc* T, uint16_t* values24, uint16_t* index24,
+                                   smat_stream stream);
+
+typedef struct {
+  int32_t dtype;             /* SMAT_F16 or SMAT_BF16: element type of values, values24 and B */
+  int32_t out_dtype;         /* SMAT_F16 or SMAT_F32 */
+  const void* values;        /* NULL: the tiles' own FP16 values; else [ntiles*16*16*bk] in dtype */
+  const uint16_t* values24;  /* with index24: info-1 sub-blocks on SMFMAC (smat_tiles_compress_24 of */
+  const uint16_t* index24;   /* the same values); both NULL: every sub-block on dense MFMA */
+  const void* b;             /* row-major B: K rows of ldb elements, 16-byte aligned */
+  int64_t ldb;               /* multiple of 8, >= N */
+  int64_t N;
+  void* c;                   /* row-major C: M rows of ldc elements, every row written */
+  int64_t ldc;
+} smat_tiles_operands;
+
+/* C = A * B over the tiles, FP32 accumulation.  Empty sub-blocks are
+ * skipped; with values24/index24 a 64-deep window of a sub-row whose
+ * sub-blocks are all empty or 2:4 runs on SMFMAC, otherwise on MFMA. */
+smat_status smat_tiles_spmm(const smat_tiles_desc* T, const smat_tiles_operands* ops, smat_stream stream);
+
+/* ------------------------------------------------------------------------ */
 /* SpMM: C = A * B, A BCSR (FP16), B dense K x N, FP32 accumulation         */
 /* ------------------------------------------------------------------------ */
 #define SMAT_KERNEL_AUTO 0
@@ -635,6 +703,56 @@ void mmaOBTSKernel_large_separate(uint16_t* bcsrValuesA_sparse, int* bcsrRowPtrA
 void preprocessing_mmaSTKernel_large(uint16_t* bcsrValuesA, char* metadata, uint16_t* sparseMatrixA,
                                      size_t M, size_t N, size_t K, size_t nonzeroBlocks,
                                      int* blockInfo, int* relativeBlockIndexMapping);
+
+/*
+ * HGEMM_FUNC_SPARSE2 (main.cu:25-28) on merged 64 x 64 tiles — replaces
+ * mmaOBTKernel_tiled (src/cuda_hgemm/src/mma/mmaOBT_tiled.cu:23-218), fed by
+ * tester.h:279-286 with the merged arrays of bcsrBlocking: bcsrValuesA,
+ * bcsrRowPtrA (one entry per tile row), bcsrColIdxA (tile first columns) and
+ * relativeBlockIndexMapping over the tile grid, as written by
+ * smat_tiles_from_bcsr (bk 16) and smat_tiles_grid_maps.  Tile values are
+ * read through relativeBlockIndexMapping like the reference; every
+ * sub-block is multiplied as dense (there is no tileInfo argument).  B[n*K +
+ * k], C row-major M x N FP16.  nonzeroBlocks and blockInfo are not read.
+ * The arrays are checked on the GPU first (rowptr monotone from 0, column
+ * starts inside K, relative indices inside [0, rowptr[last])); a failure is
+ * recorded for smat_last_error() and C is left alone.
+ */
+void mmaOBTKernel_tiled(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdxA, uint16_t* B,
+                        uint16_t* C, size_t M, size_t N, size_t K, size_t nonzeroBlocks,
+                        int* blockInfo, int* relativeBlockIndexMapping);
+
+/*
+ * Replaces preprocessing_mmaOBTSKernel_tiled_large (main.cu:74-77,
+ * mmaOBTS_tiled_large.cu:21-150, 416-429): compresses every tileInfo 1 sub-block of
+ * the merged 64 x 128 tiles in bcsrValuesA into (sparseMatrixA, metadata) in
+ * the SMAT_24_SMFMAC layout of 16 x 32 blocks, indexed by sub-block
+ * (16*t + sb): 256 halves and 64 bytes per sub-block, within the sizes the
+ * reference allocates (tester.h:385-394).  The reference copies dense
+ * sub-blocks into sparseMatrixA as well; mmaOBTSKernel_tiled_large reads
+ * those from bcsrValuesA.  tileInfo is indexed by stored tile, as
+ * bcsrBlocking writes it (matrix.h:1081); the tile count is taken from
+ * relativeBlockIndexMapping (largest index + 1).  blockInfo is not read.
+ */
+void preprocessing_mmaOBTSKernel_tiled_large(uint16_t* bcsrValuesA, char* metadata,
+                                             uint16_t* sparseMatrixA, size_t M, size_t N, size_t K,
+                                             size_t nonzeroBlocks, int* blockInfo,
+                                             int* relativeBlockIndexMapping, int* tileInfo);
+
+/*
+ * HGEMM_FUNC_SPARSE24_2_tiled (main.cu:29-33) — replaces
+ * mmaOBTSKernel_tiled_large (src/cuda_hgemm/src/mma/mmaOBTS_tiled_large.cu:
+ * 152-449): mixed SpMM over merged 64 x 128 tiles (bk 32).  Per 16-row
+ * sub-row of a tile and 64-column window: sub-blocks with tileInfo 2 make
+ * the window dense (MFMA on bcsrValuesA), otherwise its tileInfo 1
+ * sub-blocks run on SMFMAC from the pair of
+ * preprocessing_mmaOBTSKernel_tiled_large; tileInfo 0 sub-blocks are
+ * skipped.  Arrays, layouts and checks as mmaOBTKernel_tiled.
+ */
+void mmaOBTSKernel_tiled_large(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdxA, char* metadata,
+                               uint16_t* sparseMatrixA, uint16_t* B, uint16_t* C, size_t M, size_t N,
+                               size_t K, size_t nonzeroBlocks, int* blockInfo,
+                               int* relativeBlockIndexMapping, int* tileInfo);
 
 #ifdef __cplusplus
 }

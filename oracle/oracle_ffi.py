@@ -30,6 +30,8 @@ _SIG = {
     "or_decompress24_k16": (None, [_i64, _p, _p, _p]),
     "or_spmm_bcsr_f64": (None, [_i64, ctypes.c_int, _p, _p, _p, _i64, _p, _i64, _p, _i64]),
     "or_spmm_csr_f64": (None, [_i64, _p, _p, _p, _i64, _p, _i64, _p, _i64]),
+    "or_bcsr_to_tiles": (_i64, [_i64, _i64, ctypes.c_int, _p, _p, _p, _p, _p, _p, _p, _i64]),
+    "or_spmm_tiles_f64": (None, [_i64, _i64, ctypes.c_int, _p, _p, _p, _i64, _p, _i64, _p, _i64]),
     "or_spmm_refsim_fp16acc": (None, [_i64, ctypes.c_int, _p, _p, _p, _i64, _p, _i64, _p, _i64]),
     "or_csr_multiply_f32": (None, [_i64, _p, _p, _p, _p, _i64, _p]),
     "or_jaccard_distance_group": (_f32, [_p, _i64, _i64, _p, _i64, _i64, _i64]),
@@ -166,6 +168,32 @@ def spmm_bcsr_f64(bk, rowptr, colidx, values, B_u16):
     lib().or_spmm_bcsr_f64(nbrows, bk, P(np.ascontiguousarray(rowptr, np.int32)),
                            P(np.ascontiguousarray(colidx, np.int32)), P(np.ascontiguousarray(values, np.uint16)),
                            N, P(B_u16), N, P(C), N)
+    return C
+
+
+def bcsr_to_tiles(M, K, bk, rowptr, colidx, values):
+    """Merged 64 x 4bk tiles of a BCSR: (rowptr, colidx, values[u16], info)."""
+    rowptr = np.ascontiguousarray(rowptr, np.int32)
+    colidx = np.ascontiguousarray(colidx, np.int32)
+    values = np.ascontiguousarray(values, np.uint16)
+    cap = max(int(colidx.size), 1)
+    ntr = (M + 63) // 64
+    o_rp = np.zeros(ntr + 1, np.int32)
+    o_ci = np.zeros(cap, np.int32)
+    o_v = np.zeros(cap * 256 * bk, np.uint16)
+    o_i = np.zeros(cap * 16, np.int32)
+    nt = lib().or_bcsr_to_tiles(M, K, bk, P(rowptr), P(colidx), P(values), P(o_rp), P(o_ci), P(o_v), P(o_i), cap)
+    assert nt >= 0
+    return o_rp, o_ci[:nt].copy(), o_v[: nt * 256 * bk].copy(), o_i[: nt * 16].copy()
+
+
+def spmm_tiles_f64(M, K, bk, rowptr, colidx, values, B_u16):
+    Kb, N = B_u16.shape
+    C = np.zeros((M, N), np.float64)
+    B_u16 = np.ascontiguousarray(B_u16, np.uint16)
+    lib().or_spmm_tiles_f64(M, K, bk, P(np.ascontiguousarray(rowptr, np.int32)),
+                            P(np.ascontiguousarray(colidx, np.int32)), P(np.ascontiguousarray(values, np.uint16)),
+                            N, P(B_u16), N, P(C), N)
     return C
 
 

@@ -22,6 +22,12 @@
 //   or_csr_to_bcsr             invariants (every nnz at its offset, value sums,
 //                              monotone rowptr), hand cases of the tag rule
 //   or_filter_blocks           partition invariants vs or_csr_to_bcsr
+//   or_bcsr_to_tiles           dense expansion equals the BCSR's and the CSR's;
+//                              tile set equals the distinct (row/64, col/4bk)
+//                              cells of the nonzeros; slot info against a
+//                              dense-matrix classification; hand cases (-0,
+//                              a 3-of-4 group, an explicitly stored zero block)
+//   or_spmm_tiles_f64          dense brute-force GEMM; equals or_spmm_csr_f64
 //   or_compress24_k16          round trip on exact-2:4 blocks, hand-built bytes
 //   or_spmm_*_f64              dense brute-force GEMM, all-ones closed form,
 //                              identity / permutation / linearity properties
@@ -336,6 +342,97 @@ void or_spmm_csr_f64(int64_t M, const int32_t* rowptr, const int32_t* colidx,
       for (int64_t n = 0; n < N; n++) crow[n] += av * (double)h2f(brow[n]);
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Merged tiles.  Restates SparseMatrix::bcsrBlocking
+// (src/cuda_hgemm/src/common/matrix.h:794-1137, BLOCK = 4): the 16 x bk blocks
+// of a BCSR are grouped into tiles of 4 x 4 blocks (64 rows x 4*bk columns);
+// a block at block row br and column start c joins the tile at tile row br/4,
+// first column c - c % (4*bk) (matrix.h:836-837), in slot sb = 4*(br % 4) +
+// (c % (4*bk)) / bk, whose 16 x bk values sit at (16*t + sb)*16*bk
+// (matrix.h:848-852: offset 4*bk*py + 16*px for the block's row / column
+// offsets py = 16*(br % 4), px = bk*((c % 4bk)/bk) inside the tile).  Tile
+// info per slot: 0 all zero, 1 every aligned 1x4 group of every row holds at
+// most two nonzeros, else 2 (matrix.h:1031-1079; the half comparison with 0
+// counts -0 as zero).  Restated with the library's conventions where the
+// reference's producer and consumers disagree: tiles in (tile row, tile
+// column) order — the numbering of the reference's relative index map,
+// matrix.h:919-927 — rowptr over every one of ceil(M/64) tile rows, and the
+// info of a slot from the slot's own values.  Outputs t_rowptr[ceil(M/64)+1],
+// t_colidx[ntiles], t_val[ntiles*16*16*bk], t_info[ntiles*16].  Returns
+// ntiles, or -1 if cap (tiles) is too small.
+// ---------------------------------------------------------------------------
+int64_t or_bcsr_to_tiles(int64_t M, int64_t K, int bk, const int32_t* rowptr,
+                         const int32_t* colidx, const uint16_t* val,
+                         int32_t* t_rowptr, int32_t* t_colidx, uint16_t* t_val,
+                         int32_t* t_info, int64_t cap) {
+  const int64_t tw = 4 * (int64_t)bk, nbrows = M / 16, ntrows = (M + 63) / 64;
+  const int64_t bsz = 16 * (int64_t)bk;
+  (void)K;
+  // tile columns of every tile row
+  std::vector<std::set<int64_t>> cols((size_t)ntrows);
+  for (int64_t br = 0; br < nbrows; br++)
+    for (int64_t p = rowptr[br]; p < rowptr[br + 1]; p++) cols[(size_t)(br / 4)].insert(colidx[p] / tw);
+  int64_t ntiles = 0;
+  for (const auto& s : cols) ntiles += (int64_t)s.size();
+  if (ntiles > cap) return -1;
+  std::map<std::pair<int64_t, int64_t>, int64_t> id;
+  int64_t t = 0;
+  for (int64_t tr = 0; tr < ntrows; tr++) {
+    t_rowptr[tr] = (int32_t)t;
+    for (int64_t tc : cols[(size_t)tr]) {
+      id[{tr, tc}] = t;
+      t_colidx[t] = (int32_t)(tc * tw);
+      t++;
+    }
+  }
+  t_rowptr[ntrows] = (int32_t)t;
+  std::memset(t_val, 0, sizeof(uint16_t) * (size_t)(ntiles * 16 * bsz));
+  for (int64_t br = 0; br < nbrows; br++)
+    for (int64_t p = rowptr[br]; p < rowptr[br + 1]; p++) {
+      const int64_t c = colidx[p];
+      const int64_t tile = id[{br / 4, c / tw}];
+      const int64_t sb = 4 * (br % 4) + (c % tw) / bk;
+      std::memcpy(t_val + (tile * 16 + sb) * bsz, val + p * bsz, sizeof(uint16_t) * (size_t)bsz);
+    }
+  for (int64_t s = 0; s < ntiles * 16; s++) {
+    const uint16_t* v = t_val + s * bsz;
+    bool any = false, dense = false;
+    for (int64_t g = 0; g < bsz / 4; g++) {
+      int n = 0;
+      for (int i = 0; i < 4; i++) n += (v[4 * g + i] & 0x7fff) != 0;
+      any = any || n > 0;
+      dense = dense || n > 2;
+    }
+    t_info[s] = !any ? 0 : dense ? 2 : 1;
+  }
+  return ntiles;
+}
+
+// FP64 SpMM over merged tiles: C = A * B, A as or_bcsr_to_tiles writes it, B
+// row-major K x N (FP16 bits); rows >= M of a last partial tile row and
+// columns >= K are not touched.
+void or_spmm_tiles_f64(int64_t M, int64_t K, int bk, const int32_t* t_rowptr,
+                       const int32_t* t_colidx, const uint16_t* t_val, int64_t N,
+                       const uint16_t* B, int64_t ldb, double* C, int64_t ldc) {
+  const int64_t ntrows = (M + 63) / 64, bsz = 16 * (int64_t)bk;
+  for (int64_t i = 0; i < M; i++)
+    for (int64_t n = 0; n < N; n++) C[i * ldc + n] = 0.0;
+  for (int64_t tr = 0; tr < ntrows; tr++)
+    for (int64_t t = t_rowptr[tr]; t < t_rowptr[tr + 1]; t++)
+      for (int sb = 0; sb < 16; sb++)
+        for (int r = 0; r < 16; r++) {
+          const int64_t row = tr * 64 + 16 * (sb / 4) + r;
+          if (row >= M) continue;
+          for (int c = 0; c < bk; c++) {
+            const uint16_t a = t_val[(t * 16 + sb) * bsz + r * bk + c];
+            const int64_t k = t_colidx[t] + (int64_t)bk * (sb % 4) + c;
+            if ((a & 0x7fff) == 0 || k >= K) continue;
+            const double av = h2f(a);
+            for (int64_t n = 0; n < N; n++) C[row * ldc + n] += av * (double)h2f(B[k * ldb + n]);
+          }
+        }
 }
 
 // Emulation of the reference plain kernel's arithmetic

@@ -16,6 +16,7 @@
 //    the scratch) they run k_ref_spmm: one wave per (block row, 64-column
 //    tile), blocks read through relativeBlockIndexMapping and blockInfo like
 //    the reference kernels (mmaOBT.cu:56-61).
+#include <algorithm>
 #include <mutex>
 
 #include "smat_internal.h"
@@ -23,6 +24,10 @@
 namespace smat {
 smat_status compress_ref16(int64_t nnzb, const uint16_t* val, const int32_t* tags, uint16_t* sp,
                            uint8_t* meta, hipStream_t s, bool* overflowed);
+smat_status tiles_spmm_rel(int bk, bool mixed, int64_t M, int64_t K, int64_t N, const int32_t* rowptr,
+                           const int32_t* colidx, const int32_t* relmap, const uint16_t* values,
+                           const int32_t* info, const uint16_t* values24, const uint16_t* index24,
+                           const uint16_t* B, int64_t ldb, uint16_t* C, int64_t ldc, hipStream_t s);
 }
 
 namespace {
@@ -221,10 +226,10 @@ __global__ void k_check_planned(int64_t nbrows, int64_t nnzb, int64_t K, const i
 }
 
 // B in the reference's layout (element (k, n) at n*K + k) -> row-major K x N
-// (k*N + n), 64 x 64 tiles through LDS: 16-byte reads along k, 2-byte
+// with row pitch ldo (k*ldo + n), 64 x 64 tiles through LDS: 16-byte reads along k, 2-byte
 // writes along n coalesced per wave row.
 __global__ __launch_bounds__(256) void k_transpose_b(const uint16_t* __restrict__ bt, uint16_t* __restrict__ b,
-                                                     int64_t K, int64_t N) {
+                                                     int64_t K, int64_t N, int64_t ldo) {
   __shared__ uint16_t tile[64][65];
   const int64_t k0 = (int64_t)blockIdx.x * 64, n0 = (int64_t)blockIdx.y * 64;
   const int t = threadIdx.x;
@@ -245,7 +250,7 @@ __global__ __launch_bounds__(256) void k_transpose_b(const uint16_t* __restrict_
   for (int i = t; i < 64 * 64; i += 256) {
     const int kk = i / 64, nn = i % 64;
     const int64_t k = k0 + kk, n = n0 + nn;
-    if (k < K && n < N) b[k * N + n] = tile[kk][nn];
+    if (k < K && n < N) b[k * ldo + n] = tile[kk][nn];
   }
 }
 
@@ -326,7 +331,7 @@ bool planned_spmm(const uint16_t* vals, const int* rowptr, const int* colidx, co
   uint16_t* Brm = (reinterpret_cast<uintptr_t>(B) & 15) == 0 ? staged_b(K * N * sizeof(uint16_t)) : nullptr;
   if (Brm) {
     hipLaunchKernelGGL(k_transpose_b, dim3((unsigned)((K + 63) / 64), (unsigned)((N + 63) / 64)), 256, 0, 0, B, Brm,
-                       (int64_t)K, (int64_t)N);
+                       (int64_t)K, (int64_t)N, (int64_t)N);
     o.b_layout = SMAT_B_ROW_MAJOR;
     ops.b = Brm;
     ops.ldb = (int64_t)N;
@@ -340,6 +345,103 @@ bool planned_spmm(const uint16_t* vals, const int* rowptr, const int* colidx, co
     return false;
   }
   return true;
+}
+
+// Array check of the tiled launchers, one thread per row pointer and per
+// tile position: rowptr starts at 0 and never decreases, every column start
+// is a multiple of tw inside K, and the relative index of every tile's grid
+// cell lies in [0, ntiles), ntiles = rowptr[ntrows].  Any mismatch sets *flag.
+__global__ void k_check_tiled(int64_t ntrows, int64_t ntiles, int64_t ntc, int tw, int64_t K,
+                              const int* __restrict__ rowptr, const int* __restrict__ colidx,
+                              const int* __restrict__ relMap, int* __restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0 && rowptr[0] != 0) atomicOr(flag, 1);
+  if (i < ntrows && rowptr[i] > rowptr[i + 1]) atomicOr(flag, 1);
+  if (i >= ntiles) return;
+  int64_t lo = 0, hi = ntrows;  // tile row of position i
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (rowptr[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  const int col = colidx[i];
+  if (col < 0 || col % tw != 0 || col >= K) {
+    atomicOr(flag, 1);
+    return;
+  }
+  const int r = relMap[lo * ntc + col / tw];
+  if (r < 0 || r >= ntiles) atomicOr(flag, 1);
+}
+
+// Largest entry of a relative index map (-1 when every cell is empty).
+__global__ __launch_bounds__(256) void k_max_rel(int64_t n, const int* __restrict__ rel, int* __restrict__ out) {
+  int m = -1;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    m = max(m, rel[i]);
+  for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+__global__ void k_set_flag(int* flag, int v) {
+  if (threadIdx.x == 0) *flag = v;
+}
+
+// The tiled SpMM launchers: checks the arrays (two host syncs), stages B
+// row-major and runs the tile kernel through relativeBlockIndexMapping.
+void tiled_spmm(const char* who, int bk, bool mixed, const uint16_t* vals, const int* rowptr, const int* colidx,
+                const int* relMap, const int* tileInfo, const uint8_t* meta, const uint16_t* sparseA,
+                const uint16_t* B, uint16_t* C, size_t M, size_t N, size_t K) {
+  const int tw = 4 * bk;
+  if (!vals || !rowptr || !colidx || !relMap || !B || !C || (mixed && (!tileInfo || !meta || !sparseA))) {
+    smat::set_error(SMAT_ERR_INVALID, "%s: null pointer", who);
+    return;
+  }
+  if (M % 16 || K % bk || N == 0 || (reinterpret_cast<uintptr_t>(B) & 15) != 0) {
+    smat::set_error(SMAT_ERR_INVALID, "%s: M must be a multiple of 16, K of %d, N > 0 and B 16-byte aligned "
+                    "(M=%zu N=%zu K=%zu)", who, bk, M, N, K);
+    return;
+  }
+  const int64_t ntrows = ((int64_t)M + 63) / 64, ntc = ((int64_t)K + tw - 1) / tw;
+  int ntiles = -1;
+  if (hipMemcpy(&ntiles, rowptr + ntrows, 4, hipMemcpyDeviceToHost) != hipSuccess) {
+    smat::set_error(SMAT_ERR_HIP, "%s: reading bcsrRowPtrA failed", who);
+    return;
+  }
+  smat::DevBuf flag;
+  if (flag.alloc(4) != SMAT_OK) return;
+  hipLaunchKernelGGL(k_set_flag, 1, 64, 0, 0, flag.as<int>(), ntiles < 0 ? 1 : 0);
+  const int64_t nchk = std::max<int64_t>(ntrows, ntiles);
+  if (ntiles >= 0)
+    hipLaunchKernelGGL(k_check_tiled, (unsigned)((nchk + 255) / 256), 256, 0, 0, ntrows, (int64_t)ntiles, ntc, tw,
+                       (int64_t)K, rowptr, colidx, relMap, flag.as<int>());
+  int h = 1;
+  if (hipGetLastError() != hipSuccess || hipMemcpy(&h, flag.p, 4, hipMemcpyDeviceToHost) != hipSuccess) {
+    smat::set_error(SMAT_ERR_HIP, "%s: array check failed to run", who);
+    return;
+  }
+  if (h != 0) {
+    smat::set_error(SMAT_ERR_INVALID,
+                    "%s: the tile arrays are inconsistent (rowptr must start at 0 and have an entry per tile row, "
+                    "column starts must be multiples of %d inside K, relative indices inside [0, %d))",
+                    who, tw, ntiles);
+    return;
+  }
+  const int64_t ldo = ((int64_t)N + 7) / 8 * 8;
+  uint16_t* Brm = staged_b((size_t)K * (size_t)ldo * sizeof(uint16_t));
+  if (!Brm) {
+    smat::set_error(SMAT_ERR_ALLOC, "%s: staging B failed", who);
+    return;
+  }
+  hipLaunchKernelGGL(k_transpose_b, dim3((unsigned)((K + 63) / 64), (unsigned)((N + 63) / 64)), 256, 0, 0, B, Brm,
+                     (int64_t)K, (int64_t)N, ldo);
+  if (hipGetLastError() != hipSuccess) {
+    smat::set_error(SMAT_ERR_HIP, "%s: B transpose launch failed", who);
+    return;
+  }
+  (void)smat::tiles_spmm_rel(bk, mixed, (int64_t)M, (int64_t)K, (int64_t)N, rowptr, colidx, relMap, vals,
+                             mixed ? tileInfo : nullptr, mixed ? sparseA : nullptr,
+                             mixed ? reinterpret_cast<const uint16_t*>(meta) : nullptr, Brm, ldo, C, (int64_t)N,
+                             nullptr);
 }
 
 bool check_ref_args(const void* a, const void* b, const void* c, size_t M, size_t N, size_t K, const char* who,
@@ -551,6 +653,73 @@ void preprocessing_mmaSTKernel_large(uint16_t* bcsrValuesA, char* metadata, uint
     smat::set_error(SMAT_ERR_NOT_24,
                     "preprocessing_mmaSTKernel_large: a blockInfo==1 block holds a 1x4 group with more than two "
                     "nonzeros; the extra values were dropped");
+}
+
+// Dense SpMM over merged 64 x 64 tiles (mmaOBT_tiled.cu:208-218, called
+// with bcsrBlocking's arrays by tester.h:279-286): every sub-block of a tile
+// is multiplied as dense, tiles found through relativeBlockIndexMapping.
+void mmaOBTKernel_tiled(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdxA, uint16_t* B, uint16_t* C,
+                        size_t M, size_t N, size_t K, size_t nonzeroBlocks, int* blockInfo,
+                        int* relativeBlockIndexMapping) {
+  (void)nonzeroBlocks;
+  (void)blockInfo;
+  tiled_spmm("mmaOBTKernel_tiled", 16, false, bcsrValuesA, bcsrRowPtrA, bcsrColIdxA, relativeBlockIndexMapping,
+             nullptr, nullptr, nullptr, B, C, M, N, K);
+}
+
+// Compressor of the 64 x 128 tiles (mmaOBTS_tiled_large.cu:21-150, declared at
+// main.cu:74-77): the SMFMAC compressor over the tiles' 16 x 32 sub-blocks,
+// tagged by tileInfo (indexed by stored tile).
+void preprocessing_mmaOBTSKernel_tiled_large(uint16_t* bcsrValuesA, char* metadata, uint16_t* sparseMatrixA,
+                                             size_t M, size_t N, size_t K, size_t nonzeroBlocks, int* blockInfo,
+                                             int* relativeBlockIndexMapping, int* tileInfo) {
+  (void)N;
+  (void)nonzeroBlocks;
+  (void)blockInfo;
+  const char* who = "preprocessing_mmaOBTSKernel_tiled_large";
+  if (!bcsrValuesA || !metadata || !sparseMatrixA || !relativeBlockIndexMapping || !tileInfo) {
+    smat::set_error(SMAT_ERR_INVALID, "%s: null pointer", who);
+    return;
+  }
+  if (M % 16 || K % 32) {
+    smat::set_error(SMAT_ERR_INVALID, "%s: M must be a multiple of 16 and K of 32", who);
+    return;
+  }
+  const int64_t ng = (((int64_t)M + 63) / 64) * (((int64_t)K + 127) / 128);
+  smat::DevBuf mx;
+  if (mx.alloc(4) != SMAT_OK) return;
+  hipLaunchKernelGGL(k_set_flag, 1, 64, 0, 0, mx.as<int>(), -1);
+  if (ng > 0)
+    hipLaunchKernelGGL(k_max_rel, (unsigned)std::min<int64_t>((ng + 255) / 256, 4096), 256, 0, 0, ng,
+                       relativeBlockIndexMapping, mx.as<int>());
+  int last = -1;
+  if (hipGetLastError() != hipSuccess || hipMemcpy(&last, mx.p, 4, hipMemcpyDeviceToHost) != hipSuccess) {
+    smat::set_error(SMAT_ERR_HIP, "%s: tile count failed", who);
+    return;
+  }
+  smat_bcsr_desc d{};
+  d.M = 16;
+  d.K = 32;
+  d.bm = 16;
+  d.bk = 32;
+  d.nnzb = (int64_t)(last + 1) * 16;
+  d.values = bcsrValuesA;
+  d.tags = tileInfo;
+  if (smat_compress_24(&d, SMAT_24_SMFMAC, sparseMatrixA, metadata, nullptr) == SMAT_ERR_NOT_24)
+    smat::set_error(SMAT_ERR_NOT_24,
+                    "%s: a tileInfo==1 sub-block holds a 1x4 group with more than two nonzeros; the extra values "
+                    "were dropped", who);
+}
+
+// Mixed SpMM over merged 64 x 128 tiles (mmaOBTS_tiled_large.cu:431-449).
+void mmaOBTSKernel_tiled_large(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdxA, char* metadata,
+                               uint16_t* sparseMatrixA, uint16_t* B, uint16_t* C, size_t M, size_t N, size_t K,
+                               size_t nonzeroBlocks, int* blockInfo, int* relativeBlockIndexMapping, int* tileInfo) {
+  (void)nonzeroBlocks;
+  (void)blockInfo;
+  tiled_spmm("mmaOBTSKernel_tiled_large", 32, true, bcsrValuesA, bcsrRowPtrA, bcsrColIdxA,
+             relativeBlockIndexMapping, tileInfo, reinterpret_cast<const uint8_t*>(metadata), sparseMatrixA, B, C, M,
+             N, K);
 }
 
 }  // extern "C"

@@ -44,6 +44,20 @@ class BcsrDesc(ctypes.Structure):
     ]
 
 
+class TilesDesc(ctypes.Structure):
+    _fields_ = [
+        ("M", _i64), ("K", _i64), ("bk", _i32), ("tw", _i32), ("ntrows", _i64), ("ntcols", _i64),
+        ("ntiles", _i64), ("rowptr", _p), ("colidx", _p), ("values", _p), ("info", _p),
+    ]
+
+
+class TilesOperands(ctypes.Structure):
+    _fields_ = [
+        ("dtype", _i32), ("out_dtype", _i32), ("values", _p), ("values24", _p), ("index24", _p), ("b", _p),
+        ("ldb", _i64), ("N", _i64), ("c", _p), ("ldc", _i64),
+    ]
+
+
 class SpmmOperands(ctypes.Structure):
     _fields_ = [
         ("a_values", _p), ("a_scale", _p), ("b", _p), ("b_scale", _p), ("ldb", _i64), ("c", _p),
@@ -151,6 +165,15 @@ SIGNATURES = {
     "smat_blocking_info": (_i32, [_i64, _i64, _i64, _p, _p, _p, _i64, _p, _p, _p]),
     "smat_permute_csr_rows": (_i32, [_i64, _p, _p, _p, _p, _p, _p, _p, _p]),
     "smat_locality_order": (_i32, [_i64, _i64, _p, _p, _p, _p]),
+    "smat_tiles_from_bcsr": (_i32, [ctypes.POINTER(BcsrDesc), _p, ctypes.POINTER(_p)]),
+    "smat_tiles_describe": (_i32, [_p, ctypes.POINTER(TilesDesc)]),
+    "smat_tiles_destroy": (_i32, [_p]),
+    "smat_tiles_grid_maps": (_i32, [ctypes.POINTER(TilesDesc), _p, _p, _p]),
+    "smat_tiles_compress_24": (_i32, [ctypes.POINTER(TilesDesc), _p, _p, _p]),
+    "smat_tiles_spmm": (_i32, [ctypes.POINTER(TilesDesc), ctypes.POINTER(TilesOperands), _p]),
+    "mmaOBTKernel_tiled": (None, [_p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
+    "preprocessing_mmaOBTSKernel_tiled_large": (None, [_p, _p, _p, _sz, _sz, _sz, _sz, _p, _p, _p]),
+    "mmaOBTSKernel_tiled_large": (None, [_p, _p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p, _p]),
     "mmaOBTKernel": (None, [_p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
     "mmaOBTSKernel": (None, [_p, _p, _p, _p, _p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
     "preprocessing_mmaSTKernel": (None, [_p, _p, _p, _sz, _sz, _sz, _sz, _p, _p]),
@@ -282,6 +305,80 @@ class Bcsr:
     def close(self):
         if self.handle:
             lib().smat_bcsr_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Tiles:
+    """Merged 64 x 4bk tiles (4 x 4 sub-blocks of 16 x bk, bk 16 or 32) built
+    on the GPU from a Bcsr (SparseMatrix::bcsrBlocking, matrix.h:794-1137)."""
+
+    def __init__(self, A: Bcsr | BcsrDesc, stream=None):
+        desc = A.desc if isinstance(A, Bcsr) else A
+        h = _p()
+        check(lib().smat_tiles_from_bcsr(ctypes.byref(desc), stream_ptr(stream), ctypes.byref(h)))
+        self.handle = h
+        self.desc = TilesDesc()
+        check(lib().smat_tiles_describe(h, ctypes.byref(self.desc)))
+
+    @property
+    def ntiles(self) -> int:
+        return int(self.desc.ntiles)
+
+    def arrays(self):
+        """Copies (rowptr, colidx, values[u16 as int16], info) to torch CPU tensors."""
+        import torch
+        d = self.desc
+        out = []
+        for p, n, dt in ((d.rowptr, d.ntrows + 1, torch.int32), (d.colidx, d.ntiles, torch.int32),
+                         (d.values, d.ntiles * 256 * d.bk, torch.int16), (d.info, d.ntiles * 16, torch.int32)):
+            t = torch.empty(int(n), dtype=dt)
+            if n:
+                torch.cuda.synchronize()
+                copy_to_host(t.data_ptr(), p, int(n) * t.element_size())
+            out.append(t)
+        return out
+
+    def grid_maps(self, stream=None):
+        """(mergedBlockInfo, mergedRelativeBlockIndexMapping) over the tile grid, device int32."""
+        import torch
+        d = self.desc
+        ng = max(int(d.ntrows * d.ntcols), 1)
+        info = torch.empty(ng, dtype=torch.int32, device="cuda")
+        rel = torch.empty(ng, dtype=torch.int32, device="cuda")
+        check(lib().smat_tiles_grid_maps(ctypes.byref(d), ptr(info), ptr(rel), stream_ptr(stream)))
+        return info, rel
+
+    def compress_24(self, stream=None):
+        """(values24, index24) of the info-1 sub-blocks, SMFMAC layout, device tensors."""
+        import torch
+        d = self.desc
+        n = max(int(d.ntiles) * 16, 1)
+        vals = torch.empty(n * 16 * (d.bk // 2), dtype=torch.int16, device="cuda")
+        idx = torch.empty(n * 16 * (d.bk // 16), dtype=torch.int16, device="cuda")
+        check(lib().smat_tiles_compress_24(ctypes.byref(d), ptr(vals), ptr(idx), stream_ptr(stream)))
+        return vals, idx
+
+    def spmm(self, B, C, *, N: int | None = None, dtype: int = SMAT_F16, values=None, values24=None, index24=None,
+             ldb: int | None = None, ldc: int | None = None, stream=None):
+        """C = A * B (B row-major K x N, C row-major M x N, FP16 or FP32 by C's dtype)."""
+        import torch
+        n = int(B.shape[-1]) if N is None else N
+        out = SMAT_F32 if C.dtype == torch.float32 else SMAT_F16
+        ops = TilesOperands(dtype, out, ptr(values), ptr(values24), ptr(index24), ptr(B),
+                            int(B.shape[-1]) if ldb is None else ldb, n, ptr(C),
+                            int(C.shape[-1]) if ldc is None else ldc)
+        check(lib().smat_tiles_spmm(ctypes.byref(self.desc), ctypes.byref(ops), stream_ptr(stream)))
+        return C
+
+    def close(self):
+        if self.handle:
+            lib().smat_tiles_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -63,12 +63,27 @@ CONFIG_TRAFFIC = {
                                                             "be11342f66f3"),
 }
 KERNEL_SOURCE = os.path.join(ROOT, "sptc-smat_amd", "csrc", "spmm.hip")
+TILES_SOURCE = os.path.join(ROOT, "sptc-smat_amd", "csrc", "tiles.hip")
+# HBM bytes per call of the merged-tile kernel (FETCH_SIZE x2 + WRITE_SIZE),
+# keyed (workload, N, bk, kernel, dtype), valid while tiles.hip is unchanged
+_TILE_PROFILE = "profiles/spmm_tiles_tiled4000_clustered24_n256.md"
+TILE_TRAFFIC = {
+    ("tiled", 256, 16, "dense", "f16"): (176.3e6, _TILE_PROFILE, "0fd5ffd15c1b"),
+    ("tiled", 256, 16, "mixed24", "f16"): (176.3e6, _TILE_PROFILE, "0fd5ffd15c1b"),
+    ("tiled", 256, 32, "dense", "f16"): (176.2e6, _TILE_PROFILE, "0fd5ffd15c1b"),
+    ("tiled", 256, 32, "mixed24", "f16"): (176.3e6, _TILE_PROFILE, "0fd5ffd15c1b"),
+    ("clustered", 256, 16, "dense", "f16"): (166.4e6, _TILE_PROFILE, "0fd5ffd15c1b"),
+    ("clustered", 256, 16, "mixed24", "f16"): (166.3e6, _TILE_PROFILE, "0fd5ffd15c1b"),
+    ("clustered", 256, 32, "dense", "f16"): (260.7e6, _TILE_PROFILE, "0fd5ffd15c1b"),
+    ("clustered", 256, 32, "mixed24", "f16"): (260.5e6, _TILE_PROFILE, "0fd5ffd15c1b"),
+}
 
 
-def kernel_source_id() -> str:
-    """First 12 hex digits of the SHA-256 of the SpMM kernel source."""
+def kernel_source_id(path: str = KERNEL_SOURCE) -> str:
+    """First 12 hex digits of the SHA-256 of a kernel source file (the SpMM
+    kernels by default)."""
     import hashlib
-    with open(KERNEL_SOURCE, "rb") as f:
+    with open(path, "rb") as f:
         return hashlib.sha256(f.read()).hexdigest()[:12]
 
 
@@ -77,7 +92,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="rmat", choices=["rmat", "band", "clustered", "powerlaw", "batch64"])
+    ap.add_argument("--workload", default="rmat", choices=["rmat", "band", "clustered", "tiled", "powerlaw", "batch64"])
     ap.add_argument("--scale", type=int, default=18, help="R-MAT scale")
     ap.add_argument("--n", type=int, default=256, help="columns of B")
     ap.add_argument("--bk", type=int, default=16, help="block width of a single (non-sweep) run")
@@ -102,6 +117,8 @@ def parse():
                     help="batch64: time only the learned picks (no sweeps, no plain / sweep-best legs)")
     ap.add_argument("--pre-scale", type=int, default=14, help="R-MAT scale of the reorder comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline duration")
+    ap.add_argument("--tiles", action="store_true",
+                    help="only time the merged-tile SpMM (64x64 / 64x128 tiles) on --workload and exit")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run a device copy of this many MiB (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -212,6 +229,9 @@ def _make16(args):
     if args.workload == "clustered":
         return gen.block_clustered(n=65536, tiles=12000, seed=1003), (
             "block-clustered n=65536, 12000 16x16 tiles, every 1x4 group pruned to 2 of 4")
+    if args.workload == "tiled":
+        return gen.block_clustered(n=32768, tiles=4000, seed=1005, prune=False, bh=64, bw=64), (
+            "tile-clustered n=32768, 4000 dense 64x64 tiles")
     raise ValueError(args.workload)
 
 
@@ -290,6 +310,90 @@ class Run:
     def close(self):
         self.plan.close()
         self.A.close()
+
+
+class TileRun:
+    """Merged 64 x 4bk tiles of a workload (smat.Tiles, built from a 16 x bk
+    BCSR): build + steps of the tile SpMM, dense MFMA or mixed MFMA/SMFMAC."""
+
+    def __init__(self, ctx, bk, mixed):
+        import torch
+        import smat
+        self.bk, self.mixed, self.ctx = bk, mixed, ctx
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        self.A = smat.Bcsr(ctx.M, ctx.K, ctx.d_rp, ctx.d_ci, ctx.d_v, bk=bk)
+        self.T = smat.Tiles(self.A)
+        self.v24 = self.i24 = None
+        if mixed:
+            self.v24, self.i24 = self.T.compress_24()
+        torch.cuda.synchronize()
+        self.build_ms = (time.perf_counter() - t0) * 1e3
+        self.dt = smat.SMAT_BF16 if ctx.dtype == "bf16" else smat.SMAT_F16
+
+    def call(self):
+        self.T.spmm(self.ctx.B, self.ctx.C, dtype=self.dt, values24=self.v24, index24=self.i24)
+
+    time = Run.time
+
+    def stats(self):
+        import numpy as np
+        ctx, bk = self.ctx, self.bk
+        info = self.T.arrays()[3].numpy().reshape(-1, 4, 4)  # tile, sub-row, sub-column
+        nt = info.shape[0]
+        nonempty = info > 0
+        packed = np.zeros_like(nonempty)
+        if self.mixed:  # the kernel's window rule: no dense sub-block and both 32-deep steps needed
+            if bk == 32:
+                for w in (0, 1):
+                    win = info[:, :, 2 * w:2 * w + 2]
+                    on = (win > 0).all(-1) & (win == 1).all(-1)
+                    packed[:, :, 2 * w:2 * w + 2] |= on[..., None] & (win == 1)
+            else:
+                on = nonempty[:, :, :2].any(-1) & nonempty[:, :, 2:].any(-1) & ~(info == 2).any(-1)
+                packed |= on[..., None] & (info == 1)
+        n_packed = int(packed.sum())
+        n_dense = int(nonempty.sum()) - n_packed
+        a_bytes = n_dense * 32 * bk + n_packed * (16 * bk + 2 * bk)
+        alg = a_bytes + 68 * nt + 4 * ((ctx.M + 63) // 64 + 1) + ctx.K * ctx.N * 2 + ctx.M * ctx.N * 4
+        useful = 2.0 * ctx.nnz * ctx.N
+        return {"bk": bk, "tile": f"64x{4 * bk}", "kernel": "mixed24" if self.mixed else "dense",
+                "tiles": nt, "sub_blocks": int(nonempty.sum()), "sub_blocks_on_smfmac": n_packed,
+                "build_ms": round(self.build_ms, 2), "event_ms": round(self.event_ms, 4),
+                "wall_ms": round(self.wall_ms, 4), "useful_gflops": round(useful / (self.wall_ms * 1e-3) / 1e9, 2),
+                "alg_bytes": int(alg), "achieved_gbs": round(alg / (self.event_ms * 1e-3) / 1e9, 1)}
+
+    def close(self):
+        self.T.close()
+        self.A.close()
+
+
+def tiles_leg(ctx, log, steps, warmup):
+    """The merged-tile SpMM (64 x 64 and 64 x 128 tiles, dense and mixed) on
+    one workload, each timed and checked against the FP64 product."""
+    import smat
+    out = []
+    for bk in (16, 32):
+        for mixed in (False, True):
+            r = TileRun(ctx, bk, mixed).time(steps, warmup)
+            err = smat.rel_fro_error(ctx.C, ctx.R_in)
+            st = r.stats()
+            st["rel_fro_err"] = err
+            st["roofline_frac"] = round(st["achieved_gbs"] / HBM_PEAK_GBS, 4)
+            t = TILE_TRAFFIC.get((ctx.workload, ctx.N, bk, st["kernel"], ctx.dtype))
+            if t and t[2] == kernel_source_id(TILES_SOURCE):
+                st["traffic"], st["traffic_source"] = t[0], t[1]
+                st["traffic_ratio"] = round(t[0] / st["alg_bytes"], 2)
+            log(f"tiles {st['tile']} {st['kernel']}: {st['event_ms']:.4f} ms, {st['useful_gflops']:.1f} useful "
+                f"GFLOP/s, {st['achieved_gbs']:.0f} GB/s, rel err {err:.2e}")
+            r.close()
+            if not err < 1e-5:
+                print(json.dumps({"error": f"tile SpMM result check failed (bk={bk}, mixed={mixed}): {err}"}))
+                sys.exit(3)
+            out.append(st)
+    best = min(out, key=lambda x: x["event_ms"])
+    return {"results": out, "best": {k: best[k] for k in ("tile", "kernel", "event_ms", "useful_gflops",
+                                                          "roofline_frac")}}
 
 
 class Ctx:
@@ -808,6 +912,7 @@ def config_legs(args, log, head_ctx, head_cfg):
         r = learned_config(ctx, log, steps, warmup)
         r["workload"] = ctx.desc + ", N 128"
         r["dropin"] = dropin_leg(ctx, log, steps)
+        r["merged_tiles"] = tiles_leg(ctx, log, steps, warmup)
         return r
 
     def c2_64():
@@ -827,9 +932,21 @@ def config_legs(args, log, head_ctx, head_cfg):
             dn = measure_config(ctx, 16, 0, "dense", steps, warmup, log)
             rr["smfmac_24"] = compact(sm, ctx)
             rr["dense_mfma_16"] = compact(dn, ctx)
+            rr["merged_tiles"] = tiles_leg(ctx, log, steps, warmup)
             rr["workload"] = ctx.desc + f", N 256, {dt}"
             r[dt] = rr
             del ctx
+        return r
+
+    def c3_tiles():
+        ctx = build_ctx(args, "tiled", 256, "f16", log)
+        r = learned_config(ctx, log, steps, warmup)
+        r["workload"] = ctx.desc + ", N 256"
+        r["bcsr_16x32"] = compact(measure_config(ctx, 32, 0, "dense", steps, warmup, log), ctx)
+        r["merged_tiles"] = tiles_leg(ctx, log, steps, warmup)
+        best = r["merged_tiles"]["best"]["event_ms"]
+        r["tiles_speedup_vs_plain_bcsr"] = round(r["plain"]["event_ms"] / best, 3)
+        r["tiles_speedup_vs_bcsr_16x32"] = round(r["bcsr_16x32"]["event_ms"] / best, 3)
         return r
 
     def c4():
@@ -856,6 +973,7 @@ def config_legs(args, log, head_ctx, head_cfg):
     leg("band4096_n128", c1)
     leg("rmat18_n64", c2_64)
     leg("clustered24_n256", c3)
+    leg("tiled4000_n256", c3_tiles)
     leg("powerlaw_n128", c4)
     leg("batch64_n128", c5)
     if not args.no_tau_sweep:
@@ -897,6 +1015,10 @@ def main():
                                                               "nnz": r["nnz"]}}))
         return
     ctx = build_ctx(args, args.workload, args.n, args.dtype, log)
+    if args.tiles:
+        print(json.dumps({"workload": f"{ctx.desc}, N {ctx.N}, {args.dtype}",
+                          "merged_tiles": tiles_leg(ctx, log, args.steps, args.warmup)}))
+        return
     M, K, rp, ci, v, v32, Bf, desc = ctx.M, ctx.K, ctx.rp, ctx.ci, ctx.v, ctx.v32, ctx.Bf, ctx.desc
     measure = lambda bk, nt, kernel, steps, warmup, order=None: measure_config(  # noqa: E731
         ctx, bk, nt, kernel, steps, warmup, log, order)

@@ -330,7 +330,7 @@ typedef struct {
   int64_t ntiles;   /* stored tiles */
   int32_t* rowptr;  /* [ntrows + 1] */
   int32_t* colidx;  /* [ntiles] first column of the tile, a multiple of tw */
-  uint16_t* values; /* [ntiles * 16 * 16 * bk] FP16 */
+  uint16_t* values; /* [ntiles * 16 * 16 * bk] the BCSR's 16-bit values */
   int32_t* info;    /* [ntiles * 16] sub-block types */
 } smat_tiles_desc;
 
@@ -355,7 +355,7 @@ smat_status smat_tiles_compress_24(const smat_tiles_desc* T, uint16_t* values24,
 typedef struct {
   int32_t dtype;             /* SMAT_F16 or SMAT_BF16: element type of values, values24 and B */
   int32_t out_dtype;         /* SMAT_F16 or SMAT_F32 */
-  const void* values;        /* NULL: the tiles' own FP16 values; else [ntiles*16*16*bk] in dtype */
+  const void* values;        /* NULL: the tiles' own values, read as dtype; else [ntiles*16*16*bk] */
   const uint16_t* values24;  /* with index24: info-1 sub-blocks on SMFMAC (smat_tiles_compress_24 of */
   const uint16_t* index24;   /* the same values); both NULL: every sub-block on dense MFMA */
   const void* b;             /* row-major B: K rows of ldb elements, 16-byte aligned */
@@ -366,8 +366,9 @@ typedef struct {
 } smat_tiles_operands;
 
 /* C = A * B over the tiles, FP32 accumulation.  Empty sub-blocks are
- * skipped; with values24/index24 a 64-deep window of a sub-row whose
- * sub-blocks are all empty or 2:4 runs on SMFMAC, otherwise on MFMA. */
+ * skipped; with values24/index24 a 64-deep window of a sub-row that holds no
+ * dense sub-block and would need both of its 32-deep MFMA steps runs on one
+ * SMFMAC, every other window on MFMA. */
 smat_status smat_tiles_spmm(const smat_tiles_desc* T, const smat_tiles_operands* ops, smat_stream stream);
 
 /* ------------------------------------------------------------------------ */
@@ -743,11 +744,11 @@ void preprocessing_mmaOBTSKernel_tiled_large(uint16_t* bcsrValuesA, char* metada
  * HGEMM_FUNC_SPARSE24_2_tiled (main.cu:29-33) — replaces
  * mmaOBTSKernel_tiled_large (src/cuda_hgemm/src/mma/mmaOBTS_tiled_large.cu:
  * 152-449): mixed SpMM over merged 64 x 128 tiles (bk 32).  Per 16-row
- * sub-row of a tile and 64-column window: sub-blocks with tileInfo 2 make
- * the window dense (MFMA on bcsrValuesA), otherwise its tileInfo 1
- * sub-blocks run on SMFMAC from the pair of
- * preprocessing_mmaOBTSKernel_tiled_large; tileInfo 0 sub-blocks are
- * skipped.  Arrays, layouts and checks as mmaOBTKernel_tiled.
+ * sub-row of a tile and 64-column window: two tileInfo 1 sub-blocks run on
+ * one SMFMAC from the pair of preprocessing_mmaOBTSKernel_tiled_large; a
+ * window with a tileInfo 2 sub-block, or with one non-empty sub-block, runs
+ * on MFMA from bcsrValuesA; tileInfo 0 sub-blocks are skipped.  Arrays,
+ * layouts and checks as mmaOBTKernel_tiled.
  */
 void mmaOBTSKernel_tiled_large(uint16_t* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdxA, char* metadata,
                                uint16_t* sparseMatrixA, uint16_t* B, uint16_t* C, size_t M, size_t N,

@@ -35,8 +35,8 @@
 // next tile's B rows and A operands are in flight, and one barrier per tile
 // separates the buffers.  Each wave multiplies its four sub-blocks: on MFMA
 // 16x16x32 per 32-deep step (two 16-wide sub-blocks, or one 32-wide), or,
-// when the tile has a 2:4 pair and a 64-deep window of the sub-row holds no
-// dense sub-block, on SMFMAC 16x16x64 per window; empty steps are skipped.
+// when a 64-deep window of the sub-row holds no dense sub-block and needs
+// both of its steps, on one SMFMAC 16x16x64; empty steps are skipped.
 // Which path a wave takes depends on the tile's sub-block types only, so
 // every branch is wave-uniform.  Per tile row, each wave reads the tile
 // columns, tile indices and packed sub-block types of up to 64 tiles with one
@@ -222,6 +222,17 @@ __device__ inline uint32_t pack_info(const int32_t* __restrict__ info, int64_t t
   return w;
 }
 
+// A 64-deep window of one sub-row (sub-block types ws, 2 bits each) runs on
+// SMFMAC when it holds no dense sub-block and both of its 32-deep MFMA steps
+// would be needed; otherwise its non-empty steps run on MFMA (one SMFMAC
+// reads as many B operands as two MFMA steps, so it only pays when it
+// replaces two).
+template <int BK>
+__device__ inline bool window_on_smfmac(uint32_t ws) {
+  if constexpr (BK == 32) return (ws & 0x3u) && (ws & 0xCu) && !(ws & 0xAu);
+  else return (ws & 0x0Fu) && (ws & 0xF0u) && !(ws & 0xAAu);
+}
+
 // Bit sc set when sub-block column sc is non-empty in some sub-row.
 __device__ inline uint32_t column_mask(uint32_t w) {
   const uint32_t nz = (w | (w >> 1)) & 0x55555555u;  // bit 2*sb: sub-block sb non-empty
@@ -230,7 +241,7 @@ __device__ inline uint32_t column_mask(uint32_t w) {
 }
 
 template <int BK, int DT, bool OUT_F16, bool MIXED, bool REL>
-__global__ __launch_bounds__(256) void k_spmm_tiles(TileArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_spmm_tiles(TileArgs a) {
   static_assert(BK == 16 || BK == 32, "sub-block width");
   constexpr int ROWS = 4 * BK;      // panel rows (tile width)
   constexpr int NL = ROWS / 32;     // B rows each thread stages per tile
@@ -314,22 +325,20 @@ __global__ __launch_bounds__(256) void k_spmm_tiles(TileArgs a) {
     } else {
       constexpr int NWIN = BK == 32 ? 2 : 1;  // 64-deep windows per tile
       constexpr int SPW = 64 / BK;            // sub-blocks per window
-      constexpr int STW = 2;                  // 32-deep steps per window
 #pragma unroll
       for (int v = 0; v < NWIN; v++) {
-        const uint32_t ws = (sw >> (2 * SPW * v)) & ((1u << (2 * SPW)) - 1);
-        if (ws & 0xAAu) {  // a dense sub-block: the window runs on MFMA
-          dense_steps(STW * v, STW * v + STW);
-        } else if (ws) {
-          // lane group grp meets window rows 16*grp .. : sub-block SPW*v + grp*BK/64 ... as below
+        if (!window_on_smfmac<BK>((sw >> (2 * SPW * v)) & ((1u << (2 * SPW)) - 1))) {
+          dense_steps(2 * v, 2 * v + 2);
+        } else {
+          // lane group grp meets window rows 16*grp .. 16*grp+15
           const int sc = BK == 32 ? 2 * v + (grp >> 1) : grp;
           const int hf = BK == 32 ? (grp & 1) : 0;  // 16-column half of a 32-wide sub-block
           const int64_t sb = (t * 16 + 4 * wave + sc) * 16 + r16;
           if ((sw >> (2 * sc)) & 3) {
-            av[STW * v] = *reinterpret_cast<const uint4*>(a.values24 + sb * (BK / 2) + 8 * hf);
+            av[2 * v] = *reinterpret_cast<const uint4*>(a.values24 + sb * (BK / 2) + 8 * hf);
             ix[v] = a.index24[sb * (BK / 16) + hf];
           } else {
-            av[STW * v] = zero4;
+            av[2 * v] = zero4;
             ix[v] = 0x4444;  // positions (0, 1) of every group; the values are zero
           }
         }
@@ -376,8 +385,8 @@ __global__ __launch_bounds__(256) void k_spmm_tiles(TileArgs a) {
 #pragma unroll
       for (int v = 0; v < NWIN; v++) {
         const uint32_t ws = (sw >> (2 * SPW * v)) & ((1u << (2 * SPW)) - 1);
-        if (ws & 0xAAu) steps(2 * v, 2 * v + 2);
-        else if (ws) sparse_window(buf, v, av[2 * v], ix[v]);
+        if (!window_on_smfmac<BK>(ws)) steps(2 * v, 2 * v + 2);
+        else sparse_window(buf, v, av[2 * v], ix[v]);
       }
     }
   };
@@ -644,7 +653,6 @@ smat_status smat_tiles_spmm(const smat_tiles_desc* T, const smat_tiles_operands*
   SMAT_REQUIRE(ops->ldc >= ops->N, "smat_tiles_spmm: ldc < N");
   SMAT_REQUIRE((ops->values24 == nullptr) == (ops->index24 == nullptr),
                "smat_tiles_spmm: values24 and index24 go together");
-  SMAT_REQUIRE(ops->dtype == SMAT_F16 || ops->values, "smat_tiles_spmm: BF16 needs BF16 tile values");
   TileArgs a{};
   a.rowptr = T->rowptr;
   a.colidx = T->colidx;

@@ -86,18 +86,20 @@ def prune_2of4(dense: np.ndarray) -> np.ndarray:
     return np.where(mask, g, 0).reshape(dense.shape)
 
 
-def block_clustered(n: int = 65536, tiles: int = 12000, seed: int = 1003, prune: bool = True):
+def block_clustered(n: int = 65536, tiles: int = 12000, seed: int = 1003, prune: bool = True, bh: int = 16,
+                    bw: int = 16):
+    """`tiles` distinct aligned bh x bw blocks of an n x n matrix, U[-1, 1]
+    values, optionally pruned to 2 of every aligned 1x4 group."""
     rng = np.random.default_rng(seed)
-    nb = n // 16
-    flat = rng.choice(nb * nb, size=tiles, replace=False)
-    br, bc = flat // nb, flat % nb
-    vals = rng.uniform(-1.0, 1.0, size=(tiles, 16, 16)).astype(np.float16)
+    nbr, nbc = n // bh, n // bw
+    flat = rng.choice(nbr * nbc, size=tiles, replace=False)
+    br, bc = flat // nbc, flat % nbc
+    vals = rng.uniform(-1.0, 1.0, size=(tiles, bh, bw)).astype(np.float16)
     vals[vals == 0] = np.float16(0.5)
     if prune:
         vals = prune_2of4(vals)
-    r = np.arange(16)
-    rows = (br[:, None, None] * 16 + r[None, :, None]) * np.ones((1, 1, 16), dtype=np.int64)
-    cols = (bc[:, None, None] * 16 + r[None, None, :]) * np.ones((1, 16, 1), dtype=np.int64)
+    rows = (br[:, None, None] * bh + np.arange(bh)[None, :, None]) * np.ones((1, 1, bw), dtype=np.int64)
+    cols = (bc[:, None, None] * bw + np.arange(bw)[None, None, :]) * np.ones((1, bh, 1), dtype=np.int64)
     keep = vals != 0
     return _to_csr(n, n, rows[keep], cols[keep], vals[keep])
 

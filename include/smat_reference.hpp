@@ -401,7 +401,35 @@ class SparseMatrix {
                                                                      numberOfBlocks);
   }
 
+  // matrix.h:794-1137: merged 64 x 4k tiles (BLOCK = 4) of the BCSR on the
+  // GPU (smat_tiles_from_bcsr), the tile-grid maps and one type per 16 x k
+  // sub-block.  The tiles are stored in (tile row, tile column) order with a
+  // row pointer per tile row and types computed per sub-block, the layout
+  // the reference's consumers index (bcsrBlocking itself stores tiles in
+  // order of first appearance); see smat.h.
+  void bcsrBlocking() {
+    release_tiles();
+    smat_ref_detail::check(smat_tiles_from_bcsr(&desc_, nullptr, &tiles_));
+    smat_ref_detail::check(smat_tiles_describe(tiles_, &tdesc_));
+    mergedNonzeroBlocks = (size_t)tdesc_.ntiles;
+    numberOfMergedRegions = (size_t)(tdesc_.ntrows * tdesc_.ntcols);
+    mergedBlockInfo_dev = smat_ref_detail::DeviceArray<int>(numberOfMergedRegions).release();
+    mergedRelativeBlockIndexMapping_dev = smat_ref_detail::DeviceArray<int>(numberOfMergedRegions).release();
+    smat_ref_detail::check(
+        smat_tiles_grid_maps(&tdesc_, mergedBlockInfo_dev, mergedRelativeBlockIndexMapping_dev, nullptr));
+    hip_sync();
+    const size_t tsz = 16 * 16 * (size_t)mMMA_K;
+    mergedBcsrRowPtr_host = smat_ref_detail::to_host(tdesc_.rowptr, (size_t)tdesc_.ntrows + 1);
+    mergedBcsrColIdx_host = smat_ref_detail::to_host(tdesc_.colidx, mergedNonzeroBlocks);
+    mergedBcsrVal_host = reinterpret_cast<half*>(smat_ref_detail::to_host(tdesc_.values, mergedNonzeroBlocks * tsz));
+    mergedTileInfo_host = smat_ref_detail::to_host(tdesc_.info, mergedNonzeroBlocks * 16);
+    mergedBlockInfo_host = smat_ref_detail::to_host(mergedBlockInfo_dev, numberOfMergedRegions);
+    mergedRelativeBlockIndexMapping_host =
+        smat_ref_detail::to_host(mergedRelativeBlockIndexMapping_dev, numberOfMergedRegions);
+  }
+
   ~SparseMatrix() {
+    release_tiles();
     release_filter();
     release_bcsr();
     std::free(csrRowPtr_host);
@@ -447,7 +475,21 @@ class SparseMatrix {
   half* getDenseBcsrValuesHost() { return denseBcsrVal_host; }
   int* getDenseBcsrRowPtrHost() { return denseBcsrRowPtr_host; }
   int* getDenseBcsrColIdxHost() { return denseBcsrColIdx_host; }
+  half* getMergedBcsrValues() { return reinterpret_cast<half*>(tdesc_.values); }
+  int* getMergedBcsrRowPtr() { return tdesc_.rowptr; }
+  int* getMergedBcsrColIdx() { return tdesc_.colidx; }
+  half* getMergedBcsrValuesHost() { return mergedBcsrVal_host; }
+  int* getMergedBcsrRowPtrHost() { return mergedBcsrRowPtr_host; }
+  int* getMergedBcsrColIdxHost() { return mergedBcsrColIdx_host; }
+  int* getMergedRelativeBlockIndexMapping_dev() { return mergedRelativeBlockIndexMapping_dev; }
+  int* getMergedRelativeBlockIndexMapping_host() { return mergedRelativeBlockIndexMapping_host; }
+  int* getMergedBlockInfo_dev() { return mergedBlockInfo_dev; }
+  int* getMergedBlockInfo_host() { return mergedBlockInfo_host; }
+  int* getMergedTileInfo_dev() { return tdesc_.info; }
+  int* getMergedTileInfo_host() { return mergedTileInfo_host; }
+  size_t getMergedNonzeroBlocks() { return mergedNonzeroBlocks; }
   const smat_bcsr_desc& desc() const { return desc_; }  // the native view of the same BCSR
+  const smat_tiles_desc& tiles_desc() const { return tdesc_; }  // and of its merged tiles
 
  private:
   static void hip_sync() { smat_ref_detail::hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
@@ -466,6 +508,23 @@ class SparseMatrix {
     blockInfo_dev = relativeBlockIndexMapping_dev = nullptr;
     bcsrRowPtr_host = bcsrColIdx_host = blockInfo_host = relativeBlockIndexMapping_host = nullptr;
     bcsrVal_host = nullptr;
+  }
+
+  void release_tiles() {
+    if (tiles_) smat_tiles_destroy(tiles_);
+    tiles_ = nullptr;
+    tdesc_ = smat_tiles_desc{};
+    (void)hipFree(mergedBlockInfo_dev);
+    (void)hipFree(mergedRelativeBlockIndexMapping_dev);
+    for (int* p : {mergedBcsrRowPtr_host, mergedBcsrColIdx_host, mergedTileInfo_host, mergedBlockInfo_host,
+                   mergedRelativeBlockIndexMapping_host})
+      std::free(p);
+    std::free(mergedBcsrVal_host);
+    mergedBlockInfo_dev = mergedRelativeBlockIndexMapping_dev = nullptr;
+    mergedBcsrRowPtr_host = mergedBcsrColIdx_host = mergedTileInfo_host = mergedBlockInfo_host = nullptr;
+    mergedRelativeBlockIndexMapping_host = nullptr;
+    mergedBcsrVal_host = nullptr;
+    mergedNonzeroBlocks = numberOfMergedRegions = 0;
   }
 
   void release_filter() {
@@ -509,6 +568,13 @@ class SparseMatrix {
   half *sparseBcsrVal_host = nullptr, *denseBcsrVal_host = nullptr;
   int *sparseBlockInfo_dev = nullptr, *sparseRelativeBlockIndexMapping_dev = nullptr;
   int *sparseBlockInfo_host = nullptr, *sparseRelativeBlockIndexMapping_host = nullptr;
+  smat_tiles tiles_ = nullptr;
+  smat_tiles_desc tdesc_{};
+  size_t mergedNonzeroBlocks = 0, numberOfMergedRegions = 0;
+  int *mergedBlockInfo_dev = nullptr, *mergedRelativeBlockIndexMapping_dev = nullptr;
+  int *mergedBcsrRowPtr_host = nullptr, *mergedBcsrColIdx_host = nullptr, *mergedTileInfo_host = nullptr;
+  int *mergedBlockInfo_host = nullptr, *mergedRelativeBlockIndexMapping_host = nullptr;
+  half* mergedBcsrVal_host = nullptr;
 };
 
 // ---- main.cu:10-77: the launchers with the reference's half* parameters -----
@@ -556,4 +622,31 @@ inline void preprocessing_mmaSTKernel_large(half* bcsrValuesA, char* metadata, h
   preprocessing_mmaSTKernel_large(reinterpret_cast<uint16_t*>(bcsrValuesA), metadata,
                                   reinterpret_cast<uint16_t*>(sparseMatrixA), M, N, K, nonzeroBlocks, blockInfo,
                                   relativeBlockIndexMapping);
+}
+
+inline void mmaOBTKernel_tiled(half* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdxA, half* B, half* C, size_t M,
+                               size_t N, size_t K, size_t nonzeroBlocks, int* blockInfo,
+                               int* relativeBlockIndexMapping) {
+  mmaOBTKernel_tiled(reinterpret_cast<uint16_t*>(bcsrValuesA), bcsrRowPtrA, bcsrColIdxA,
+                     reinterpret_cast<uint16_t*>(B), reinterpret_cast<uint16_t*>(C), M, N, K, nonzeroBlocks,
+                     blockInfo, relativeBlockIndexMapping);
+}
+
+inline void preprocessing_mmaOBTSKernel_tiled_large(half* bcsrValuesA, char* metadata, half* sparseMatrixA,
+                                                    size_t M, size_t N, size_t K, size_t nonzeroBlocks,
+                                                    int* blockInfo, int* relativeBlockIndexMapping,
+                                                    int* tileInfo) {
+  preprocessing_mmaOBTSKernel_tiled_large(reinterpret_cast<uint16_t*>(bcsrValuesA), metadata,
+                                          reinterpret_cast<uint16_t*>(sparseMatrixA), M, N, K, nonzeroBlocks,
+                                          blockInfo, relativeBlockIndexMapping, tileInfo);
+}
+
+inline void mmaOBTSKernel_tiled_large(half* bcsrValuesA, int* bcsrRowPtrA, int* bcsrColIdxA, char* metadata,
+                                      half* sparseMatrixA, half* B, half* C, size_t M, size_t N, size_t K,
+                                      size_t nonzeroBlocks, int* blockInfo, int* relativeBlockIndexMapping,
+                                      int* tileInfo) {
+  mmaOBTSKernel_tiled_large(reinterpret_cast<uint16_t*>(bcsrValuesA), bcsrRowPtrA, bcsrColIdxA, metadata,
+                            reinterpret_cast<uint16_t*>(sparseMatrixA), reinterpret_cast<uint16_t*>(B),
+                            reinterpret_cast<uint16_t*>(C), M, N, K, nonzeroBlocks, blockInfo,
+                            relativeBlockIndexMapping, tileInfo);
 }

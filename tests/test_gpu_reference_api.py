@@ -7,6 +7,10 @@ code calls it; what it writes is checked here:
     filtered 2:4 / dense lists against the oracle's csrToBcsr /
     filterBcsrBlocks restatement on the same padded CSR;
   * mmaOBTKernel on its arrays: the FP64 oracle product (FP16 C);
+  * bcsrBlocking(): the merged tiles, tile types and tile-grid maps against
+    the oracle's restatement (or_bcsr_to_tiles), and mmaOBTKernel_tiled /
+    preprocessing_mmaOBTSKernel_tiled_large + mmaOBTSKernel_tiled_large
+    (SparseMatrix with k = 32) on them against the FP64 product;
   * CSR(ifstream) + BlockingEngine(Keeper) + get_fixed_size_grouping +
     CollectBlockingInfo + CSR::reorder on the reference's cs_department.el
     fixture: the compiled reference's groupings, counters, blocking info
@@ -93,6 +97,28 @@ def test_reference_api_from_cpp(tmp_path, golden_dir, tau, cb, rb):
     C = _read(out, "C.bin", np.uint16).view(np.float16).astype(np.float64).reshape(M, N)
     R = of.spmm_bcsr_f64(16, brp, bci, bv, B)
     assert np.linalg.norm(C - R) / np.linalg.norm(R) < 1e-3
+
+    # ---- bcsrBlocking + the tiled launchers
+    trp, tci, tv, tinfo = of.bcsr_to_tiles(M, K, 16, brp, bci, bv)
+    assert np.array_equal(_read(out, "tiles_rowptr.bin", np.int32), trp)
+    assert np.array_equal(_read(out, "tiles_colidx.bin", np.int32), tci)
+    assert np.array_equal(_read(out, "tiles_values.bin", np.uint16), tv)
+    assert np.array_equal(_read(out, "tiles_info.bin", np.int32), tinfo)
+    ntr, ntc = (M + 63) // 64, (K + 63) // 64
+    exp_tinfo = np.zeros(ntr * ntc, np.int32)
+    exp_trel = np.full(ntr * ntc, -1, np.int32)
+    cells = np.repeat(np.arange(ntr), np.diff(trp)) * ntc + tci // 64
+    exp_tinfo[cells], exp_trel[cells] = 1, np.arange(tci.size)
+    assert np.array_equal(_read(out, "tiles_block_info.bin", np.int32), exp_tinfo)
+    assert np.array_equal(_read(out, "tiles_rel_map.bin", np.int32), exp_trel)
+    Ct = _read(out, "C_tiled.bin", np.uint16).view(np.float16).astype(np.float64).reshape(M, N)
+    assert np.linalg.norm(Ct - R) / np.linalg.norm(R) < 1e-3
+    h32 = smat.read_mtx(str(mtx), k=32)
+    M32, K32 = h32["rows"], h32["cols"]
+    B32 = _read(out, "B32.bin", np.uint16).reshape(N, K32).T.copy()
+    R32 = of.spmm_csr_f64(M32, h32["rowptr"], h32["colidx"], h32["values_f16"], B32)
+    C32 = _read(out, "C_tiled_large.bin", np.uint16).view(np.float16).astype(np.float64).reshape(M32, N)
+    assert np.linalg.norm(C32 - R32) / np.linalg.norm(R32) < 1e-3
 
     # ---- CSR + BlockingEngine + reorder against the compiled reference's goldens
     g = np.load(os.path.join(golden_dir, "keeper_golden.npz"))

@@ -50,6 +50,8 @@ MFMA_PEAK_TFLOPS = {"f16": 2500.0, "bf16": 2500.0, "fp8": 5000.0}
 MEASURED_TRAFFIC = {
     ("rmat", 18, 256, 2, "auto", "locality"): (1930e6, "profiles/spmm_lds_bk2_locality_rmat18_n256_hbm_traffic.md",
                                                  "be11342f66f3"),
+    ("rmat", 18, 256, 1, "auto", "locality"): (1240e6, "profiles/spmm_lds_bk1_locality_rmat18_n256_hbm_traffic.md",
+                                                 "be11342f66f3"),
 }
 # the same for the other workloads' calls:
 # (workload, N, bk, n_tile, order, kernel, dtype) -> (bytes, profile file, kernel source id)

@@ -87,15 +87,18 @@ def prune_2of4(dense: np.ndarray) -> np.ndarray:
 
 
 def block_clustered(n: int = 65536, tiles: int = 12000, seed: int = 1003, prune: bool = True, bh: int = 16,
-                    bw: int = 16):
+                    bw: int = 16, fill: float = 1.0):
     """`tiles` distinct aligned bh x bw blocks of an n x n matrix, U[-1, 1]
-    values, optionally pruned to 2 of every aligned 1x4 group."""
+    values, each entry kept with probability `fill`, optionally pruned to 2
+    of every aligned 1x4 group."""
     rng = np.random.default_rng(seed)
     nbr, nbc = n // bh, n // bw
     flat = rng.choice(nbr * nbc, size=tiles, replace=False)
     br, bc = flat // nbc, flat % nbc
     vals = rng.uniform(-1.0, 1.0, size=(tiles, bh, bw)).astype(np.float16)
     vals[vals == 0] = np.float16(0.5)
+    if fill < 1.0:
+        vals[rng.random(vals.shape) >= fill] = 0
     if prune:
         vals = prune_2of4(vals)
     rows = (br[:, None, None] * bh + np.arange(bh)[None, :, None]) * np.ones((1, 1, bw), dtype=np.int64)

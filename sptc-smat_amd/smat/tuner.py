@@ -133,6 +133,25 @@ def make_matrix(seed: int, large: bool = False):
     return kind, gen.power_law(n=n, max_len=L, seed=gs)
 
 
+def make_tiled_matrix(seed: int):
+    """Seeded tile-clustered matrix (nonzeros clustered in aligned bh x bw
+    tiles, the structure merged tiles are for): n log-uniform in [4k, 131k],
+    tile shape from {16x64, 32x32, 64x64, 64x128, 128x64, 128x128}, entry fill
+    U[0.25, 1], pruned to 2:4 with probability 0.3, nnz log-uniform in
+    [5e4, 8e6] (at most half the tile grid).  Returns ("tiled", CSR)."""
+    rng = np.random.default_rng(seed)
+    bh, bw = ((16, 64), (32, 32), (64, 64), (64, 128), (128, 64), (128, 128))[int(rng.integers(6))]
+    n = int(math.exp(rng.uniform(math.log(4e3), math.log(1.31e5))))
+    n = max(4, n // 128) * 128
+    fill = float(rng.uniform(0.25, 1.0))
+    prune = bool(rng.random() < 0.3)
+    nnz = math.exp(rng.uniform(math.log(5e4), math.log(8e6)))
+    per_tile = bh * bw * fill * (0.5 if prune else 1.0)
+    tiles = int(min(max(8, nnz / per_tile), (n // bh) * (n // bw) // 2))
+    gs = int(rng.integers(0, 2 ** 31))
+    return "tiled", gen.block_clustered(n=n, tiles=tiles, seed=gs, prune=prune, bh=bh, bw=bw, fill=fill)
+
+
 # ---- the sweep (fallback and judge) --------------------------------------------
 def reorder_csr(rp, ci, v, tau):
     """Keeper grouping -> fixed-size groups -> row permutation, applied on the
@@ -427,7 +446,12 @@ def _gen_worker_large(seed):
     return seed, kind, M, K, rp, ci, v
 
 
-def collect(first, count, out, N=128, workers=8, log=print, n_list=None, large=False):
+def _gen_worker_tiled(seed):
+    kind, (M, K, rp, ci, v) = make_tiled_matrix(seed)
+    return seed, kind, M, K, rp, ci, v
+
+
+def collect(first, count, out, N=128, workers=8, log=print, n_list=None, large=False, family="workload"):
     import multiprocessing as mp
 
     import torch
@@ -438,8 +462,8 @@ def collect(first, count, out, N=128, workers=8, log=print, n_list=None, large=F
     ctx = mp.get_context("spawn")
     with ctx.Pool(workers) as pool:
         torch.zeros(1).cuda()
-        for k, (seed, kind, M, K, rp, ci, v) in enumerate(pool.imap(_gen_worker_large if large else _gen_worker,
-                                                                                   range(first, first + count))):
+        worker = _gen_worker_tiled if family == "tiled" else _gen_worker_large if large else _gen_worker
+        for k, (seed, kind, M, K, rp, ci, v) in enumerate(pool.imap(worker, range(first, first + count))):
             n_cols = n_list[seed % len(n_list)] if n_list else N
             d_rp, d_ci = torch.from_numpy(rp).cuda(), torch.from_numpy(ci).cuda()
             d_v = torch.from_numpy(v.view(np.int16)).cuda()
@@ -477,6 +501,8 @@ def main(argv=None):
     c.add_argument("--n-list", default="", help="comma-separated N values cycled over the seeds")
     c.add_argument("--workers", type=int, default=8)
     c.add_argument("--large", action="store_true", help="single-matrix sizes (n 100k-600k, nnz 1e6-1.2e7)")
+    c.add_argument("--family", default="workload", choices=["workload", "tiled"],
+                   help="workload: the batched workload's generators; tiled: tile-clustered matrices")
     t = sub.add_parser("train")
     t.add_argument("--data", nargs="+", required=True)
     t.add_argument("--out", required=True)
@@ -488,7 +514,8 @@ def main(argv=None):
     log = lambda *x: print(*x, flush=True)  # noqa: E731
     if a.cmd == "collect":
         n_list = [int(x) for x in a.n_list.split(",")] if a.n_list else None
-        collect(a.first, a.count, a.out, N=a.n, workers=a.workers, log=log, n_list=n_list, large=a.large)
+        collect(a.first, a.count, a.out, N=a.n, workers=a.workers, log=log, n_list=n_list, large=a.large,
+                family=a.family)
     elif a.cmd == "train":
         fs = [np.load(p) for p in a.data]
         feats = np.concatenate([f["feats"] for f in fs])

@@ -275,3 +275,24 @@ def test_trained_model_on_held_out_matrices():
         print(os.path.basename(f), json.dumps({k: rf[k] for k in ("n", "within_10pct", "geo_speedup_vs_plain")}),
               flush=True)
         assert rf["within_10pct"] >= 0.9, f
+
+
+def test_trained_model_on_held_out_tile_clustered_matrices():
+    """Tile-clustered matrices (make_tiled_matrix, seeds 60000-60047, none in
+    the training data): most calls take 5-20 us, so a 1 us difference is 10 %
+    and the within-10 % rate is noisier than on the larger families; the pick
+    must still beat the plain configuration clearly and land within 10 % of
+    the sweep's best on three quarters of the matrices (the model trained
+    without this family reached 31 % and 1.01x)."""
+    _need_gpu()
+    from smat import tuner
+    path = os.path.join(TUNER_DIR, "sweep_heldout_tiled.npz")
+    if not (os.path.exists(path) and os.path.exists(tuner.WEIGHTS_PATH)):
+        pytest.skip("no committed sweep data / weights")
+    d = np.load(path)
+    r = tuner.evaluate(tuner.Model(), d["feats"], d["times"], d["rows"])
+    print(json.dumps({k: r[k] for k in ("n", "within_10pct", "median_ratio_to_best", "geo_speedup_vs_plain")}),
+          flush=True)
+    assert r["within_10pct"] >= 0.75
+    assert r["median_ratio_to_best"] <= 1.05
+    assert r["geo_speedup_vs_plain"] >= 1.08

@@ -417,7 +417,7 @@ def learned_pick(ctx, log, order=None):
     feats = tuner.features(st, ctx.N)
     # the Keeper candidates (tau set) are not used here: the matrix is timed as
     # given or in the locality order
-    avail = tuner.available(ctx.M) * (tuner.ENC[:, -1] == 0)
+    avail = tuner.available(ctx.M) * (tuner.ENC[:, tuner.ENC_REORDER] == 0)
     _, b = tuner.Model().scores(feats, avail.astype("uint8"))
     cfg = tuner.CONFIGS[b]
     ms = (time.perf_counter() - t0) * 1e3
@@ -1054,6 +1054,9 @@ def main():
                     tau, bk, nt, kernel = learned["locality"]["config"]
             tuned = measure(bk, nt, kernel, args.steps, args.warmup, order)
             learned["ratio_to_sweep_best"] = round(tuned["event_ms"] / sweep_best["event_ms"], 3)
+            # the sweep times the matrix as given; the tuned call may run on the locality-ordered copy
+            learned["ratio_basis"] = ("tuned call on the locality-ordered matrix / sweep best on the matrix as "
+                                      "given" if order is not None else "same matrix")
         else:
             tuned = sweep_best
 

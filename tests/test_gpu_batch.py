@@ -51,7 +51,7 @@ def _build(mats, N, seed=1, with_rowmap=False):
         n_tile = [0, 16, 32, 64, 128][i % 5]
         split = [0, 0, 16][i % 3]
         perm = None
-        if with_rowmap and i % 2 == 1:
+        if with_rowmap and i % 4 in (1, 2):  # dense entries and the mixed dense / 2:4 ones
             perm = rng.permutation(M).astype(np.int32)  # A's row r is C's row perm[r]
         rp_a, ci_a, v_a = rp, ci, v
         if perm is not None:

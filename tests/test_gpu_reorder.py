@@ -345,6 +345,35 @@ def test_locality_order_spmm_writes_original_rows(variant):
         A.close()
 
 
+def test_locality_order_mixed24_with_split_rows():
+    """The mixed dense / 2:4 kernel on a locality-ordered matrix (as the
+    locality probe and the batched workload may pick it): split rows, C
+    written back through the row map, against the FP64 product of the
+    original matrix."""
+    _need_gpu()
+    import smat
+    M, K, rp, ci, v = gen.block_clustered(n=4096, tiles=900, seed=17)
+    N = 64
+    d = [torch.from_numpy(x).cuda() for x in (rp, ci, v.view(np.int16))]
+    perm = smat.locality_order(M, K, d[0], d[1])
+    prp, pci, pv = smat.permute_rows(d[0], d[1], d[2], perm)
+    B = gen.dense_b(K, N, seed=4)
+    R = of.spmm_csr_f64(M, rp, ci, v, B)
+    A = smat.Bcsr(M, K, prp, pci, pv, bk=16, tag_rule=smat.SMAT_TAG_EXACT24)
+    v24, i24 = A.compress_24(smat.SMAT_24_SMFMAC)
+    plan = smat.SpmmPlan(A, N, kernel=smat.SMAT_KERNEL_MIXED24, max_blocks_per_item=2)
+    assert plan.info().n_split_rows > 0
+    C = torch.full((M, N), float("nan"), dtype=torch.float32, device="cuda")
+    plan.execute(torch.from_numpy(B.view(np.int16)).cuda(), C, values24=v24, index24=i24,
+                 c_rowmap=perm.to(torch.int32))
+    torch.cuda.synchronize()
+    Cn = C.cpu().numpy().astype(np.float64)
+    assert np.all(np.isfinite(Cn))
+    assert np.linalg.norm(Cn - R) / np.linalg.norm(R) < 1e-5
+    plan.close()
+    A.close()
+
+
 def test_locality_order_rejects_bad_arguments():
     _need_gpu()
     import smat

@@ -130,10 +130,12 @@ def test_probe_locality_contract(margin):
         assert not use and order is None
     rp2, ci2, v2, rowmap = order if use else (d[0], d[1], d[2], None)
     _, bk, nt, kernel = pick
-    A = smat.Bcsr(M, K, rp2, ci2, v2, bk=bk)
-    plan = smat.SpmmPlan(A, N, n_tile=nt)
+    mixed = kernel == "mixed24"  # rebuilt exactly as the probe ran it
+    A = smat.Bcsr(M, K, rp2, ci2, v2, bk=bk, tag_rule=smat.SMAT_TAG_EXACT24 if mixed else smat.SMAT_TAG_REFERENCE)
+    v24, i24 = A.compress_24(smat.SMAT_24_SMFMAC) if mixed else (None, None)
+    plan = smat.SpmmPlan(A, N, n_tile=nt, kernel=smat.SMAT_KERNEL_MIXED24 if mixed else smat.SMAT_KERNEL_DENSE)
     C.fill_(float("nan"))
-    plan.execute(Bd, C, c_rowmap=rowmap)
+    plan.execute(Bd, C, values24=v24, index24=i24, c_rowmap=rowmap)
     torch.cuda.synchronize()
     R = of.spmm_csr_f64(M, rp, ci, v, B)
     Cn = C.cpu().numpy().astype(np.float64)

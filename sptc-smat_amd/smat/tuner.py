@@ -38,7 +38,9 @@ NF = 26                      # SMAT_TUNER_NF
 BKS = (1, 2, 4, 8, 16, 32)
 NTILES = (16, 32, 64, 128)
 NTILES_24 = (16, 32, 64)     # tile widths of the mixed dense / 2:4 kernel
-TAUS = (None, 0.5)           # None: no reordering; else Keeper threshold (16-row groups, 16-column blocks)
+# None: no reordering; else the Keeper threshold (16-row groups, 16-column
+# blocks), the four values of the reference's R-MAT threshold sweep
+TAUS = (None, 0.1, 0.3, 0.5, 0.7)
 REORDER_MAX_ROWS = 16384     # the exact Keeper's host replay grows ~quadratically; larger matrices skip it
 HIDDEN = 64
 WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tuner", "mlp_weights.npz")
@@ -67,7 +69,9 @@ def configs():
 
 
 CONFIGS = configs()
-NE = len(BKS) + len(NTILES) + 2  # one-hot block width, one-hot N-tile, 2:4 kernel, reordered
+# encoding: one-hot block width, one-hot N-tile, 2:4 kernel, reordered, tau (0 without reordering)
+ENC_REORDER = len(BKS) + len(NTILES) + 1
+NE = ENC_REORDER + 2
 PLAIN = CONFIGS.index((None, 16, 64, "dense"))  # the reference's configuration (16x16, dense, no reorder)
 
 
@@ -77,7 +81,8 @@ def encode(cfg) -> np.ndarray:
     e[BKS.index(bk)] = 1
     e[len(BKS) + NTILES.index(nt)] = 1
     e[len(BKS) + len(NTILES)] = kernel == "mixed24"
-    e[len(BKS) + len(NTILES) + 1] = tau is not None
+    e[ENC_REORDER] = tau is not None
+    e[ENC_REORDER + 1] = tau or 0.0
     return e
 
 
@@ -482,6 +487,53 @@ def collect(first, count, out, N=128, workers=8, log=print, n_list=None, large=F
     return out
 
 
+def _gen_worker_any(job):
+    seed, family, large = job
+    kind, (M, K, rp, ci, v) = make_tiled_matrix(seed) if family == "tiled" else make_matrix(seed, large=large)
+    return seed, kind, M, K, rp, ci, v
+
+
+def refresh(path_in, path_out, family="workload", large=False, start=0, count=None, workers=8, log=print):
+    """Carries a sweep file over to the current candidate list: columns of
+    configurations present in `path_in` are kept, and the matrices small
+    enough for the Keeper (whose reordered configurations are new) are
+    regenerated from their seeds and swept again in full.  Resumable: when
+    `path_out` exists it is the starting point, and only the small matrices
+    number start .. start+count-1 are swept in this call."""
+    import multiprocessing as mp
+
+    import torch
+    base = np.load(path_out if os.path.exists(path_out) else path_in)
+    old_cfgs = [tuple(json.loads(c)) for c in base["configs"]]
+    col = {c: i for i, c in enumerate(old_cfgs)}
+    n = len(base["seed"])
+    times = np.full((n, len(CONFIGS)), np.inf)
+    for j, c in enumerate(CONFIGS):
+        if c in col:
+            times[:, j] = base["times"][:, col[c]]
+    rec = {k: list(base[k]) for k in ("seed", "kind", "rows", "nnz", "N", "stats")}
+    rec["feats"] = list(base["feats"])
+    small = [i for i in range(n) if base["rows"][i] <= REORDER_MAX_ROWS]
+    todo = small[start:start + count if count is not None else None]
+    t0 = time.time()
+    if not todo:
+        _save(path_out, dict(rec, times=list(times)))
+        return len(small)
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(workers) as pool:
+        torch.zeros(1).cuda()
+        jobs = [(int(base["seed"][i]), family, large) for i in todo]
+        for k, (i, (seed, kind, M, K, rp, ci, v)) in enumerate(zip(todo, pool.imap(_gen_worker_any, jobs))):
+            assert seed == base["seed"][i] and M == base["rows"][i] and ci.size == base["nnz"][i], seed
+            times[i], _ = sweep(M, K, rp, ci, v, int(base["N"][i]))
+            if k % 20 == 0 or k == len(todo) - 1:
+                log(f"[{k + 1}/{len(todo)}] seed {seed} rows {M}: best {CONFIGS[int(np.argmin(times[i]))]} "
+                    f"({time.time() - t0:.0f}s)")
+                _save(path_out, dict(rec, times=list(times)))
+    _save(path_out, dict(rec, times=list(times)))
+    return len(small)
+
+
 def _save(path, rec):
     np.savez_compressed(path, seed=np.asarray(rec["seed"]), kind=np.asarray(rec["kind"]),
                         rows=np.asarray(rec["rows"]), nnz=np.asarray(rec["nnz"]), N=np.asarray(rec["N"]),
@@ -507,6 +559,14 @@ def main(argv=None):
     t.add_argument("--data", nargs="+", required=True)
     t.add_argument("--out", required=True)
     t.add_argument("--epochs", type=int, default=300)
+    f = sub.add_parser("refresh")
+    f.add_argument("--data", required=True)
+    f.add_argument("--out", required=True)
+    f.add_argument("--family", default="workload", choices=["workload", "tiled"])
+    f.add_argument("--large", action="store_true")
+    f.add_argument("--start", type=int, default=0)
+    f.add_argument("--count", type=int, default=None)
+    f.add_argument("--workers", type=int, default=8)
     e = sub.add_parser("eval")
     e.add_argument("--data", nargs="+", required=True)
     e.add_argument("--weights", default=WEIGHTS_PATH)
@@ -516,6 +576,9 @@ def main(argv=None):
         n_list = [int(x) for x in a.n_list.split(",")] if a.n_list else None
         collect(a.first, a.count, a.out, N=a.n, workers=a.workers, log=log, n_list=n_list, large=a.large,
                 family=a.family)
+    elif a.cmd == "refresh":
+        refresh(a.data, a.out, family=a.family, large=a.large, start=a.start, count=a.count, workers=a.workers,
+                log=log)
     elif a.cmd == "train":
         fs = [np.load(p) for p in a.data]
         feats = np.concatenate([f["feats"] for f in fs])

@@ -34,6 +34,7 @@ _SIG = {
     "or_spmm_tiles_f64": (None, [_i64, _i64, ctypes.c_int, _p, _p, _p, _i64, _p, _i64, _p, _i64]),
     "or_spmm_refsim_fp16acc": (None, [_i64, ctypes.c_int, _p, _p, _p, _i64, _p, _i64, _p, _i64]),
     "or_csr_multiply_f32": (None, [_i64, _p, _p, _p, _p, _i64, _p]),
+    "or_csr_multiply_f32_omp": (ctypes.c_int32, [_i64, _p, _p, _p, _p, _i64, _p, ctypes.c_int32]),
     "or_jaccard_distance_group": (_f32, [_p, _i64, _i64, _p, _i64, _i64, _i64]),
     "or_merge_rows": (_i64, [_p, _i64, _p, _i64, _p]),
     "or_blocking_algo0": (None, [_i64, _p, _p, _f32, _i64, ctypes.c_int, _p, _p]),
@@ -221,6 +222,14 @@ def csr_multiply_f32(rows, rowptr64, colidx64, vals32, B_colmajor, B_cols):
     C = np.zeros(rows * B_cols, np.float32)
     lib().or_csr_multiply_f32(rows, P(rowptr64), P(colidx64), P(vals32), P(B_colmajor), B_cols, P(C))
     return C
+
+
+def csr_multiply_f32_omp(rows, rowptr64, colidx64, vals32, B_colmajor, B_cols, threads):
+    """Row-parallel (OpenMP) CSR::multiply; returns (C, threads used)."""
+    C = np.zeros(rows * B_cols, np.float32)
+    used = lib().or_csr_multiply_f32_omp(rows, P(rowptr64), P(colidx64), P(vals32), P(B_colmajor), B_cols, P(C),
+                                         threads)
+    return C, int(used)
 
 
 def jaccard(a, ga, b, gb, block, use_ref=False):

@@ -1,7 +1,7 @@
 // SMaT CPU oracle — TEST INFRASTRUCTURE ONLY.
 //
-// Plain, serial C++ restatements of the reference SMaT algorithms on the SpMM
-// hot path. Only the test-suite, the smoke check and the CPU-baseline leg of
+// Plain C++ restatements of the reference SMaT algorithms on the SpMM hot
+// path, serial except the all-cores CPU baseline (or_csr_multiply_f32_omp). Only the test-suite, the smoke check and the CPU-baseline leg of
 // the benchmark may load this library; the product (libsmat.so) never links
 // or calls it.
 //
@@ -41,6 +41,7 @@
 //                              reference test holds its outputs: that part
 //                              stays unpinned
 //   or_csr_multiply_f32        dense brute force (FP32)
+//   or_csr_multiply_f32_omp    bit-identical to or_csr_multiply_f32
 //   or_e4m3_encode/decode      all 254 finite codes round-trip; every midpoint
 //                              between neighbouring codes ties to the even code;
 //                              hand values (448 = 0x7e, 1 = 0x38, 2^-9 = 0x01)
@@ -62,6 +63,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <omp.h>
 #include <numeric>
 #include <set>
 #include <utility>
@@ -473,6 +475,31 @@ void or_csr_multiply_f32(int64_t rows, const int64_t* rowptr,
       float v = val[nz];
       for (int64_t j = 0; j < B_cols; j++) C[i + j * rows] += v * B[col + j * rows];
     }
+}
+
+// The same product with the rows shared over OpenMP threads (the CPU baseline
+// on all host cores; the reference itself has no parallel region).  Every C
+// element is still summed by one thread in the row's entry order, so the
+// result is bit-identical to or_csr_multiply_f32.  Rows are dealt in chunks
+// of 64 dynamically: graph rows are skewed.  Returns the threads used.
+int32_t or_csr_multiply_f32_omp(int64_t rows, const int64_t* rowptr,
+                                const int64_t* colidx, const float* val,
+                                const float* B, int64_t B_cols, float* C,
+                                int32_t threads) {
+  int32_t used = 1;
+#pragma omp parallel num_threads(threads > 0 ? threads : 1)
+  {
+#pragma omp single
+    used = omp_get_num_threads();
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t i = 0; i < rows; i++)
+      for (int64_t nz = rowptr[i]; nz < rowptr[i + 1]; nz++) {
+        const int64_t col = colidx[nz];
+        const float v = val[nz];
+        for (int64_t j = 0; j < B_cols; j++) C[i + j * rows] += v * B[col + j * rows];
+      }
+  }
+  return used;
 }
 
 // ---------------------------------------------------------------------------

@@ -282,14 +282,19 @@ def forward_backward(net: Net, X, y, need_grad=True):
 
 def samples(feats, times):
     """(matrix, config) rows with finite times: inputs and per-matrix centred
-    log times (the model ranks configurations of one matrix)."""
+    log times (the model ranks configurations of one matrix).  The centre is
+    the mean log time of the configurations without reordering, which every
+    matrix has, so the offset does not depend on which reordered candidates a
+    matrix was swept with."""
     X, y, owner = [], [], []
+    plain_order = np.array([c[0] is None for c in CONFIGS])
     for i in range(feats.shape[0]):
         ok = np.isfinite(times[i])
         if ok.sum() < 2:
             continue
+        ref = ok & plain_order
         lt = np.log(times[i][ok])
-        lt = lt - lt.mean()
+        lt = lt - (np.log(times[i][ref]).mean() if ref.any() else lt.mean())
         for c, t in zip(np.nonzero(ok)[0], lt):
             X.append(np.concatenate([feats[i], ENC[c]]))
             y.append(t)

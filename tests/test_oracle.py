@@ -356,6 +356,28 @@ def test_csr_multiply_golden(golden_dir):
     n, bc = d["n"]
     C = of.csr_multiply_f32(int(n), d["rowptr"], d["colidx"], d["vals"], d["B"], int(bc))
     assert np.array_equal(C, d["C"])
+    C4, used = of.csr_multiply_f32_omp(int(n), d["rowptr"], d["colidx"], d["vals"], d["B"], int(bc), 4)
+    assert used >= 1 and np.array_equal(C4, d["C"])
+
+
+def test_csr_multiply_omp_bit_identical_on_skewed_rows():
+    """The all-cores CPU baseline sums each C element in the serial order, so
+    it is bit-identical to the serial product on a skewed R-MAT sample (long
+    rows spread over chunks and threads), and both agree with the FP64
+    product to FP32 accuracy."""
+    from smat import gen
+    M, K, rp, ci, v = gen.rmat(scale=11, edge_factor=16, seed=4)
+    rp64, ci64 = rp.astype(np.int64), ci.astype(np.int64)
+    vals = v.view(np.float16).astype(np.float32)
+    N = 8
+    B = np.random.default_rng(2).uniform(-1, 1, M * N).astype(np.float32)
+    C1 = of.csr_multiply_f32(M, rp64, ci64, vals, B, N)
+    C8, used = of.csr_multiply_f32_omp(M, rp64, ci64, vals, B, N, 8)
+    assert used >= 1 and np.array_equal(C1, C8)
+    rows = np.repeat(np.arange(M), np.diff(rp64))
+    R = np.zeros((M, N))
+    np.add.at(R, rows, vals[:, None].astype(np.float64) * B.reshape(N, M).T[ci64].astype(np.float64))
+    assert np.allclose(C8.reshape(N, M).T, R, rtol=1e-4, atol=1e-4)
 
 
 def test_fp16_conversions():

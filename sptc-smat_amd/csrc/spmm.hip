@@ -417,16 +417,18 @@ __device__ inline void load_a_any(const void* __restrict__ values, int64_t lo_of
 //     column L of it, i.e. B[k0..k0+3][n0+L] — 4 of the 8 k-values the
 //     operand layout wants; two such reads give a lane its 8 values;
 //   * FP8, ds_read_b64_tr_b8: one read of an 8 x 16 sub-tile gives all 8.
-//
-// Pipeline (per wave, two register stages X/Y, loop unrolled by two so the
-// stage index is static): while step p is transposed and multiplied, the B
-// rows and A values of steps p+STEP and p+2*STEP are in flight.  A step's
-// block column starts are fetched one phase before its B rows, with one
-// coalesced load (lane l holds colidx[p + l % STEP]) and handed to the B-row
-// lanes by a lane shuffle; because each phase issues the column-index load
-// before the B loads, the in-order vmcnt wait for an index never drains the
-// other stage's B loads.  Waves never share LDS, so no barrier is needed: a
-// wave's LDS accesses complete in issue order.
+// Waves never share LDS, so no barrier is needed: a wave's LDS accesses
+// complete in issue order.  Two pipelines (both two register stages):
+//   * lds_tile_narrow (16x1 blocks): a step's A values are one coalesced
+//     128-bit load per lane, transposed through LDS like B; loads are
+//     predicated by buffer resources, never by branches, and the order of
+//     loads is fixed, so the compiler's wait counts keep two steps in flight;
+//   * lds_tile_wide (wider blocks): A operands loaded straight into registers,
+//     loads guarded by branches.  (Measured at R-MAT scale 18, N = 256: the
+//     narrow pipeline takes 16x1 blocks from 0.353 to 0.247 ms and power-law
+//     16x1 from 0.504 to 0.393 ms, while the same pipeline for 16x2, 16x4,
+//     16x8 and 16x16 blocks was 4-28 % slower than this one, with any
+//     pipeline depth, load form or occupancy tried.)
 //
 // Scaled codecs multiply the FP32 tile by a_scale[block row] * b_scale[col]
 // before it is stored (split rows scale their partial tiles, so the slice
@@ -447,17 +449,264 @@ struct Operands {
   int64_t ldc;
   float* scratch;
   const int32_t* rowmap;  // NULL, or the C row of each A row
+  uint32_t b_bytes;       // bytes of one B array (the lo array of a split B is as long)
 };
 
-// LDS bytes one wave of the LDS kernel needs.
+// LDS bytes one wave of the LDS kernel needs: the B tile of one step (32
+// rows of 16*NSUB elements) and, for 16x1 blocks, behind it the A tile of
+// one step (32 blocks of 16 elements); rows padded by 16 bytes.
 template <int NSUB, int DT>
-constexpr int lds_wave_bytes() {
+constexpr int lds_b_bytes() {
   return Codec<DT>::NARR * 32 * (16 * NSUB * Codec<DT>::ES + 16);
 }
+template <int DT>
+constexpr int lds_a_bytes() {
+  return Codec<DT>::NARR * 32 * (16 * Codec<DT>::ES + 16);
+}
+template <int BK, int NSUB, int DT>
+constexpr int lds_wave_bytes() {
+  return lds_b_bytes<NSUB, DT>() + (BK == 1 ? lds_a_bytes<DT>() : 0);
+}
 
-template <int BK, int NSUB, int DT, bool OUT_F16, bool PRE = false>
-__device__ __forceinline__ void lds_tile(const Item& it, int64_t n0, const Operands& op, uint8_t* __restrict__ lds,
-                                         int lane) {
+// Buffer loads of the LDS kernel.  Every load goes through a raw buffer
+// resource (gfx9 layout) over exactly the bytes it may read; an offset at or
+// past the end (kOob) returns zeros without a memory access.  That is how the
+// kernel predicates its loads: no branch around a load and no select on
+// loaded data, so nothing between a load's issue and its use makes the
+// compiler drain the other stages' loads (a conditional load followed by a
+// shift of its value did exactly that: a wait for every outstanding load after
+// each step's issue).
+typedef unsigned smat_u4v __attribute__((ext_vector_type(4)));
+typedef unsigned smat_u2v __attribute__((ext_vector_type(2)));
+constexpr uint32_t kOob = 0xFFFFFFF0u;          // past every resource (they are capped below it)
+constexpr double kMaxResourceBytes = 4294967040.0;  // 2^32 - 256
+
+__device__ inline __amdgpu_buffer_rsrc_t rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+__device__ inline uint4 bload16(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  const smat_u4v v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+  return uint4{v.x, v.y, v.z, v.w};
+}
+__device__ inline uint2 bload8(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  const smat_u2v v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+  return uint2{v.x, v.y};
+}
+__device__ inline uint32_t bload4(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+}
+__device__ inline uint32_t bload2(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+}
+
+// A values of one step of 16x1 blocks as loaded: lane l holds elements
+// 8*(l%2) .. 8*(l%2)+7 of block p + l/2 (16 bytes, or 8 for FP8), blocks past
+// the item's end reading as zeros (their offsets fall outside the item's
+// resource ra, whose offsets are relative to block pbeg; q0 = p - pbeg).
+template <int ES>
+__device__ inline void a_issue_bk1(__amdgpu_buffer_rsrc_t ra, int32_t q0, int lane, uint32_t (&w)[2 * ES]) {
+  const uint32_t off = ((uint32_t)(q0 + (lane >> 1)) * 16u + (uint32_t)(lane & 1) * 8u) * ES;
+  if constexpr (ES == 2) {
+    const uint4 v = bload16(ra, off);
+    w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+  } else {
+    const uint2 v = bload8(ra, off);
+    w[0] = v.x; w[1] = v.y;
+  }
+}
+
+// One task (item, N tile) of 16x1 blocks: the narrow pipeline.
+template <int NSUB, int DT, bool OUT_F16, bool PRE, int DEPTH = 2>
+__device__ __forceinline__ void lds_tile_narrow(const Item& it, int64_t n0, const Operands& op,
+                                                uint8_t* __restrict__ lds, int lane) {
+  constexpr int BK = 1;
+  using CD = Codec<DT>;
+  constexpr int ES = CD::ES, NARR = CD::NARR;
+  constexpr int NT = 16 * NSUB;           // columns per wave
+  constexpr int RB = NT * ES;             // bytes per B row segment
+  constexpr int CPR = RB / 16;            // 16-byte chunks per B row segment
+  constexpr int RPL = 64 / CPR;           // B rows per load instruction
+  static_assert(RPL <= 32, "a load instruction must not span more than one step");
+  constexpr int NLD = 32 / RPL;           // load instructions per MFMA step and array
+  constexpr int PITCH = RB + 16;          // LDS row pitch of the B tile in bytes (16-byte pad)
+  constexpr int PA = 16 * ES + 16;        // LDS row pitch of the A tile
+  constexpr int STEP = 32 / BK;           // blocks per MFMA step
+  constexpr int AW = 2 * ES;              // 32-bit words of A per lane and step
+  uint8_t* __restrict__ lds_a = lds + lds_b_bytes<NSUB, DT>();
+  const uint8_t* __restrict__ B = static_cast<const uint8_t*>(op.b);
+  const int64_t ldb = op.ldb, N = op.N;
+
+  const int r16 = lane & 15, grp = lane >> 4;
+  const int32_t crl = PRE ? c_row_of_lane(it, lane, op.rowmap) : 0;
+  // load coordinates: instruction i covers rows i*RPL .. i*RPL+RPL-1
+  const int lrow = lane / CPR, lch = lane % CPR;
+  const int64_t ncol = n0 + lch * (16 / ES);
+  const bool col_ok = ncol < N;  // N is a multiple of the chunk: a chunk is all in or all out
+
+  smat_f4 acc[NSUB];
+#pragma unroll
+  for (int j = 0; j < NSUB; j++) acc[j] = smat_f4{0.f, 0.f, 0.f, 0.f};
+
+  const int32_t pb = it.pbeg, pe = it.pend;
+  // resources: the item's column starts and A blocks (offsets relative to
+  // block pb), and each B array
+  const __amdgpu_buffer_rsrc_t rc = rsrc(op.colidx + pb, (uint32_t)(pe - pb) * 4u);
+  __amdgpu_buffer_rsrc_t ra[NARR], rb[NARR];
+#pragma unroll
+  for (int h = 0; h < NARR; h++) {
+    ra[h] = rsrc(static_cast<const uint8_t*>(op.a) + ((int64_t)pb * 16 * BK + h * op.a_lo) * ES,
+                 (uint32_t)(pe - pb) * (16u * BK * ES));
+    rb[h] = rsrc(B + h * op.b_lo * ES, op.b_bytes);
+  }
+  // column starts of the blocks of step p: lane l holds colidx[p + l % STEP]
+  // (zero past the end of the item)
+  auto cload = [&](int32_t p) -> int32_t {
+    return (int32_t)bload4(rc, (uint32_t)(p - pb + lane % STEP) * 4u);
+  };
+  // A words and B rows of step p (zeros past the end of the item), and the
+  // column starts of step pc into cid once step p's are taken from it.
+  // Issued for every step the loop reaches, so the loop body has no branch
+  // around a load; the fixed order (A, column starts, B) is the same in the
+  // prologue and the loop, so the compiler's wait counts stay exact.
+  auto issue = [&](int32_t p, int32_t& cid, int32_t pc, uint4 (&bb)[NARR][NLD], uint32_t (&aw)[NARR][AW]) {
+#pragma unroll
+    for (int h = 0; h < NARR; h++) a_issue_bk1<ES>(ra[h], p - pb, lane, aw[h]);
+    int32_t cb[NLD];
+#pragma unroll
+    for (int i = 0; i < NLD; i++) cb[i] = __shfl(cid, (i * RPL + lrow) / BK);
+    cid = cload(pc);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < NLD; i++) {
+      const int row = i * RPL + lrow;  // 0..31 = k-slot
+      const int boff = row / BK;
+      const int32_t c = cb[i];
+      const bool ok = p + boff < pe && col_ok;
+      // 32-bit byte offset from the array base (the launcher checks that B,
+      // hi and lo arrays together, spans under 4 GiB)
+      const uint32_t off = ok ? ((uint32_t)(c + row % BK) * (uint32_t)ldb + (uint32_t)ncol) * ES : kOob;
+#pragma unroll
+      for (int h = 0; h < NARR; h++) bb[h][i] = bload16(rb[h], off);
+    }
+  };
+  // landed B rows and A values to their LDS tiles
+  auto consume = [&](const uint4 (&bb)[NARR][NLD], const uint32_t (&aw)[NARR][AW]) {
+#pragma unroll
+    for (int h = 0; h < NARR; h++)
+#pragma unroll
+      for (int i = 0; i < NLD; i++)
+        *reinterpret_cast<uint4*>(lds + (h * 32 + i * RPL + lrow) * PITCH + lch * 16) = bb[h][i];
+#pragma unroll
+    for (int h = 0; h < NARR; h++) {
+      uint8_t* dst = lds_a + (h * 32 + (lane >> 1)) * PA + (lane & 1) * 8 * ES;
+      if constexpr (ES == 2) *reinterpret_cast<uint4*>(dst) = uint4{aw[h][0], aw[h][1], aw[h][2], aw[h][3]};
+      else *reinterpret_cast<uint2*>(dst) = uint2{aw[h][0], aw[h][1]};
+    }
+  };
+  auto compute = [&]() {
+    uint4 a[NARR];  // A operands from the A tile, transposed like B
+#pragma unroll
+    for (int h = 0; h < NARR; h++) {
+      if constexpr (DT == 2) {
+        const uint2 v = tr8(lds_a + (8 * grp + (r16 >> 1)) * PA + (r16 & 1) * 8);
+        a[h] = uint4{v.x, v.y, 0, 0};
+      } else {
+        const uint8_t* ab = lds_a + (h * 32 + 8 * grp + (r16 >> 2)) * PA + (r16 & 3) * 8;
+        a[h] = tr_pair(reinterpret_cast<const uint16_t*>(ab), reinterpret_cast<const uint16_t*>(ab + 4 * PA));
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NSUB; j++) {
+      if constexpr (DT == 2) {
+        const uint8_t* base = lds + (8 * grp + (r16 >> 1)) * PITCH + 16 * j + (r16 & 1) * 8;
+        acc[j] = mma_fp8(uint2{a[0].x, a[0].y}, tr8(base), acc[j]);
+      } else {
+        const uint8_t* base = lds + (8 * grp + (r16 >> 2)) * PITCH + (16 * j + (r16 & 3) * 4) * 2;
+        const uint4 bh = tr_pair(reinterpret_cast<const uint16_t*>(base),
+                                 reinterpret_cast<const uint16_t*>(base + 4 * PITCH));
+        if constexpr (DT == 3) {
+          const uint8_t* lb = base + 32 * PITCH;
+          const uint4 bl = tr_pair(reinterpret_cast<const uint16_t*>(lb),
+                                   reinterpret_cast<const uint16_t*>(lb + 4 * PITCH));
+          acc[j] = mma<0>(a[1], bh, acc[j]);  // small terms first
+          acc[j] = mma<0>(a[0], bl, acc[j]);
+          acc[j] = mma<0>(a[0], bh, acc[j]);
+        } else {
+          acc[j] = mma<DT>(a[0], bh, acc[j]);
+        }
+      }
+    }
+  };
+
+  // DEPTH register stages, stage d holding steps d, d + DEPTH, ...; the round
+  // loops are unrolled DEPTH times so every stage index is static.
+  uint4 bS[DEPTH][NARR][NLD];
+  uint32_t aS[DEPTH][NARR][AW];
+  int32_t cS[DEPTH];
+  // one step of stage d: its landed loads to LDS, the loads of the step
+  // DEPTH further (when `more`), the matrix-core work.  The barrier keeps the
+  // next stage's LDS stores (and their wait for its loads) behind this
+  // stage's issue: hoisted above it they drain every load in flight.
+  auto step = [&](int d, int32_t p_next, bool more) {
+    consume(bS[d], aS[d]);
+    if (more) issue(p_next, cS[d], p_next + DEPTH * STEP, bS[d], aS[d]);
+    compute();
+    __builtin_amdgcn_sched_barrier(0);
+  };
+#pragma unroll
+  for (int d = 0; d < DEPTH; d++) cS[d] = cload(pb + d * STEP);
+  if (pe - pb > DEPTH * STEP) {
+    // prologue, then rounds in which every stage issues its next step: no
+    // branch in the loop body and the same load order on entry as around
+    // the loop, so the compiler's wait counts let DEPTH steps fly
+#pragma unroll
+    for (int d = 0; d < DEPTH; d++) {
+      issue(pb + d * STEP, cS[d], pb + (d + DEPTH) * STEP, bS[d], aS[d]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    int32_t p = pb;
+    for (; p + (2 * DEPTH - 1) * STEP < pe; p += DEPTH * STEP) {
+#pragma unroll
+      for (int d = 0; d < DEPTH; d++) step(d, p + (d + DEPTH) * STEP, true);
+    }
+    // drain: steps p .. p + (DEPTH-1)*STEP are in flight (all inside the
+    // item), at most DEPTH - 1 more follow
+#pragma unroll
+    for (int d = 0; d < DEPTH; d++) step(d, p + (d + DEPTH) * STEP, p + (d + DEPTH) * STEP < pe);
+#pragma unroll
+    for (int d = 0; d < DEPTH; d++)
+      if (p + (d + DEPTH) * STEP < pe) step(d, 0, false);
+  } else {
+    // short items (at most DEPTH steps): every load up front
+#pragma unroll
+    for (int d = 0; d < DEPTH; d++)
+      if (pb + d * STEP < pe) issue(pb + d * STEP, cS[d], pe, bS[d], aS[d]);
+#pragma unroll
+    for (int d = 0; d < DEPTH; d++)
+      if (pb + d * STEP < pe) step(d, 0, false);
+  }
+  if constexpr (CD::SCALED) {
+    const float sa = op.a_scale[it.brow];
+#pragma unroll
+    for (int j = 0; j < NSUB; j++) {
+      const int64_t n = n0 + 16 * j + r16;
+      const float sc = sa * (n < N ? op.b_scale[n] : 0.f);
+#pragma unroll
+      for (int i = 0; i < 4; i++) acc[j][i] *= sc;
+    }
+  }
+  store_tile<NSUB, OUT_F16, PRE>(acc, it, n0, r16, grp, N, op.c, op.ldc, op.scratch, op.rowmap, crl);
+}
+
+// One task of blocks 16 x BK, BK >= 2: the wide pipeline (two register
+// stages X/Y, loop unrolled by two so the stage index is static; while step
+// p is transposed and multiplied, the B rows and A values of steps p+STEP
+// and p+2*STEP are in flight; a step's block column starts are fetched one
+// phase before its B rows with one coalesced load, lane l holding
+// colidx[p + l % STEP], and handed to the B-row lanes by a lane shuffle).
+template <int BK, int NSUB, int DT, bool OUT_F16, bool PRE>
+__device__ __forceinline__ void lds_tile_wide(const Item& it, int64_t n0, const Operands& op,
+                                              uint8_t* __restrict__ lds, int lane) {
   using CD = Codec<DT>;
   constexpr int ES = CD::ES, NARR = CD::NARR;
   constexpr int NT = 16 * NSUB;           // columns per wave
@@ -576,10 +825,17 @@ __device__ __forceinline__ void lds_tile(const Item& it, int64_t n0, const Opera
   store_tile<NSUB, OUT_F16, PRE>(acc, it, n0, r16, grp, N, op.c, op.ldc, op.scratch, op.rowmap, crl);
 }
 
-template <int BK, int NSUB, int DT, bool OUT_F16, bool PRE>
+template <int BK, int NSUB, int DT, bool OUT_F16, bool PRE = false, int DEPTH = 2>
+__device__ __forceinline__ void lds_tile(const Item& it, int64_t n0, const Operands& op, uint8_t* __restrict__ lds,
+                                         int lane) {
+  if constexpr (BK == 1) lds_tile_narrow<NSUB, DT, OUT_F16, PRE, DEPTH>(it, n0, op, lds, lane);
+  else lds_tile_wide<BK, NSUB, DT, OUT_F16, PRE>(it, n0, op, lds, lane);
+}
+
+template <int BK, int NSUB, int DT, bool OUT_F16, bool PRE, int DEPTH = 2>
 __device__ __forceinline__ void lds_kernel_body(const Item* __restrict__ items, int64_t n_items, int32_t ntiles,
                                                 const Operands& op, int32_t order) {
-  constexpr int WB = lds_wave_bytes<NSUB, DT>();
+  constexpr int WB = lds_wave_bytes<BK, NSUB, DT>();
   __shared__ __attribute__((aligned(16))) uint8_t lds_all[kWaves * WB];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
@@ -587,7 +843,7 @@ __device__ __forceinline__ void lds_kernel_body(const Item* __restrict__ items, 
   int nt;
   if (!wave_task(blockIdx.x, gridDim.x, wave, n_items, ntiles, order, item_id, nt)) return;
   const Item it = items[item_id];
-  lds_tile<BK, NSUB, DT, OUT_F16, PRE>(it, (int64_t)nt * 16 * NSUB, op, lds_all + wave * WB, lane);
+  lds_tile<BK, NSUB, DT, OUT_F16, PRE, DEPTH>(it, (int64_t)nt * 16 * NSUB, op, lds_all + wave * WB, lane);
 }
 
 template <int BK, int NSUB, int DT, bool OUT_F16>
@@ -599,10 +855,13 @@ __global__ __launch_bounds__(256) void k_spmm_lds(const Item* __restrict__ items
 // Calls with a row map on 64-column FP16 / BF16 tiles (NSUB = 4, DT <= 1):
 // the row map read when a task starts (PRE), the kernel held to 4 waves per
 // SIMD (128 VGPR + AGPR), which the row-map VGPR would otherwise cost.
+#ifndef SMAT_LDS64_DEPTH
+#define SMAT_LDS64_DEPTH 2
+#endif
 template <int BK, int DT, bool OUT_F16>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_spmm_lds64(
     const Item* __restrict__ items, int64_t n_items, int32_t ntiles, Operands op, int32_t order) {
-  lds_kernel_body<BK, 4, DT, OUT_F16, true>(items, n_items, ntiles, op, order);
+  lds_kernel_body<BK, 4, DT, OUT_F16, true, SMAT_LDS64_DEPTH>(items, n_items, ntiles, op, order);
 }
 
 // Mixed dense / 2:4 kernel for 16x16 blocks and row-major B.  Dense blocks
@@ -850,6 +1109,7 @@ __device__ inline Operands uni(const Operands& o) {
   u.ldc = uni(o.ldc);
   u.scratch = uni(o.scratch);
   u.rowmap = uni(o.rowmap);
+  u.b_bytes = (uint32_t)__builtin_amdgcn_readfirstlane((int)o.b_bytes);
   return u;
 }
 
@@ -882,8 +1142,9 @@ template <int DT, int NS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NS == 8 ? 2 : NS == 4 ? 3 : 4)))
 void k_batch(const BatchMat* __restrict__ mats, int32_t nmats, int64_t total, int64_t chunk,
              unsigned long long* __restrict__ counters) {
-  constexpr int WB = NS > 4 || lds_wave_bytes<NS, DT>() > mixed_wave_bytes<NS>() ? lds_wave_bytes<NS, DT>()
-                                                                                  : mixed_wave_bytes<NS>();
+  // (the block width is only known per task: room for the A tile of 16x1 blocks)
+  constexpr int WB = NS > 4 || lds_wave_bytes<1, NS, DT>() > mixed_wave_bytes<NS>() ? lds_wave_bytes<1, NS, DT>()
+                                                                                     : mixed_wave_bytes<NS>();
   constexpr int NX = 8;
   __shared__ __attribute__((aligned(16))) uint8_t lds_all[kWaves * WB];
   const int lane = threadIdx.x & 63;
@@ -1152,12 +1413,20 @@ void launch_mixed(const smat_plan_t* P, const uint16_t* v24, const uint16_t* i24
 // LDS staging of row-major B: 16-byte chunks (N and ldb multiples of the
 // chunk, B 16-byte aligned) and 32-bit byte offsets (every B array the call
 // reads under 4 GiB).
+// Bytes of one row-major B array of a call (the range of its buffer resource
+// in the LDS kernel; 0 where the LDS kernel cannot run).
+inline uint32_t b_array_bytes(const smat_plan_t* P, int64_t ldb) {
+  const int es = P->opts.in_dtype == SMAT_FP8_E4M3 ? 1 : 2;
+  const double bytes = (double)P->A.K * (double)ldb * es;
+  return bytes <= kMaxResourceBytes ? (uint32_t)(P->A.K * ldb * es) : 0u;
+}
+
 inline bool lds_aligned(const smat_plan_t* P, const Operands& op, int elems_per_chunk) {
   const int es = P->opts.in_dtype == SMAT_FP8_E4M3 ? 1 : 2;
   const int narr = P->opts.in_dtype == SMAT_SPLIT_F16 ? 2 : 1;
   const double bytes = (double)P->A.K * (double)op.ldb * es * narr;
   return P->N % elems_per_chunk == 0 && op.ldb % elems_per_chunk == 0 &&
-         (reinterpret_cast<uintptr_t>(op.b) & 15) == 0 && bytes < 4294967296.0;
+         (reinterpret_cast<uintptr_t>(op.b) & 15) == 0 && bytes <= kMaxResourceBytes;
 }
 
 template <int BK, int BLAYOUT, int DT, bool OUT_F16>
@@ -1494,7 +1763,7 @@ smat_status prepare_operands(smat_plan P, const smat_spmm_operands* ops, Operand
   const void* a = ops->a_values ? ops->a_values : P->A.values;
   SMAT_REQUIRE(a, "smat_spmm_execute: no A values");
   Operands op{P->A.colidx, a, 0, ops->a_scale, ops->b, ldb, 0, ops->b_scale, P->N, ops->c, ldc,
-              P->scratch.as<float>(), ops->c_rowmap};
+              P->scratch.as<float>(), ops->c_rowmap, b_array_bytes(P, ldb)};
   const int chunk = dt == SMAT_FP8_E4M3 ? 16 : 8;  // elements per 16-byte B load
   const bool aligned = lds_aligned(P, op, chunk);
   if ((P->opts.b_staging == SMAT_BSTAGE_LDS && P->opts.b_layout == SMAT_B_ROW_MAJOR) || P->mixed || scaled)
@@ -1861,7 +2130,7 @@ smat_status smat_batch_create_ex(int32_t count, const smat_plan* plans, const sm
     const void* a = o.a_values ? o.a_values : P->A.values;
     BatchMat& m = mats[(size_t)pos];
     m.op = Operands{P->A.colidx, a, 0, nullptr, o.b, o.ldb, 0, nullptr, P->N, o.c, o.ldc, P->scratch.as<float>(),
-                    o.c_rowmap};
+                    o.c_rowmap, b_array_bytes(P, o.ldb)};
     SMAT_REQUIRE(lds_aligned(P, m.op, 8), "smat_batch_create: entry %d: N and ldb must be multiples of 8, B "
                                           "16-byte aligned and under 4 GiB", i);
     m.items = P->items.as<Item>();

@@ -50,14 +50,14 @@ MFMA_PEAK_TFLOPS = {"f16": 2500.0, "bf16": 2500.0, "fp8": 5000.0}
 MEASURED_TRAFFIC = {
     ("rmat", 18, 256, 2, "auto", "locality"): (1930e6, "profiles/spmm_lds_bk2_locality_rmat18_n256_hbm_traffic.md",
                                                  "be11342f66f3"),
-    ("rmat", 18, 256, 1, "auto", "locality"): (1240e6, "profiles/spmm_lds_bk1_locality_rmat18_n256_hbm_traffic.md",
-                                                 "be11342f66f3"),
+    ("rmat", 18, 256, 1, "auto", "locality"): (1359e6, "profiles/spmm_lds_bk1_locality_rmat18_n256_hbm_traffic.md",
+                                                 "47ef04834828"),
 }
 # the same for the other workloads' calls:
 # (workload, N, bk, n_tile, order, kernel, dtype) -> (bytes, profile file, kernel source id)
 CONFIG_TRAFFIC = {
-    ("powerlaw", 128, 1, 64, "locality", "dense", "f16"): (2476e6, "profiles/spmm_lds_bk1_locality_powerlaw_n128.md",
-                                                           "be11342f66f3"),
+    ("powerlaw", 128, 1, 64, "locality", "dense", "f16"): (2836e6, "profiles/spmm_lds_bk1_locality_powerlaw_n128.md",
+                                                           "47ef04834828"),
     ("clustered", 256, 16, "auto", "none", "mixed24", "f16"): (163.7e6,
                                                               "profiles/spmm_mixed_vs_dense_clustered24_n256.md",
                                                               "be11342f66f3"),

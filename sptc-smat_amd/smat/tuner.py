@@ -17,6 +17,7 @@ Command line (run on a GPU):
     python -m smat.tuner collect --first 10000 --count 2000 --out sweep_train.npz
     python -m smat.tuner train --data sweep_train.npz --out mlp.npz
     python -m smat.tuner eval --data sweep_test.npz --weights mlp.npz
+    python -m smat.tuner resweep --data sweep_train.npz --out sweep_train.npz   # after a kernel change
 """
 from __future__ import annotations
 
@@ -498,13 +499,13 @@ def _gen_worker_any(job):
     return seed, kind, M, K, rp, ci, v
 
 
-def refresh(path_in, path_out, family="workload", large=False, start=0, count=None, workers=8, log=print):
-    """Carries a sweep file over to the current candidate list: columns of
-    configurations present in `path_in` are kept, and the matrices small
-    enough for the Keeper (whose reordered configurations are new) are
-    regenerated from their seeds and swept again in full.  Resumable: when
-    `path_out` exists it is the starting point, and only the small matrices
-    number start .. start+count-1 are swept in this call."""
+def resweep(path_in, path_out, family="workload", large=False, count=None, workers=8, deadline_s=None, log=print):
+    """Times every configuration again, with the current kernels, for the
+    matrices of a sweep file (regenerated from their seeds and checked against
+    the recorded sizes).  Resumable: the output keeps a `fresh` mask, and when
+    `path_out` exists it is the starting point; at most `count` matrices (or
+    until `deadline_s` seconds have passed) per call.  Returns the number of
+    fresh matrices."""
     import multiprocessing as mp
 
     import torch
@@ -516,35 +517,40 @@ def refresh(path_in, path_out, family="workload", large=False, start=0, count=No
     for j, c in enumerate(CONFIGS):
         if c in col:
             times[:, j] = base["times"][:, col[c]]
+    fresh = base["fresh"].copy() if "fresh" in base else np.zeros(n, bool)
     rec = {k: list(base[k]) for k in ("seed", "kind", "rows", "nnz", "N", "stats")}
     rec["feats"] = list(base["feats"])
-    small = [i for i in range(n) if base["rows"][i] <= REORDER_MAX_ROWS]
-    todo = small[start:start + count if count is not None else None]
+    todo = [i for i in range(n) if not fresh[i]][:count]
     t0 = time.time()
-    if not todo:
-        _save(path_out, dict(rec, times=list(times)))
-        return len(small)
+    # the workers only run numpy generators; they are spawned before the GPU
+    # is touched, so no process that has used the GPU forks
     ctx = mp.get_context("spawn")
     with ctx.Pool(workers) as pool:
         torch.zeros(1).cuda()
         jobs = [(int(base["seed"][i]), family, large) for i in todo]
         for k, (i, (seed, kind, M, K, rp, ci, v)) in enumerate(zip(todo, pool.imap(_gen_worker_any, jobs))):
+            if deadline_s is not None and time.time() - t0 > deadline_s:
+                break
             assert seed == base["seed"][i] and M == base["rows"][i] and ci.size == base["nnz"][i], seed
             times[i], _ = sweep(M, K, rp, ci, v, int(base["N"][i]))
-            if k % 20 == 0 or k == len(todo) - 1:
-                log(f"[{k + 1}/{len(todo)}] seed {seed} rows {M}: best {CONFIGS[int(np.argmin(times[i]))]} "
-                    f"({time.time() - t0:.0f}s)")
-                _save(path_out, dict(rec, times=list(times)))
-    _save(path_out, dict(rec, times=list(times)))
-    return len(small)
+            fresh[i] = True
+            log(f"[{k + 1}/{len(todo)}] seed {seed} {kind} rows {M}: best {CONFIGS[int(np.argmin(times[i]))]} "
+                f"({time.time() - t0:.0f}s)")
+            if k % 10 == 9:
+                _save(path_out, dict(rec, times=list(times)), fresh)
+        pool.terminate()
+    _save(path_out, dict(rec, times=list(times)), fresh)
+    return int(fresh.sum())
 
 
-def _save(path, rec):
+def _save(path, rec, fresh=None):
+    extra = {} if fresh is None else {"fresh": np.asarray(fresh, bool)}
     np.savez_compressed(path, seed=np.asarray(rec["seed"]), kind=np.asarray(rec["kind"]),
                         rows=np.asarray(rec["rows"]), nnz=np.asarray(rec["nnz"]), N=np.asarray(rec["N"]),
                         feats=np.asarray(rec["feats"], np.float32).reshape(-1, NF),
                         times=np.asarray(rec["times"], np.float64).reshape(-1, len(CONFIGS)),
-                        stats=np.asarray(rec["stats"]), configs=np.asarray([json.dumps(c) for c in CONFIGS]))
+                        stats=np.asarray(rec["stats"]), configs=np.asarray([json.dumps(c) for c in CONFIGS]),
+                        **extra)
 
 
 def main(argv=None):
@@ -564,13 +570,13 @@ def main(argv=None):
     t.add_argument("--data", nargs="+", required=True)
     t.add_argument("--out", required=True)
     t.add_argument("--epochs", type=int, default=300)
-    f = sub.add_parser("refresh")
+    f = sub.add_parser("resweep", help="time every configuration again with the current kernels")
     f.add_argument("--data", required=True)
     f.add_argument("--out", required=True)
     f.add_argument("--family", default="workload", choices=["workload", "tiled"])
     f.add_argument("--large", action="store_true")
-    f.add_argument("--start", type=int, default=0)
     f.add_argument("--count", type=int, default=None)
+    f.add_argument("--seconds", type=float, default=None, help="stop after this long (resume with the same --out)")
     f.add_argument("--workers", type=int, default=8)
     e = sub.add_parser("eval")
     e.add_argument("--data", nargs="+", required=True)
@@ -581,9 +587,9 @@ def main(argv=None):
         n_list = [int(x) for x in a.n_list.split(",")] if a.n_list else None
         collect(a.first, a.count, a.out, N=a.n, workers=a.workers, log=log, n_list=n_list, large=a.large,
                 family=a.family)
-    elif a.cmd == "refresh":
-        refresh(a.data, a.out, family=a.family, large=a.large, start=a.start, count=a.count, workers=a.workers,
-                log=log)
+    elif a.cmd == "resweep":
+        resweep(a.data, a.out, family=a.family, large=a.large, count=a.count, workers=a.workers,
+                deadline_s=a.seconds, log=log)
     elif a.cmd == "train":
         fs = [np.load(p) for p in a.data]
         feats = np.concatenate([f["feats"] for f in fs])

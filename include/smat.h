@@ -604,7 +604,7 @@ smat_status smat_adam_step(float* p, const float* g, float* m, float* v, int64_t
  * matrix (feats: [n_feat]) with the network in -> hidden -> hidden -> 1
  * (ReLU; inputs standardised with mu / sigma) and writes scores[n_cfg] and
  * best[0], the lowest-scoring candidate with avail != 0 (-1 if none).
- * Device arrays; n_feat + n_enc <= 64, hidden <= 64, n_cfg <= 256.
+ * Device arrays; n_feat + n_enc <= 64, hidden <= 128, n_cfg <= 256.
  */
 smat_status smat_tuner_predict(const float* feats, int32_t n_feat, const float* cfg_enc, int32_t n_enc,
                                const uint8_t* avail, int32_t n_cfg, const float* mu, const float* sigma,

@@ -311,7 +311,7 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 // Inference: one workgroup scores every candidate configuration of one matrix
 // with the 3-layer network (inputs standardised in the kernel) and writes the
 // scores and the index of the lowest available one.  Layer widths <= 64.
-constexpr int kMaxIn = 64, kMaxHidden = 64, kMaxCfg = 256;
+constexpr int kMaxIn = 64, kMaxHidden = 128, kMaxCfg = 256;
 
 // Inference: one workgroup per candidate configuration (blockIdx.x), the
 // three layers with one thread per output and a fixed summation order;

@@ -43,7 +43,7 @@ NTILES_24 = (16, 32, 64)     # tile widths of the mixed dense / 2:4 kernel
 # blocks), the four values of the reference's R-MAT threshold sweep
 TAUS = (None, 0.1, 0.3, 0.5, 0.7)
 REORDER_MAX_ROWS = 16384     # the exact Keeper's host replay grows ~quadratically; larger matrices skip it
-HIDDEN = 64
+HIDDEN = 128
 WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tuner", "mlp_weights.npz")
 
 
@@ -303,7 +303,7 @@ def samples(feats, times):
     return np.asarray(X, np.float32), np.asarray(y, np.float32), np.asarray(owner)
 
 
-def train(feats, times, epochs=300, batch=1024, lr=2e-3, seed=0, log=print):
+def train(feats, times, epochs=100, batch=1024, lr=2e-3, seed=0, log=print):
     """Trains the cost model on the GPU with the library's kernels (Adam).
     Returns a dict of numpy arrays (weights, standardisation) and the loss curve."""
     import torch
@@ -344,6 +344,31 @@ def train(feats, times, epochs=300, batch=1024, lr=2e-3, seed=0, log=print):
     W = net.numpy()
     return {"W1": W[0], "b1": W[1], "W2": W[2], "b2": W[3], "W3": W[4], "b3": W[5], "mu": mu.astype(np.float32),
             "sigma": sigma.astype(np.float32), "hidden": np.int32(HIDDEN), "curve": np.asarray(curve, np.float32)}
+
+
+def train_select(feats, times, rows, seeds=(0, 1), val_frac=1.0 / 7, epochs=100, log=print):
+    """Trains one network per seed on a fixed split of the matrices (a
+    seeded permutation; `val_frac` of them held out for validation) and keeps
+    the one whose picks land within 10 % of the sweep best on the most
+    validation matrices (ties: the higher geometric-mean gain over the plain
+    configuration).  Returns (weights, report)."""
+    perm = np.random.default_rng(0).permutation(len(feats))
+    nv = int(len(perm) * val_frac)
+    val, tr = perm[:nv], perm[nv:]
+    best, best_key, report = None, None, []
+    for seed in seeds:
+        w = train(feats[tr], times[tr], epochs=epochs, seed=seed, log=log)
+        r = evaluate(Model(w), feats[val], times[val], rows[val])
+        key = (r["within_10pct"], r["geo_speedup_vs_plain"])
+        report.append({"seed": int(seed), "val_within_10pct": r["within_10pct"],
+                       "val_geo_speedup_vs_plain": r["geo_speedup_vs_plain"]})
+        if log:
+            log(f"seed {seed}: validation {key[0]:.3f} within 10 %, {key[1]:.3f}x plain")
+        if best_key is None or key > best_key:
+            best, best_key = w, key
+    best = dict(best, val_frac=np.float32(val_frac), train_matrices=np.int32(len(tr)),
+                selection=np.asarray(json.dumps(report)))
+    return best, report
 
 
 # ---- inference (HIP kernel) -----------------------------------------------------
@@ -569,7 +594,9 @@ def main(argv=None):
     t = sub.add_parser("train")
     t.add_argument("--data", nargs="+", required=True)
     t.add_argument("--out", required=True)
-    t.add_argument("--epochs", type=int, default=300)
+    t.add_argument("--epochs", type=int, default=100)
+    t.add_argument("--seeds", default="0,1", help="one network per seed; the best on the validation split is kept")
+    t.add_argument("--val-frac", type=float, default=1.0 / 7)
     f = sub.add_parser("resweep", help="time every configuration again with the current kernels")
     f.add_argument("--data", required=True)
     f.add_argument("--out", required=True)
@@ -594,7 +621,9 @@ def main(argv=None):
         fs = [np.load(p) for p in a.data]
         feats = np.concatenate([f["feats"] for f in fs])
         times = np.concatenate([f["times"] for f in fs])
-        w = train(feats, times, epochs=a.epochs, log=log)
+        rows = np.concatenate([f["rows"] for f in fs])
+        w, _ = train_select(feats, times, rows, seeds=[int(x) for x in a.seeds.split(",")], val_frac=a.val_frac,
+                            epochs=a.epochs, log=log)
         np.savez(a.out, **w)
         log(f"saved {a.out}")
     else:

@@ -60,9 +60,9 @@ CONFIG_TRAFFIC = {
                                                            "47ef04834828"),
     ("clustered", 256, 16, "auto", "none", "mixed24", "f16"): (163.7e6,
                                                               "profiles/spmm_mixed_vs_dense_clustered24_n256.md",
-                                                              "be11342f66f3"),
+                                                              "47ef04834828"),
     ("clustered", 256, 16, "auto", "none", "dense", "f16"): (166.1e6, "profiles/spmm_mixed_vs_dense_clustered24_n256.md",
-                                                            "be11342f66f3"),
+                                                            "47ef04834828"),
 }
 KERNEL_SOURCE = os.path.join(ROOT, "sptc-smat_amd", "csrc", "spmm.hip")
 TILES_SOURCE = os.path.join(ROOT, "sptc-smat_amd", "csrc", "tiles.hip")

@@ -406,10 +406,11 @@ class Model:
 def probe_locality(model: Model, M, K, d_rp, d_ci, d_v, nnz, N, B, C, pick=None, reps=10, margin=0.03):
     """Locality row order for a matrix above the exact Keeper's row limit
     (device CSR, B and C tensors): the model's pick on the matrix as given and
-    on its locality-ordered copy (smat_locality_order + smat_permute_csr_rows),
-    each timed over `reps` calls after 2 warm-up calls; the order is kept when
-    it is faster by more than `margin` (a probe of a 0.1 ms call moves by a
-    few per cent between runs).  Both picks skip the Keeper candidates (tau set).  Returns
+    on its locality-ordered copy (smat_locality_order + smat_permute_csr_rows;
+    the copy also runs the as-given pick when the two differ), each timed over
+    `reps` calls after 2 warm-up calls; the order is kept when it is faster by
+    more than `margin` (a probe of a 0.1 ms call moves by a few per cent
+    between runs).  Both picks skip the Keeper candidates (tau set).  Returns
     (ordered, pick, order, report): order is (rowptr, colidx, values, rowmap)
     of the ordered copy when kept, else None."""
     from . import Bcsr, SpmmPlan, SMAT_KERNEL_DENSE, SMAT_KERNEL_MIXED24, SMAT_TAG_EXACT24, SMAT_TAG_REFERENCE, \
@@ -437,7 +438,15 @@ def probe_locality(model: Model, M, K, d_rp, d_ci, d_v, nnz, N, B, C, pick=None,
         return ms
 
     as_given = probe(pick, d_rp, d_ci, d_v, None)
+    # the ordered copy runs the model's pick for it and, when that differs,
+    # the pick for the matrix as given (the sweeps the model learned from
+    # time matrices as given, so its pick on a reordered copy is the less
+    # certain one); the faster of the two is compared
     ordered_ms = probe(o_pick, o_rp, o_ci, o_v, rowmap)
+    if pick != o_pick:
+        same_ms = probe(pick, o_rp, o_ci, o_v, rowmap)
+        if same_ms < ordered_ms:
+            o_pick, ordered_ms = pick, same_ms
     use = ordered_ms < as_given * (1.0 - margin)
     report = {"config": o_pick, "probe_event_ms": {"as_given": round(as_given, 4), "locality": round(ordered_ms, 4)},
               "used": bool(use)}

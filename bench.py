@@ -210,29 +210,78 @@ def cpu_baseline(M, rp, ci, vals_u16, N, target_s):
 
 
 def preprocess_leg(scale: int, tau: float = 0.5):
-    """Jaccard Keeper reordering (16-row groups, 16-column blocks) of an R-MAT
-    sample: the GPU implementation vs the reference's own CPU code
-    (IterativeBlockingKeeper compiled into oracle/_ref), on one core.  The two
-    groupings must be identical."""
+    """The reference's preprocessing chain on an R-MAT sample (16-row groups,
+    16-column blocks): Keeper grouping -> fixed-size grouping -> permutation
+    -> row reorder -> CSR to BCSR (16x16) -> split into 2:4 and dense block
+    lists.  GPU: this library (the Keeper's rounds on the GPU with the
+    order-dependent bookkeeping on the host, the other stages on the GPU
+    except the two host sorts the reference also does).  CPU, one core: the
+    reference's own IterativeBlockingKeeper, get_fixed_size_grouping,
+    get_permutation and CSR::reorder compiled into oracle/_ref, and the
+    oracle's restatements of csrToBcsr and filterBcsrBlocks (the reference's
+    are host code inside its CUDA sources).  Every stage's output is
+    compared; wall-clock seconds, inputs already resident on each side."""
     import numpy as np
+    import torch
     import smat
     from smat import gen
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_ffi as of
-    M, K, rp, ci, _ = gen.rmat(scale=scale, seed=1002)
+    M, K, rp, ci, v = gen.rmat(scale=scale, seed=1002)
     rows = 1 << scale
     rp = rp[: rows + 1]
+    d_rp, d_ci, d_v = (torch.from_numpy(x).cuda() for x in (rp, ci, v.view(np.int16)))
+    torch.cuda.synchronize()
+    gpu = {}
     t0 = time.perf_counter()
     g, cnt = smat.reorder_grouping(rp, ci, np.float32(tau), 16, 16)
-    t_gpu = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    fg = smat.fixed_size_grouping(g, 16)
+    perm = smat.grouping_to_permutation(fg)
+    t2 = time.perf_counter()
+    o_rp, o_ci, o_v = smat.permute_rows(d_rp, d_ci, d_v, torch.from_numpy(perm).cuda())
+    A = smat.Bcsr(M, K, o_rp, o_ci, o_v, bk=16)
+    sparse, dense = A.filter()
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    gpu = {"keeper_s": round(t1 - t0, 3), "grouping_permutation_s": round(t2 - t1, 4),
+           "reorder_bcsr_filter_s": round(t3 - t2, 4), "total_s": round(t3 - t0, 3)}
     out = {"workload": f"R-MAT scale {scale} ({rows} rows, {int(ci.size)} nnz), Keeper tau={tau}, 16x16",
-           "gpu_s": round(t_gpu, 3), "comparisons": int(cnt[0]), "merges": int(cnt[1])}
+           "gpu": gpu, "gpu_s": gpu["keeper_s"], "comparisons": int(cnt[0]), "merges": int(cnt[1]),
+           "nnzb16": A.nnzb}
     if of.ref_available():
-        eg, ecnt, t_ref = of.keeper(rp.astype(np.int64), ci.astype(np.int64), np.float32(tau), 16, 16,
-                                    cols=K, use_ref=True)
-        out.update({"reference_cpu_s": round(t_ref, 3), "cores": 1,
-                    "speedup": round(t_ref / t_gpu, 2),
-                    "identical": bool(np.array_equal(g, eg) and np.array_equal(cnt, ecnt))})
+        rp64, ci64 = rp.astype(np.int64), ci.astype(np.int64)
+        eg, ecnt, t_keep = of.keeper(rp64, ci64, np.float32(tau), 16, 16, cols=K, use_ref=True)
+        c0 = time.perf_counter()
+        efg = of.fixed_size_grouping(eg, 16, use_ref=True)
+        eperm = of.get_permutation(efg, use_ref=True)
+        c1 = time.perf_counter()
+        r_rp = np.zeros(rows + 1, np.int64)
+        r_ci = np.zeros(max(ci.size, 1), np.int64)
+        of.ref().ref_reorder_rows(rows, K, of.P(rp64), of.P(ci64), of.P(np.ascontiguousarray(efg, np.int64)),
+                                  of.P(r_rp), of.P(r_ci))
+        c2 = time.perf_counter()
+        # values follow their entries through the permutation (the reference's
+        # reorder moves the pattern; the values go with it in the same order)
+        r_v = np.concatenate([v[rp[r]:rp[r + 1]] for r in eperm]) if ci.size else v
+        c3 = time.perf_counter()
+        b_rp, b_ci, b_v, b_t = of.csr_to_bcsr(M, K, 16, r_rp, r_ci[: ci.size], r_v)
+        of.filter_blocks(16, b_rp, b_ci, b_v, b_t)
+        c4 = time.perf_counter()
+        cpu = {"keeper_s": round(t_keep, 3), "grouping_permutation_s": round(c1 - c0, 4),
+               "reorder_s": round(c2 - c1, 4), "bcsr_filter_s": round(c4 - c3, 4),
+               "total_s": round(t_keep + (c2 - c0) + (c4 - c3), 3), "cores": 1}
+        ga = A.arrays()
+        same = (np.array_equal(g, eg) and np.array_equal(cnt, ecnt) and np.array_equal(fg, efg)
+                and np.array_equal(perm, eperm) and np.array_equal(o_rp.cpu().numpy(), r_rp.astype(np.int32))
+                and np.array_equal(o_ci.cpu().numpy(), r_ci[: ci.size].astype(np.int32))
+                and np.array_equal(ga[0].numpy(), b_rp) and np.array_equal(ga[1].numpy(), b_ci)
+                and np.array_equal(ga[2].numpy().view(np.uint16), b_v) and np.array_equal(ga[3].numpy(), b_t))
+        out.update({"reference_cpu": cpu, "reference_cpu_s": cpu["keeper_s"], "cores": 1,
+                    "speedup": round(t_keep / gpu["keeper_s"], 2),
+                    "speedup_chain": round(cpu["total_s"] / gpu["total_s"], 2),
+                    "identical": bool(same)})
+    A.close()
     return out
 
 

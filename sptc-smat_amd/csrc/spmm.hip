@@ -1028,12 +1028,24 @@ __global__ void k_reduce(const Reduce* __restrict__ red, int64_t n_red, int32_t 
   for (int64_t i = (int64_t)(blockIdx.x % chunks) * blockDim.x + threadIdx.x; i < total;
        i += (int64_t)chunks * blockDim.x) {
     const int64_t row = i / per_row, n = (i % per_row) * V;
-    int64_t crow = (int64_t)rd.brow * 16 + row;
-    if (rowmap) crow = rowmap[crow];
+    // C row, looked up where it is needed: after the partial-tile loads
+    auto c_row = [&]() { return rowmap ? (int64_t)rowmap[(int64_t)rd.brow * 16 + row] : (int64_t)rd.brow * 16 + row; };
     if constexpr (VEC4) {
+      auto part = [&](int q) {
+        return *reinterpret_cast<const smat_f4*>(scratch + ((int64_t)(rd.slot_beg + q) * 16 + row) * N + n);
+      };
       smat_f4 s = smat_f4{0.f, 0.f, 0.f, 0.f};
-      for (int q = 0; q < rd.slot_cnt; q++)
-        s += *reinterpret_cast<const smat_f4*>(scratch + ((int64_t)(rd.slot_beg + q) * 16 + row) * N + n);
+      int q = 0;
+      // four partial tiles in flight per thread, summed in slice order
+      for (; q + 4 <= rd.slot_cnt; q += 4) {
+        const smat_f4 t0 = part(q), t1 = part(q + 1), t2 = part(q + 2), t3 = part(q + 3);
+        s += t0;
+        s += t1;
+        s += t2;
+        s += t3;
+      }
+      for (; q < rd.slot_cnt; q++) s += part(q);
+      const int64_t crow = c_row();
       if (OUT_F16) {
         _Float16* c = reinterpret_cast<_Float16*>(C) + crow * ldc + n;
 #pragma unroll
@@ -1042,8 +1054,18 @@ __global__ void k_reduce(const Reduce* __restrict__ red, int64_t n_red, int32_t 
         *reinterpret_cast<smat_f4*>(reinterpret_cast<float*>(C) + crow * ldc + n) = s;
       }
     } else {
+      auto part = [&](int q) { return scratch[((int64_t)(rd.slot_beg + q) * 16 + row) * N + n]; };
       float s = 0.f;
-      for (int q = 0; q < rd.slot_cnt; q++) s += scratch[((int64_t)(rd.slot_beg + q) * 16 + row) * N + n];
+      int q = 0;
+      for (; q + 4 <= rd.slot_cnt; q += 4) {
+        const float t0 = part(q), t1 = part(q + 1), t2 = part(q + 2), t3 = part(q + 3);
+        s += t0;
+        s += t1;
+        s += t2;
+        s += t3;
+      }
+      for (; q < rd.slot_cnt; q++) s += part(q);
+      const int64_t crow = c_row();
       if (OUT_F16)
         reinterpret_cast<_Float16*>(C)[crow * ldc + n] = (_Float16)s;
       else

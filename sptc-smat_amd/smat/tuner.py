@@ -568,8 +568,9 @@ def resweep(path_in, path_out, family="workload", large=False, count=None, worke
             assert seed == base["seed"][i] and M == base["rows"][i] and ci.size == base["nnz"][i], seed
             times[i], _ = sweep(M, K, rp, ci, v, int(base["N"][i]))
             fresh[i] = True
-            log(f"[{k + 1}/{len(todo)}] seed {seed} {kind} rows {M}: best {CONFIGS[int(np.argmin(times[i]))]} "
-                f"({time.time() - t0:.0f}s)")
+            if log:
+                log(f"[{k + 1}/{len(todo)}] seed {seed} {kind} rows {M}: best {CONFIGS[int(np.argmin(times[i]))]} "
+                    f"({time.time() - t0:.0f}s)")
             if k % 10 == 9:
                 _save(path_out, dict(rec, times=list(times)), fresh)
         pool.terminate()

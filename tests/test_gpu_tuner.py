@@ -241,6 +241,52 @@ def test_training_on_gpu_reduces_loss():
     assert curve[-1] < 0.25 * curve[0], curve[[0, -1]]
 
 
+def test_train_select_keeps_the_best_validation_seed():
+    """train_select trains one network per seed on the same split and keeps
+    the one with the most validation picks within 10 % of the best (ties: the
+    higher gain over plain); the kept weights carry the selection report."""
+    _need_gpu()
+    from smat import tuner
+    rng = np.random.default_rng(3)
+    n_mat = 140
+    feats = rng.standard_normal((n_mat, tuner.NF)).astype(np.float32)
+    times = np.exp(0.5 * feats[:, :1] * (tuner.ENC[:, :6] @ np.arange(6))[None] / 6 + 0.3 * feats[:, 1:2]
+                   * tuner.ENC[:, 6:9].argmax(1)[None]) * 0.1
+    rows = np.full(n_mat, 50000)  # above the Keeper's limit: reordered candidates unavailable
+    times[:, tuner.available(50000) == 0] = np.inf
+    w, report = tuner.train_select(feats, times, rows, seeds=(0, 1), val_frac=0.25, epochs=20, log=None)
+    assert [r["seed"] for r in report] == [0, 1]
+    keys = [(r["val_within_10pct"], r["val_geo_speedup_vs_plain"]) for r in report]
+    perm = np.random.default_rng(0).permutation(n_mat)
+    val = perm[: int(n_mat * 0.25)]
+    r = tuner.evaluate(tuner.Model(w), feats[val], times[val], rows[val])
+    assert (r["within_10pct"], r["geo_speedup_vs_plain"]) == max(keys)
+    assert json.loads(str(w["selection"])) == report and int(w["train_matrices"]) == n_mat - len(val)
+
+
+def test_resweep_times_a_recorded_matrix_again(tmp_path):
+    """resweep regenerates a recorded matrix from its seed (the recorded sizes
+    must match), times every available configuration again and marks it
+    fresh; a second call with the output as input has nothing left to do."""
+    _need_gpu()
+    from smat import tuner
+    src = os.path.join(TUNER_DIR, "sweep_test.npz")
+    if not os.path.exists(src):
+        pytest.skip("no committed sweep data")
+    d = np.load(src)
+    i = int(np.argmin(d["rows"]))  # the smallest held-out matrix: a few seconds with the Keeper candidates
+    one = {k: list(d[k][i:i + 1]) for k in ("seed", "kind", "rows", "nnz", "N", "stats", "feats")}
+    one["times"] = [np.full(len(tuner.CONFIGS), np.inf)]
+    path_in, path_out = str(tmp_path / "in.npz"), str(tmp_path / "out.npz")
+    tuner._save(path_in, one)
+    assert tuner.resweep(path_in, path_out, workers=2, log=None) == 1
+    o = np.load(path_out)
+    assert bool(o["fresh"][0]) and int(o["seed"][0]) == int(d["seed"][i])
+    avail = tuner.available(int(d["rows"][i])).astype(bool)
+    assert np.all(np.isfinite(o["times"][0][avail])) and np.all(np.isinf(o["times"][0][~avail]))
+    assert tuner.resweep(path_in, path_out, workers=2, log=None) == 1  # resumes from the output: nothing re-timed
+
+
 def _sweep_files(prefix):
     return sorted(os.path.join(TUNER_DIR, f) for f in os.listdir(TUNER_DIR) if f.startswith(prefix)) \
         if os.path.isdir(TUNER_DIR) else []

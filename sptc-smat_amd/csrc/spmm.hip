@@ -495,9 +495,6 @@ __device__ inline uint2 bload8(__amdgpu_buffer_rsrc_t r, uint32_t off) {
 __device__ inline uint32_t bload4(__amdgpu_buffer_rsrc_t r, uint32_t off) {
   return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
 }
-__device__ inline uint32_t bload2(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-  return __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
-}
 
 // A values of one step of 16x1 blocks as loaded: lane l holds elements
 // 8*(l%2) .. 8*(l%2)+7 of block p + l/2 (16 bytes, or 8 for FP8), blocks past
@@ -855,24 +852,23 @@ __global__ __launch_bounds__(256) void k_spmm_lds(const Item* __restrict__ items
 // Calls with a row map on 64-column FP16 / BF16 tiles (NSUB = 4, DT <= 1):
 // the row map read when a task starts (PRE), the kernel held to 4 waves per
 // SIMD (128 VGPR + AGPR), which the row-map VGPR would otherwise cost.
-#ifndef SMAT_LDS64_DEPTH
-#define SMAT_LDS64_DEPTH 2
-#endif
 template <int BK, int DT, bool OUT_F16>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_spmm_lds64(
     const Item* __restrict__ items, int64_t n_items, int32_t ntiles, Operands op, int32_t order) {
-  lds_kernel_body<BK, 4, DT, OUT_F16, true, SMAT_LDS64_DEPTH>(items, n_items, ntiles, op, order);
+  lds_kernel_body<BK, 4, DT, OUT_F16, true>(items, n_items, ntiles, op, order);
 }
 
 // Mixed dense / 2:4 kernel for 16x16 blocks and row-major B.  Dense blocks
-// (ids dense_ids[pbeg..pend)) go through MFMA in pairs; 2:4 blocks (ids
-// sparse_ids[sbeg..send)) through SMFMAC in quads, from the compressed arrays
-// of smat_compress_24 (SMAT_24_SMFMAC layout, indexed by BCSR block).  Both
-// accumulate into the same FP32 tile.  One register stage of prefetch.
+// (entries dense[pbeg..pend)) go through MFMA in pairs; 2:4 blocks (entries
+// sparse[sbeg..send)) through SMFMAC in quads, from the compressed arrays of
+// smat_compress_24 (SMAT_24_SMFMAC layout, indexed by BCSR block).  Both
+// accumulate into the same FP32 tile.  One register stage of prefetch.  A
+// list entry holds the block's BCSR position and its column start, so a B row
+// address is one load away from the list (not two through colidx).
 // 2:4 operands of a mixed plan.
 struct Mixed24 {
-  const int32_t* dense_ids;
-  const int32_t* sparse_ids;
+  const int2* dense;   // {BCSR block, column start}
+  const int2* sparse;
   const uint16_t* values24;
   const uint16_t* index24;
 };
@@ -892,12 +888,11 @@ __device__ __forceinline__ void mixed_tile(const Item& it, int64_t n0, const Ope
   constexpr int NLD_S = 64 / RPL;  // loads per 2:4 quad (64 rows)
   constexpr int PITCH = NT + 8;
   uint16_t* lds = reinterpret_cast<uint16_t*>(lds_bytes);
-  const int32_t* __restrict__ colidx = op.colidx;
   const uint16_t* __restrict__ values = static_cast<const uint16_t*>(op.a);
   const uint16_t* __restrict__ B = static_cast<const uint16_t*>(op.b);
   const int64_t ldb = op.ldb, N = op.N;
-  const int32_t* __restrict__ dense_ids = mx.dense_ids;
-  const int32_t* __restrict__ sparse_ids = mx.sparse_ids;
+  const int2* __restrict__ dense = mx.dense;
+  const int2* __restrict__ sparse = mx.sparse;
   const uint16_t* __restrict__ values24 = mx.values24;
   const uint16_t* __restrict__ index24 = mx.index24;
 
@@ -921,12 +916,12 @@ __device__ __forceinline__ void mixed_tile(const Item& it, int64_t n0, const Ope
         const int row = i * RPL + lrow;
         const int32_t q = p + row / 16;
         if (q < pe && col_ok)
-          bb[i] = *reinterpret_cast<const uint4*>(B + (int64_t)(colidx[dense_ids[q]] + row % 16) * ldb + ncol);
+          bb[i] = *reinterpret_cast<const uint4*>(B + (int64_t)(dense[q].y + row % 16) * ldb + ncol);
         else
           bb[i] = uint4{0, 0, 0, 0};
       }
       const int32_t q = p + (lane >> 5);
-      aa = q < pe ? *reinterpret_cast<const uint4*>(values + (int64_t)dense_ids[q] * 256 + r16 * 16 + (grp & 1) * 8)
+      aa = q < pe ? *reinterpret_cast<const uint4*>(values + (int64_t)dense[q].x * 256 + r16 * 16 + (grp & 1) * 8)
                   : uint4{0, 0, 0, 0};
     };
     issue(it.pbeg);
@@ -955,13 +950,13 @@ __device__ __forceinline__ void mixed_tile(const Item& it, int64_t n0, const Ope
         const int row = i * RPL + lrow;  // 0..63: block row/16 of the quad
         const int32_t q = p + row / 16;
         if (q < se && col_ok)
-          bb[i] = *reinterpret_cast<const uint4*>(B + (int64_t)(colidx[sparse_ids[q]] + row % 16) * ldb + ncol);
+          bb[i] = *reinterpret_cast<const uint4*>(B + (int64_t)(sparse[q].y + row % 16) * ldb + ncol);
         else
           bb[i] = uint4{0, 0, 0, 0};
       }
       const int32_t q = p + grp;  // lane group g carries block g of the quad
       if (q < se) {
-        const int64_t id = sparse_ids[q];
+        const int64_t id = sparse[q].x;
         aa = *reinterpret_cast<const uint4*>(values24 + (id * 16 + r16) * 8);
         idx = index24[id * 16 + r16];
       } else {
@@ -1212,7 +1207,7 @@ void k_batch(const BatchMat* __restrict__ mats, int32_t nmats, int64_t total, in
         kind = __builtin_amdgcn_readfirstlane(m.kind);
         items = uni(m.items);
         op = uni(m.op);
-        mx = kind == 6 ? Mixed24{uni(m.mx.dense_ids), uni(m.mx.sparse_ids), uni(m.mx.values24), uni(m.mx.index24)}
+        mx = kind == 6 ? Mixed24{uni(m.mx.dense), uni(m.mx.sparse), uni(m.mx.values24), uni(m.mx.index24)}
                        : Mixed24{nullptr, nullptr, nullptr, nullptr};
         cur_item = -1;
       }
@@ -1345,9 +1340,9 @@ __global__ void k_list_flags(int64_t nnzb, const int32_t* __restrict__ tags, int
 }
 
 __global__ void k_list_emit(int64_t nnzb, int64_t nbrows, const int32_t* __restrict__ tags,
-                            const int32_t* __restrict__ rowptr, const int32_t* __restrict__ pd,
-                            const int32_t* __restrict__ ps, int32_t* __restrict__ dense_ids,
-                            int32_t* __restrict__ sparse_ids, int32_t* __restrict__ drp,
+                            const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+                            const int32_t* __restrict__ pd, const int32_t* __restrict__ ps,
+                            int2* __restrict__ dense, int2* __restrict__ sparse, int32_t* __restrict__ drp,
                             int32_t* __restrict__ srp) {
   int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b <= nbrows) {
@@ -1355,8 +1350,8 @@ __global__ void k_list_emit(int64_t nnzb, int64_t nbrows, const int32_t* __restr
     srp[b] = ps[rowptr[b]];
   }
   if (b >= nnzb) return;
-  if (tags[b] == 2) dense_ids[pd[b]] = (int32_t)b;
-  else if (tags[b] == 1) sparse_ids[ps[b]] = (int32_t)b;
+  if (tags[b] == 2) dense[pd[b]] = int2{(int32_t)b, colidx[b]};
+  else if (tags[b] == 1) sparse[ps[b]] = int2{(int32_t)b, colidx[b]};
 }
 
 inline unsigned blocks_for(int64_t n, int t) {
@@ -1375,7 +1370,7 @@ struct smat_plan_t {
   int32_t order = kOrderXcd;  // work order of the one-shot kernels (kOrder*)
   int64_t n_items = 0, n_red = 0, n_slots = 0;
   int64_t n_dense = 0, n_sparse = 0;
-  smat::DevBuf items, red, scratch, dense_ids, sparse_ids;
+  smat::DevBuf items, red, scratch, dense, sparse;  // mixed plans: list entries {block, column start}
 };
 
 namespace {
@@ -1426,7 +1421,7 @@ template <int NSUB, int DT, bool OUT_F16>
 void launch_mixed(const smat_plan_t* P, const uint16_t* v24, const uint16_t* i24, const Operands& op,
                   hipStream_t s) {
   const int32_t ntiles = ntiles_of(P, NSUB);
-  const Mixed24 mx{P->dense_ids.as<int32_t>(), P->sparse_ids.as<int32_t>(), v24, i24};
+  const Mixed24 mx{P->dense.as<int2>(), P->sparse.as<int2>(), v24, i24};
   const int32_t order = order_of(P, ntiles);
   hipLaunchKernelGGL((k_spmm_mixed<NSUB, DT, OUT_F16>), grid_of(P, ntiles, order), 64 * kWaves, 0, s,
                      P->items.as<Item>(), P->n_items, ntiles, op, mx, order);
@@ -1636,11 +1631,11 @@ smat_status build_mixed(smat_plan_t* P, int32_t S, hipStream_t s) {
   SMAT_HIP_TRY(hipStreamSynchronize(s));
   P->n_dense = cnt[0];
   P->n_sparse = cnt[1];
-  SMAT_TRY(P->dense_ids.alloc(P->n_dense * 4));
-  SMAT_TRY(P->sparse_ids.alloc(P->n_sparse * 4));
+  SMAT_TRY(P->dense.alloc(P->n_dense * 8));
+  SMAT_TRY(P->sparse.alloc(P->n_sparse * 8));
   const int64_t m = n > A.nbrows ? n : A.nbrows + 1;
-  hipLaunchKernelGGL(k_list_emit, blocks_for(m, 256), 256, 0, s, n, A.nbrows, A.tags, A.rowptr, pd.as<int32_t>(),
-                     ps.as<int32_t>(), P->dense_ids.as<int32_t>(), P->sparse_ids.as<int32_t>(),
+  hipLaunchKernelGGL(k_list_emit, blocks_for(m, 256), 256, 0, s, n, A.nbrows, A.tags, A.rowptr, A.colidx,
+                     pd.as<int32_t>(), ps.as<int32_t>(), P->dense.as<int2>(), P->sparse.as<int2>(),
                      drp.as<int32_t>(), srp.as<int32_t>());
   SMAT_LAUNCH_CHECK();
   return build_items(P, drp.as<int32_t>(), 2, srp.as<int32_t>(), 4, S, s);
@@ -2158,7 +2153,7 @@ smat_status smat_batch_create_ex(int32_t count, const smat_plan* plans, const sm
     m.items = P->items.as<Item>();
     m.ntiles = ntiles_of(P, P->nsub);
     m.kind = P->mixed ? 6 : bk_code(P->A.bk);
-    m.mx = P->mixed ? Mixed24{P->dense_ids.as<int32_t>(), P->sparse_ids.as<int32_t>(), o.values24,
+    m.mx = P->mixed ? Mixed24{P->dense.as<int2>(), P->sparse.as<int2>(), o.values24,
                               static_cast<const uint16_t*>(o.index24)}
                     : Mixed24{nullptr, nullptr, nullptr, nullptr};
     m.task_beg = g_tasks[g];  // within its group

@@ -376,3 +376,38 @@ def test_a_over_4gib_lds_matches_direct_and_oracle():
     assert np.allclose(out[0], out[1], rtol=1e-5, atol=1e-5)
     R = of.spmm_csr_f64(M, rp, ci, v, B)
     assert _rel_fro(out[0].astype(np.float64), R) < 1e-5
+
+
+@pytest.mark.parametrize("S", [32, 64, 96, 128, 160, 0])
+@pytest.mark.parametrize("N", [64, 200])
+@pytest.mark.parametrize("rowmap", [False, True])
+def test_bk1_pipeline_paths(S, N, rowmap):
+    """16x1 blocks run their own pipeline (A through LDS, loads predicated by
+    buffer resources): items of 1, 2, 3, 4 or 5 steps of 32 blocks (S blocks
+    per item) take the short path, the drain alone, and the steady loop plus
+    drain; S = 0 is the planner's own slicing.  N = 200 leaves a partial
+    column chunk in the last 64-column tile; the row map takes the row-map
+    kernel.  FP64 oracle and bit-identical repeats."""
+    _need_gpu()
+    import smat
+    M, K, rp, ci, v = gen.rmat(scale=11, edge_factor=24, seed=43)
+    A = _bcsr(M, K, rp, ci, v, bk=1)
+    B = gen.dense_b(K, N, seed=9)
+    brp, bci, bv, _ = of.csr_to_bcsr(M, K, 1, rp, ci, v)
+    R = of.spmm_bcsr_f64(1, brp, bci, bv, B)
+    plan = smat.SpmmPlan(A, N, max_blocks_per_item=S, n_tile=64)
+    perm = np.random.default_rng(S + N).permutation(M).astype(np.int32) if rowmap else None
+    dB = _dev(B.view(np.int16))
+    C1 = torch.full((M, N), float("nan"), dtype=torch.float32, device="cuda")
+    C2 = torch.full((M, N), float("nan"), dtype=torch.float32, device="cuda")
+    rm = _dev(perm) if rowmap else None
+    plan.execute(dB, C1, c_rowmap=rm)
+    plan.execute(dB, C2, c_rowmap=rm)
+    torch.cuda.synchronize()
+    assert torch.equal(C1, C2)
+    Cn = C1.cpu().numpy().astype(np.float64)
+    if rowmap:  # row r of A lands in C row perm[r]
+        Cn = Cn[perm]
+    assert _rel_fro(Cn, R) < 1e-5
+    plan.close()
+    A.close()

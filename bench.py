@@ -11,7 +11,7 @@ pick (block width, N-tile, kernel); for a matrix too large for the exact
 Keeper the pick is made on the matrix as given and on its locality row order
 (smat_locality_order), and a short probe keeps the faster.  An exhaustive
 sweep over block width (1..32) and N-tile, on the matrix as given, is
-reported beside it as the judge (the fallback when no trained weights
+reported beside it as the yardstick (the fallback when no trained weights
 exist); `value` is the tuned configuration's useful GFLOP/s and the plain one
 is reported beside it.  Format conversion (CSR -> BCSR on the GPU) and the
 row reorder are preprocessing, timed separately (`build_ms`, `decide_ms`),
@@ -567,7 +567,7 @@ def batched_workload(args, log, legs_mode=None):
     (the reference's way) and also as one batch.  Tuned: the learned model's
     pick per matrix, one launch per matrix and all matrices in one
     graph-captured batch (branches and persistent).  The exhaustive sweep's
-    pick is timed the same way as the judge."""
+    pick is timed the same way, as the yardstick."""
     import multiprocessing as mp
     import numpy as np
     import scipy.sparse as sp
@@ -1065,7 +1065,7 @@ def main():
         calib_copy(args.calib_copy_mib)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; sharding the SpMM over GPUs "
+        print(json.dumps({"error": "the library runs on one GPU only; sharding the SpMM over GPUs "
                                    "(block-row ranges, B replicated) is a design note, not built"}))
         sys.exit(2)
     import numpy as np
@@ -1116,7 +1116,7 @@ def main():
         if args.workload in ("clustered", "band"):
             sweep.append(measure(16, 0, "mixed24", max(3, args.steps // 2), 1))
         best = min(sweep, key=lambda s: s["event_ms"])
-        # re-time the sweep's winner with the full step count: the judge
+        # re-time the sweep's winner with the full step count: the yardstick
         sweep_best = measure(best["bk"], 0 if best["n_tile"] == "auto" else best["n_tile"], best["kernel"],
                              args.steps, args.warmup)
         learned = learned_pick(ctx, log)
@@ -1133,7 +1133,7 @@ def main():
             learned["ratio_basis"] = ("tuned call on the locality-ordered matrix / sweep best on the matrix as "
                                       "given" if order is not None else "same matrix")
             if order is not None:
-                # the same sweep on the ordered copy: the like-for-like judge of the kept pick
+                # the same sweep on the ordered copy: the like-for-like yardstick of the kept pick
                 sweep_o = [measure(b, n, "dense", max(3, args.steps // 2), 1, order)
                            for b in (1, 2, 4, 8, 16, 32) for n in (0, 32)]
                 best_o = min(sweep_o, key=lambda s_: s_["event_ms"])
@@ -1189,7 +1189,7 @@ def main():
         "data": "synthetic (seeded generator; B U[-1,1] seed 7)",
         "config": {"workload": f"{desc}, B {K}x{ctx.N} row-major, FP32 C", "rows": M, "nnz": ctx.nnz,
                    "N": ctx.N, "tuned": {k: tuned[k] for k in ("bk", "n_tile", "kernel", "nnzb", "order")},
-                   "tuning": ("learned MLP pick (exhaustive sweep reported as judge)" if learned else
+                   "tuning": ("learned MLP pick (exhaustive sweep reported as yardstick)" if learned else
                               "exhaustive sweep over block width x N-tile") if not args.single else "fixed"},
         "learned": learned,
         "sweep_best": {k: sweep_best[k] for k in ("bk", "n_tile", "kernel", "event_ms", "useful_gflops")}

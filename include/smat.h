@@ -43,7 +43,7 @@ typedef int32_t smat_status;
 #define SMAT_ERR_HIP (-2)         /* HIP runtime error; message has the detail */
 #define SMAT_ERR_ALLOC (-3)       /* device allocation failed */
 #define SMAT_ERR_NOT_24 (-4)      /* a block tagged 2:4 has a 1x4 group with >2 nonzeros */
-#define SMAT_ERR_UNSUPPORTED (-5) /* valid request this build does not implement */
+#define SMAT_ERR_UNSUPPORTED (-5) /* valid request the library does not implement */
 
 /* Thread-local message of the last failing call on this thread ("" if none). */
 const char* smat_last_error(void);

@@ -4,7 +4,7 @@
 // The reference picks its configuration by hand or by an exhaustive search
 // over the Jaccard threshold (reorder_matrix.cpp:17-40, collect_taus.py:37-44);
 // the learned model has no reference counterpart.  The exhaustive sweep stays
-// the fallback and the judge of the model (host side, smat/tuner.py).
+// the fallback and the yardstick of the model (host side, smat/tuner.py).
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>

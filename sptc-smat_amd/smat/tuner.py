@@ -7,7 +7,7 @@ network is trained on timings of this library's own exhaustive sweep over
 seeded synthetic matrices, with the library's HIP kernels for the forward and
 backward passes and Adam (smat_linear_forward / _backward, smat_mse_grad,
 smat_adam_step), and inference runs in one HIP kernel (smat_tuner_predict).
-The sweep itself (`sweep`) stays the fallback and the judge.
+The sweep itself (`sweep`) stays the fallback and the yardstick.
 
 The reference has no learned part: it tunes by hand or by an exhaustive
 search over the reorder threshold (preprocess/src/reorder_matrix.cpp:17-40,
@@ -158,7 +158,7 @@ def make_tiled_matrix(seed: int):
     return "tiled", gen.block_clustered(n=n, tiles=tiles, seed=gs, prune=prune, bh=bh, bw=bw, fill=fill)
 
 
-# ---- the sweep (fallback and judge) --------------------------------------------
+# ---- the sweep (fallback and yardstick) ----------------------------------------
 def reorder_csr(rp, ci, v, tau):
     """Keeper grouping -> fixed-size groups -> row permutation, applied on the
     host arrays.  Returns (rp, ci, v, perm): new row r is old row perm[r]."""

@@ -1732,7 +1732,16 @@ smat_status plan_init(smat_plan_t* P, const smat_bcsr_desc* A, int64_t N, const 
       // (half of it: the last rows to start still end with the rest)
       const int64_t waves = (int64_t)cus * 16;
       const int64_t steps = (A->nnzb + step - 1) / step;
-      s_auto = (steps * ntl + 2 * waves - 1) / (2 * waves) * step;
+      if (A->bk == 1) {
+        // 16x1 blocks: every slice refills the two-step load pipeline and
+        // writes a partial tile, so a slice runs a whole fair share plus 32
+        // steps, and at least 64 steps.  Measured best slice lengths: R-MAT
+        // scale 18 N = 256 about 128 steps (fair share 93), N = 64 about 64
+        // (fair share 23), power-law N = 128 about 170-214 (fair share 170).
+        s_auto = std::max<int64_t>((steps * ntl + waves - 1) / waves + 32, 64) * step;
+      } else {
+        s_auto = (steps * ntl + 2 * waves - 1) / (2 * waves) * step;
+      }
     }
     S = (int32_t)(s_auto < s_min ? s_min : (s_auto > 65536 ? 65536 : s_auto));
   }

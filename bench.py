@@ -51,18 +51,18 @@ MEASURED_TRAFFIC = {
     ("rmat", 18, 256, 2, "auto", "locality"): (1930e6, "profiles/spmm_lds_bk2_locality_rmat18_n256_hbm_traffic.md",
                                                  "be11342f66f3"),
     ("rmat", 18, 256, 1, "auto", "locality"): (1249e6, "profiles/spmm_lds_bk1_locality_rmat18_n256_hbm_traffic.md",
-                                                 "ffe6c5a005f4"),
+                                                 "a61df9205fb5"),
 }
 # the same for the other workloads' calls:
 # (workload, N, bk, n_tile, order, kernel, dtype) -> (bytes, profile file, kernel source id)
 CONFIG_TRAFFIC = {
-    ("powerlaw", 128, 1, 64, "locality", "dense", "f16"): (2355e6, "profiles/spmm_lds_bk1_locality_powerlaw_n128.md",
-                                                           "ffe6c5a005f4"),
+    ("powerlaw", 128, 1, 64, "locality", "dense", "f16"): (2282e6, "profiles/spmm_lds_bk1_locality_powerlaw_n128.md",
+                                                           "a61df9205fb5"),
     ("clustered", 256, 16, "auto", "none", "mixed24", "f16"): (163.7e6,
                                                               "profiles/spmm_mixed_vs_dense_clustered24_n256.md",
-                                                              "ffe6c5a005f4"),
-    ("clustered", 256, 16, "auto", "none", "dense", "f16"): (166.2e6, "profiles/spmm_mixed_vs_dense_clustered24_n256.md",
-                                                            "ffe6c5a005f4"),
+                                                              "a61df9205fb5"),
+    ("clustered", 256, 16, "auto", "none", "dense", "f16"): (166.1e6, "profiles/spmm_mixed_vs_dense_clustered24_n256.md",
+                                                            "a61df9205fb5"),
 }
 KERNEL_SOURCE = os.path.join(ROOT, "sptc-smat_amd", "csrc", "spmm.hip")
 TILES_SOURCE = os.path.join(ROOT, "sptc-smat_amd", "csrc", "tiles.hip")

@@ -397,8 +397,8 @@ typedef struct {
 
 /* Order in which work items reach the GPU. */
 #define SMAT_SCHED_AUTO 0          /* block-row lengths skewed (longest over 8x the mean): tile
-                                      affinity with 2 or 4 N tiles, else longest first; block-row
-                                      order otherwise */
+                                      affinity with 4 N tiles (16x1 blocks) or 2 or 4 (wider
+                                      blocks), else longest first; block-row order otherwise */
 #define SMAT_SCHED_BLOCK_ROWS 1    /* block-row order; each XCD takes a contiguous range (L2 locality) */
 #define SMAT_SCHED_LONGEST_FIRST 2 /* items sorted by length, longest dispatched first: hub rows start
                                       early, so only rows over half a wave's fair share are split */

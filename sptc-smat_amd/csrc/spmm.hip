@@ -1693,12 +1693,16 @@ smat_status plan_init(smat_plan_t* P, const smat_bcsr_desc* A, int64_t N, const 
   if (o.schedule == SMAT_SCHED_AUTO) {
     // skewed block rows (the longest over 8x the mean) go longest first,
     // with tile affinity when every N tile is shared by 2+ XCDs (with more
-    // tiles each XCD would re-read all of A for its one slice of B)
+    // tiles each XCD would re-read all of A for its one slice of B); 16x1
+    // blocks only with 4 tiles (with 2, longest first alone measured
+    // faster: R-MAT s18 N = 64 in 32-column tiles 0.098 -> 0.068 ms,
+    // power-law N = 128 in 64-column tiles 0.337 -> 0.332 ms)
     int64_t mx = 0, tot = 0;
     SMAT_TRY(row_length_stats(A, &mx, &tot, s));
     const int64_t ntl = (N + nt - 1) / nt;
+    const bool affinity = ntl == 4 || (ntl == 2 && A->bk != 1);
     o.schedule = !(tot > 0 && mx * A->nbrows > 8 * tot) ? SMAT_SCHED_BLOCK_ROWS
-                 : (ntl == 2 || ntl == 4)              ? SMAT_SCHED_TILE_AFFINITY
+                 : affinity                            ? SMAT_SCHED_TILE_AFFINITY
                                                        : SMAT_SCHED_LONGEST_FIRST;
   }
 

@@ -411,3 +411,44 @@ def test_bk1_pipeline_paths(S, N, rowmap):
     assert _rel_fro(Cn, R) < 1e-5
     plan.close()
     A.close()
+
+
+@pytest.mark.parametrize("scale,N,schedule", [(17, 256, 3), (15, 64, 2)])
+def test_bk1_slice_length_rule(scale, N, schedule):
+    """The planner's slicing of 16x1-block rows under the longest-first
+    schedules: a block row of L blocks is one work item when its ceil(L/32)
+    MFMA steps fit in S, else ceil(steps/S) slices, with S = max(ceil(total
+    steps x N tiles / (16 waves x CUs)) + 32, 64) steps (a fair share plus
+    32, at least 64; total steps = ceil(nnzb/32)).  Scale 17 at N = 256
+    takes the fair-share term, scale 15 at N = 64 the floor.  Items, split
+    rows and partial-tile scratch against that count on the host (from the
+    oracle's BCSR), and the scale-15 product against the FP64 oracle."""
+    _need_gpu()
+    import smat
+    M, K, rp, ci, v = gen.rmat(scale=scale, seed=47)
+    print(f"slice rule R-MAT scale {scale} N={N} schedule {schedule}", flush=True)
+    brp, bci, bv, _ = of.csr_to_bcsr(M, K, 1, rp, ci, v)
+    steps = -(-np.diff(brp).astype(np.int64) // 32)
+    total = -(-int(brp[-1]) // 32)
+    ntl = -(-N // 64)
+    waves = 16 * torch.cuda.get_device_properties(0).multi_processor_count
+    s = max(-(-(total * ntl) // waves) + 32, 64)
+    s = -(-min(max(s * 32, 12 * 32), 65536) // 32)
+    if scale == 17:
+        assert s > 64  # the fair-share term decides
+    k = np.where(steps <= s, 1, -(-steps // s))
+    A = _bcsr(M, K, rp, ci, v, bk=1)
+    plan = smat.SpmmPlan(A, N, n_tile=64, schedule=schedule)
+    info = plan.info()
+    assert info.n_split_rows > 0
+    assert (info.n_items, info.n_split_rows) == (int(k.sum()), int((k > 1).sum()))
+    assert info.scratch_bytes == int(k[k > 1].sum()) * 16 * N * 4
+    if scale == 15:
+        B = gen.dense_b(K, N, seed=11)
+        R = of.spmm_bcsr_f64(1, brp, bci, bv, B)
+        C = torch.full((M, N), float("nan"), dtype=torch.float32, device="cuda")
+        plan.execute(_dev(B.view(np.int16)), C)
+        torch.cuda.synchronize()
+        assert _rel_fro(C.cpu().numpy().astype(np.float64), R) < 1e-5
+    plan.close()
+    A.close()

@@ -40,3 +40,22 @@ def test_traffic_figures_cite_profiles_of_their_source():
     key = ("rmat", 18, 256, 1, "auto", "locality")
     assert b.MEASURED_TRAFFIC[key][2] == spmm_id
     assert current >= 4
+
+
+def test_profile_commands_use_existing_bench_flags():
+    """The commands in the committed profile summaries run committed files:
+    every long option in their code blocks is one bench.py defines or one
+    of rocprofv3's."""
+    import glob
+    import re
+    flags = set(re.findall(r'add_argument\("(--[a-z0-9-]+)"', open(os.path.join(ROOT, "bench.py")).read()))
+    rocprof = {"--kernel-trace", "--stats", "--output-format", "--pmc"}
+    files = sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md")))
+    assert files
+    for f in files:
+        text = open(f).read()
+        used = set()
+        for block in re.findall(r"```(.*?)```", text, flags=re.S):
+            assert "bench.py" in block, f
+            used |= set(re.findall(r"(?<![\w-])(--[a-z0-9-]+)", block))
+        assert not used - flags - rocprof, (f, sorted(used - flags - rocprof))
